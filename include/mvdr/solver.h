@@ -1,0 +1,210 @@
+/*
+ * C ABI of the MI355X non-rigid warp-field solver.
+ *
+ * Drop-in boundary for SparseSurfelFusion::NonRigidSolver
+ * (core/NonRigidSolver/NonRigidSolver.h:78-541).  Entry points map onto the
+ * reference class as follows:
+ *
+ *   mvdr_solver_create          NonRigidSolver::NonRigidSolver(Intrinsic*)
+ *                               + AllocateBuffer()      (NonRigidSolver.cpp:10-48)
+ *   mvdr_solver_destroy         ReleaseBuffer()         (NonRigidSolver.cpp:50-74)
+ *   mvdr_solver_set_input       SetSolverInput(...)     (NonRigidSolver.h:133-140,
+ *                                                        NonRigidSolver.cpp:77-97)
+ *   mvdr_solver_solve           SolveNonRigidAlignment() (NonRigidSolver.h:254)
+ *   mvdr_solver_build_index     buildSolverIndexStreamed()          (NonRigidSolver.h:256)
+ *   mvdr_solver_gn_iteration    solverIteration{Global,Local}IterationStreamed()
+ *                                                       (NonRigidSolver.h:257-258)
+ *   mvdr_solver_get_node_se3    SolvedNodeSE3()         (NonRigidSolver.h:214)
+ *   mvdr_solver_total_residual  ComputeTotalResidualSynced() (NonRigidSolver.h:306)
+ *   mvdr_solver_debug_get       JtResidualValue(), JtJDiagonalBlockValue()
+ *                               (NonRigidSolver.h:329-330) and the other
+ *                               per-stage buffers, for parity checks.
+ *
+ * Conventions
+ *   - All map / array pointers in mvdr_frame_input are DEVICE pointers owned
+ *     by the caller; they must stay valid until the solve that uses them has
+ *     completed on its stream (same non-owning rule as the reference's
+ *     DeviceArrayView / texture inputs).
+ *   - Per-view maps are dense row-major images of width*height elements
+ *     (no pitch), element types as documented per field.
+ *   - Streams are hipStream_t passed as void* (NULL = the null stream).
+ *   - No exception crosses the ABI; every call returns an mvdr_status, and
+ *     mvdr_solver_last_error() gives the message of the last failure.
+ *   - A handle is driven by one host thread at a time (the reference class is
+ *     not thread-safe either).
+ */
+#ifndef MVDR_SOLVER_H
+#define MVDR_SOLVER_H
+
+#include "types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mvdr_solver mvdr_solver;
+
+typedef enum {
+    MVDR_OK = 0,
+    MVDR_E_INVALID = 1,   /* null pointer / malformed argument              */
+    MVDR_E_SHAPE = 2,     /* views / image / node counts outside the config */
+    MVDR_E_CAPACITY = 3,  /* a data-dependent size exceeded its capacity    */
+    MVDR_E_HIP = 4,       /* HIP runtime failure                            */
+    MVDR_E_STATE = 5      /* call order, e.g. solve before set_input        */
+} mvdr_status;
+
+typedef enum {
+    MVDR_PRECISION_FP32 = 0,   /* everything FP32 (reference precision)          */
+    MVDR_PRECISION_MIXED = 1   /* FP64 dots / alpha / beta / 6x6 inverses        */
+} mvdr_precision;
+
+typedef enum {
+    MVDR_JTJ_MATERIALIZED = 0, /* USE_MATERIALIZED_JTJ (solver_constants.h:12)   */
+    MVDR_JTJ_MATRIX_FREE = 1   /* J^T(J x) with per-term adjoints                */
+} mvdr_jtj_mode;
+
+typedef struct {
+    /* capacities (buffers are allocated once, at create) */
+    uint32_t max_views;            /* <= MVDR_MAX_VIEWS                               */
+    uint32_t max_width;
+    uint32_t max_height;
+    uint32_t max_nodes;            /* <= 65535 (u16 node ids)                          */
+    uint32_t max_graph_edges;      /* node graph pairs, normally 8 * max_nodes         */
+    uint32_t max_feature_pairs;    /* sum over views of candidate feature pairs        */
+    uint32_t max_cross_pairs;      /* candidate cross-view pairs                       */
+    /* schedule (values of the absent Constants::*; base/Constants.h:30-62) */
+    uint32_t gn_iterations;        /* kNumGaussNewtonIterations                        */
+    uint32_t pcg_iterations;       /* SolvePCGMaterialized(pcg_iterations)             */
+    uint32_t global_iterations;    /* kNumGlobalSolverItarations                       */
+    uint32_t use_elastic_penalty;  /* kUseElasticPenalty                               */
+    /* term switches (ConfigParser::Use*Term, base/ConfigParser.h:43-53) */
+    uint32_t use_density;          /* density penalty in local iterations              */
+    uint32_t use_foreground;       /* foreground-mask term                             */
+    uint32_t use_cross_view;       /* cross-view correspondence term                   */
+    uint32_t density_in_jtj;       /* USE_DENSE_IMAGE_DENSITY_TERM                     */
+    /* numerics / execution */
+    uint32_t precision;            /* mvdr_precision                                   */
+    uint32_t jtj_mode;             /* mvdr_jtj_mode                                    */
+    uint32_t refine_steps;         /* residual refinement steps after each PCG (mixed) */
+    uint32_t use_graph;            /* capture the GN iterations into one hipGraph      */
+} mvdr_solver_config;
+
+typedef struct {
+    uint32_t num_views;
+    uint32_t width;
+    uint32_t height;
+    mvdr_intrinsic intrinsic[MVDR_MAX_VIEWS];          /* ClipColorIntrinsic[v]       */
+    mvdr_mat34 world2camera[MVDR_MAX_VIEWS];           /* canonical2Live[v]           */
+    mvdr_mat34 initial_camera_se3[MVDR_MAX_VIEWS];     /* camera v -> camera 0        */
+
+    /* CameraObservation (base/CameraObservation.h:16-63) */
+    const void* vertex_confid[MVDR_MAX_VIEWS];         /* float4  vertexConfidenceMap */
+    const void* normal_radius[MVDR_MAX_VIEWS];         /* float4  normalRadiusMap     */
+    const void* filtered_foreground[MVDR_MAX_VIEWS];   /* float   filteredForegroundMask */
+    const void* foreground_gradient[MVDR_MAX_VIEWS];   /* float2  foregroundMaskGradientMap */
+    const void* gray[MVDR_MAX_VIEWS];                  /* float   grayScaleMap        */
+    const void* gray_gradient[MVDR_MAX_VIEWS];         /* float2  grayScaleGradientMap */
+    const void* edge_mask[MVDR_MAX_VIEWS];             /* uint8   edgeMaskMap         */
+
+    /* Renderer::SolverMaps (render/Renderer.h:381-388) */
+    const void* reference_vertex[MVDR_MAX_VIEWS];      /* float4 */
+    const void* reference_normal[MVDR_MAX_VIEWS];      /* float4 */
+    const void* warp_vertex[MVDR_MAX_VIEWS];           /* float4 */
+    const void* index_map[MVDR_MAX_VIEWS];             /* uint32, 0xFFFFFFFF = empty  */
+    const void* normalized_rgb[MVDR_MAX_VIEWS];        /* float4 */
+
+    /* SurfelGeometry::NonRigidSolverInput (core/Geometry/SurfelGeometry.h:179-182) */
+    const void* surfel_knn;                            /* ushort4 */
+    const void* surfel_knn_weight;                     /* float4  */
+    uint32_t num_surfels;
+
+    /* WarpField::NonRigidSolverInput (core/NonRigidSolver/WarpField.h:157-161) */
+    const void* node_se3;                              /* mvdr_dq, the initial warp   */
+    const void* canonical_nodes;                       /* float4  */
+    const void* node_graph;                            /* ushort2 */
+    uint32_t num_nodes;
+    uint32_t num_graph_edges;
+
+    /* correspondences */
+    const void* feature_pairs[MVDR_MAX_VIEWS];         /* ushort4 correspondencePixelPairs[v] */
+    uint32_t num_feature_pairs[MVDR_MAX_VIEWS];
+    const void* cross_pairs;                           /* mvdr_cross_pair crossCorrPairs */
+    uint32_t num_cross_pairs;
+} mvdr_frame_input;
+
+/* Sizes the index build found (valid after build_index / solve). */
+typedef struct {
+    uint32_t num_dense_terms;       /* T_d  */
+    uint32_t num_smooth_terms;      /* E    */
+    uint32_t num_foreground_terms;  /* T_fg */
+    uint32_t num_feature_terms;     /* T_f  */
+    uint32_t num_cross_terms;       /* T_c  */
+    uint32_t num_node2term;         /* KV1  */
+    uint32_t num_nodepair_terms;    /* valid KV2 entries */
+    uint32_t num_node_pairs;        /* P (unique i<j pairs) */
+    uint32_t num_nodes;
+    uint32_t gn_iterations_done;
+} mvdr_solver_sizes;
+
+typedef enum {
+    MVDR_TAP_NODE2TERM_OFFSET = 0,   /* u32[N+1]                                        */
+    MVDR_TAP_NODE2TERM_INDEX = 1,    /* u32[KV1]                                        */
+    MVDR_TAP_NODEPAIR_KEY = 2,       /* u32[P]  (i << 16) | j, i < j, ascending          */
+    MVDR_TAP_NODEPAIR_RANGE = 3,     /* u32[2P] [start, end) into NODEPAIR_TERM_INDEX    */
+    MVDR_TAP_NODEPAIR_TERM_INDEX = 4,/* u32[valid KV2]                                   */
+    MVDR_TAP_DENSE_TERMS = 5,        /* 16 x u32/f32 per dense term (layout in DESIGN)   */
+    MVDR_TAP_SMOOTH_TERMS = 6,       /* 8 x u32/f32 per edge                             */
+    MVDR_TAP_FEATURE_TERMS = 7,      /* 16 x u32/f32 per feature term                    */
+    MVDR_TAP_CROSS_TERMS = 8,        /* 16 x u32/f32 per cross term                      */
+    MVDR_TAP_FOREGROUND_TERMS = 9,   /* 16 x u32/f32 per foreground term                 */
+    MVDR_TAP_DENSE_PIXELS = 10,      /* u16[4] (x, y, view, 0) per dense term            */
+    MVDR_TAP_JTJ_DIAG = 11,          /* f32[36N] row-major, damping included             */
+    MVDR_TAP_JTJ_DIAG_INV = 12,      /* f32[36N] row-major                               */
+    MVDR_TAP_JTR = 13,               /* f32[6N]  b = -sum lambda^2 w J^T r               */
+    MVDR_TAP_JTJ_OFFDIAG = 14,       /* f32[36P] row-major block (i, j), i < j           */
+    MVDR_TAP_TWIST = 15,             /* f32[6N]  last PCG solution                       */
+    MVDR_TAP_NODE_SE3 = 16,          /* mvdr_dq[N] current warp field                    */
+    MVDR_TAP_PENALTY = 17,           /* f32[5] lambda: smooth, density, foreground, feature, cross */
+    MVDR_TAP_DENSITY_TERMS = 18      /* 16 x u32/f32 per dense term (density channel)     */
+} mvdr_tap;
+
+const char* mvdr_status_string(mvdr_status status);
+
+/* Fill a config with the defaults used by the reference build. */
+void mvdr_solver_default_config(mvdr_solver_config* config);
+
+mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** solver);
+void mvdr_solver_destroy(mvdr_solver* solver);
+const char* mvdr_solver_last_error(const mvdr_solver* solver);
+
+/* SetSolverInput: validates shapes against the config, stores the views,
+ * resets the warp field to input->node_se3 and the GN counter to zero. */
+mvdr_status mvdr_solver_set_input(mvdr_solver* solver, const mvdr_frame_input* input);
+
+/* SolveNonRigidAlignment: index build + gn_iterations GN steps on `stream`. */
+mvdr_status mvdr_solver_solve(mvdr_solver* solver, void* stream);
+
+/* The two halves of the solve, for stage-wise parity checks. */
+mvdr_status mvdr_solver_build_index(mvdr_solver* solver, void* stream);
+mvdr_status mvdr_solver_gn_iteration(mvdr_solver* solver, void* stream);
+
+/* SolvedNodeSE3: device view of the current warp field (solver-owned; valid
+ * until the next set_input / solve / gn_iteration on this handle). */
+mvdr_status mvdr_solver_get_node_se3(mvdr_solver* solver, const void** dq_device, uint32_t* num_nodes);
+
+/* ComputeTotalResidualSynced: sum of squared weighted residuals at the
+ * current warp field, with the current penalty constants.  Synchronises. */
+mvdr_status mvdr_solver_total_residual(mvdr_solver* solver, void* stream, double* cost);
+
+mvdr_status mvdr_solver_get_sizes(mvdr_solver* solver, mvdr_solver_sizes* sizes);
+
+/* Copy a stage buffer to host memory (synchronises).  If dst is NULL or
+ * bytes is too small, *needed receives the required byte count and the call
+ * returns MVDR_E_CAPACITY without copying (needed may be NULL). */
+mvdr_status mvdr_solver_debug_get(mvdr_solver* solver, int tap, void* dst, size_t bytes, size_t* needed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MVDR_SOLVER_H */

@@ -1,0 +1,443 @@
+// One Gauss-Newton iteration on the device: term residuals / Jacobians,
+// J^T J diagonal + J^T r + block-Jacobi inverse per node, J^T J off-diagonal
+// blocks per node pair, block-sparse PCG and the warp-field update.
+#include "common.h"
+#include "device_util.h"
+
+namespace mvdr {
+
+// ---------------------------------------------------------- term evaluation
+// computeDenseDepthJacobianKernel (DenseDepthJacobianResidual.cu:24-104).  The
+// camera of a term is the view stored with its compacted pixel.
+__device__ inline void eval_dense(const FrameDev& f, const DQ* se3, ushort4 px, ScalarRec& rec) {
+    const uint32_t v = px.z;
+    const uint32_t o = px.y * f.W + px.x;
+    const f3 cv = xyz(f.ref_v[v][o]), cn = xyz(f.ref_n[v][o]);
+    const Mat34& I = f.init[v];
+    const f3 v0 = I.apply(cv), n0 = I.rot(cn);
+    const Mat34 T = se3_of(blend4(se3, rec.knn, rec.w));
+    const f3 pl = T.apply(v0), nl = T.rot(n0);
+    f3 pc = f.w2c[v].apply(pl), nc = f.w2c[v].rot(nl);
+    pc = f.init_inv[v].apply(pc);
+    nc = f.init_inv[v].rot(nc);
+    float r = 0.f, J[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const int u = project_round(pc.x, pc.z, f.fx[v], f.cx[v]);
+    const int w = project_round(pc.y, pc.z, f.fy[v], f.cy[v]);
+    if (u >= 0 && u < (int)f.W && w >= 0 && w < (int)f.H) {
+        const float4 dv = f.vertex[v][w * f.W + u];
+        const float4 dn = f.normal[v][w * f.W + u];
+        const f3 d = xyz(dv), n = xyz(dn);
+        bool valid = !is_zero_vertex(dv);
+        if (dot3(n, nc) < 0.7f) valid = false;
+        const f3 diff = d - pc;
+        if (dot3(diff, diff) > 0.075f * 0.075f) valid = false;
+        if (valid) {
+            r = dot3(n, pc - d);
+            const f3 jt = f.w2c[v].rot_t(I.rot(n));
+            const f3 jr = cross3(pl, jt);
+            J[0] = jr.x; J[1] = jr.y; J[2] = jr.z; J[3] = jt.x; J[4] = jt.y; J[5] = jt.z;
+        }
+    }
+    rec.r = r;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) rec.J[k] = J[k];
+}
+
+// computeImageDensityJacobainAndResidual (density_map_jacobian.cuh:12-79) as
+// called for density and foreground terms (DensityForegroundJacobianResidual.cu:41-142).
+// The camera-v reference vertex is warped without the move to camera 0, as there.
+__device__ inline void eval_image(const FrameDev& f, const DQ* se3, ushort4 px, bool density, ScalarRec& rec) {
+    const uint32_t v = px.z;
+    const uint32_t o = px.y * f.W + px.x;
+    const f3 cv = xyz(f.ref_v[v][o]);
+    float geometry = 0.f;
+    if (density) {
+        const float4 rgb = f.rgb[v][o];
+        geometry = 0.299f * rgb.x + 0.587f * rgb.y + 0.114f * rgb.z;
+    }
+    const Mat34 T = se3_of(blend4(se3, rec.knn, rec.w));
+    const f3 pl = T.apply(cv);
+    const f3 pc = f.init_inv[v].apply(f.w2c[v].apply(pl));
+    float r = 0.f, J[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const float fx = f.fx[v], fy = f.fy[v];
+    const int u = project_round(pc.x, pc.z, fx, f.cx[v]);
+    const int w = project_round(pc.y, pc.z, fy, f.cy[v]);
+    if (u >= 0 && u < (int)f.W && w >= 0 && w < (int)f.H) {
+        const uint32_t q = w * f.W + u;
+        const float img = density ? f.gray[v][q] : f.ffg[v][q];
+        const float2 g = density ? f.gray_grad[v][q] : f.fg_grad[v][q];
+        f3 d;
+        d.x = g.x * fx / pc.z;
+        d.y = g.y * fy / pc.z;
+        d.z = -1.0f / (pc.z * pc.z) * (fx * pc.x * g.x + fy * pc.y * g.y);
+        d = f.init[v].rot(d);
+        d = f.w2c[v].rot_t(d);
+        J[0] = -d.y * pl.z + d.z * pl.y;
+        J[1] = d.x * pl.z - d.z * pl.x;
+        J[2] = -d.x * pl.y + d.y * pl.x;
+        J[3] = d.x; J[4] = d.y; J[5] = d.z;
+        r = img - geometry;
+    }
+    rec.r = r;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) rec.J[k] = J[k];
+}
+
+// One thread per global term; smooth terms forward-warp the graph edge
+// (NodeGraphSmoothHandler.cu:10-43), feature / cross terms their canonical
+// vertex (SparseCorrespondenceHandler.cu:193-220).
+__global__ void __launch_bounds__(256) eval_terms_kernel(Buffers b, const DQ* __restrict__ se3) {
+    const FrameDev& f = *b.frame;
+    const Counts& c = *b.counts;
+    if (c.error) return;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < c.off[4]; t += stride) {
+        if (t < c.off[0]) {
+            const ushort4 px = b.dense_px[t];
+            ScalarRec rec = b.dense[t];
+            eval_dense(f, se3, px, rec);
+            b.dense[t].r = rec.r;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) b.dense[t].J[k] = rec.J[k];
+            if (b.density) {
+                ScalarRec d = b.density[t];
+                eval_image(f, se3, px, true, d);
+                b.density[t].r = d.r;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) b.density[t].J[k] = d.J[k];
+            }
+        } else if (t < c.off[1]) {
+            const uint32_t e = t - c.off[0];
+            const ushort2 ij = f.graph[e];
+            const float4 xi = f.nodes[ij.x], xj = f.nodes[ij.y];
+            const Mat34 Ti = se3_of(se3[ij.x]), Tj = se3_of(se3[ij.y]);
+            const f3 a = Ti.apply(xyz(xj)), s = Tj.apply(xyz(xj));
+            const f3 d = xyz(xi) - xyz(xj);
+            SmoothRec rec;
+            rec.ij = (uint32_t)ij.x | ((uint32_t)ij.y << 16);
+            rec.valid = dot3(d, d) > 64 * 0.025f * 0.025f ? 0u : 1u;
+            rec.Ti_xj[0] = a.x; rec.Ti_xj[1] = a.y; rec.Ti_xj[2] = a.z;
+            rec.Tj_xj[0] = s.x; rec.Tj_xj[1] = s.y; rec.Tj_xj[2] = s.z;
+            b.smooth[e] = rec;
+        } else if (t < c.off[2]) {
+            const uint32_t k = t - c.off[1];
+            ScalarRec rec = b.fg[k];
+            eval_image(f, se3, b.fg_px[k], false, rec);
+            b.fg[k].r = rec.r;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) b.fg[k].J[q] = rec.J[q];
+        } else {
+            const bool is_cross = t >= c.off[3];
+            const uint32_t k = is_cross ? t - c.off[3] : t - c.off[2];
+            P2PRec* recs = is_cross ? b.cross : b.feat;
+            const float4 can = is_cross ? b.cross_canon[k] : b.feat_canon[k];
+            const Mat34 T = se3_of(blend4(se3, recs[k].knn, recs[k].w));
+            const f3 w = T.apply(xyz(can));
+            recs[k].warped = make_float4(w.x, w.y, w.z, 1.0f);
+        }
+    }
+}
+
+// ------------------------------------------------------- per-term Jacobians
+enum : int { kDense = 0, kSmooth = 1, kForeground = 2, kFeature = 3, kCross = 4 };
+
+__device__ inline int type_of(const Counts& c, uint32_t t, uint32_t& local) {
+    if (t < c.off[0]) { local = t; return kDense; }
+    if (t < c.off[1]) { local = t - c.off[0]; return kSmooth; }
+    if (t < c.off[2]) { local = t - c.off[1]; return kForeground; }
+    if (t < c.off[3]) { local = t - c.off[2]; return kFeature; }
+    local = t - c.off[3];
+    return kCross;
+}
+
+// Knn slot weight of node n (PreconditionerBlocked.cu:25-30): the slot index is
+// the sum 1*(n==y) + 2*(n==z) + 3*(n==w).
+__device__ inline float slot_weight(ushort4 k, float4 w, uint32_t n) {
+    const int s = 1 * (n == k.y) + 2 * (n == k.z) + 3 * (n == k.w);
+    return s == 0 ? w.x : (s == 1 ? w.y : (s == 2 ? w.z : w.w));
+}
+
+// Huber-weighted point-to-point Jacobian rows and residuals
+// (computePointToPointICPTermJacobian, geometry_icp_jacobian.cuh:91-117).
+__device__ inline void p2p_rows(const P2PRec& t, float J[3][6], float res[3]) {
+    const float4 w = t.warped;
+    const float r0 = w.x - t.target.x, r1 = w.y - t.target.y, r2 = w.z - t.target.z;
+    const float h0 = huber(r0), h1 = huber(r1), h2 = huber(r2);
+    J[0][0] = 0.f;        J[0][1] = h0 * w.z;  J[0][2] = -h0 * w.y; J[0][3] = h0;  J[0][4] = 0.f; J[0][5] = 0.f;
+    J[1][0] = -h1 * w.z;  J[1][1] = 0.f;       J[1][2] = h1 * w.x;  J[1][3] = 0.f; J[1][4] = h1;  J[1][5] = 0.f;
+    J[2][0] = h2 * w.y;   J[2][1] = -h2 * w.x; J[2][2] = 0.f;       J[2][3] = 0.f; J[2][4] = 0.f; J[2][5] = h2;
+    res[0] = h0 * r0; res[1] = h1 * r1; res[2] = h2 * r2;
+}
+
+// computeSmoothTermJacobian (geometry_icp_jacobian.cuh:323-343) for node i or j.
+__device__ inline void smooth_rows(const SmoothRec& s, bool node_i, float J[3][6], float res[3]) {
+    const float sg = node_i ? 1.f : -1.f;
+    const float* a = node_i ? s.Ti_xj : s.Tj_xj;
+    const float on = s.valid ? 1.f : 0.f;
+    J[0][0] = 0.f;             J[0][1] = on * sg * a[2];  J[0][2] = -on * sg * a[1];
+    J[1][0] = -on * sg * a[2]; J[1][1] = 0.f;             J[1][2] = on * sg * a[0];
+    J[2][0] = on * sg * a[1];  J[2][1] = -on * sg * a[0]; J[2][2] = 0.f;
+    J[0][3] = on * sg; J[0][4] = 0.f;     J[0][5] = 0.f;
+    J[1][3] = 0.f;     J[1][4] = on * sg; J[1][5] = 0.f;
+    J[2][3] = 0.f;     J[2][4] = 0.f;     J[2][5] = on * sg;
+    res[0] = on * (s.Ti_xj[0] - s.Tj_xj[0]);
+    res[1] = on * (s.Ti_xj[1] - s.Tj_xj[1]);
+    res[2] = on * (s.Ti_xj[2] - s.Tj_xj[2]);
+}
+
+// Jacobian rows of term t with respect to node n (knn weight applied),
+// residuals and per-row lambda^2.  Returns the number of rows (1..4).
+__device__ inline int term_rows(const Buffers& b, const Counts& c, const Penalty& pen, uint32_t t, uint32_t n,
+                                float J[4][6], float res[4], float lam2[4]) {
+    uint32_t k;
+    const int type = type_of(c, t, k);
+    if (type == kDense || type == kForeground) {
+        const ScalarRec& rec = type == kDense ? b.dense[k] : b.fg[k];
+        const float w = slot_weight(rec.knn, rec.w, n);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) J[0][q] = w * rec.J[q];
+        res[0] = rec.r;
+        lam2[0] = type == kDense ? 1.0f : pen.foreground * pen.foreground;
+        if (type == kDense && b.density && pen.density != 0.f) {
+            const ScalarRec& d = b.density[k];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) J[1][q] = w * d.J[q];
+            res[1] = d.r;
+            lam2[1] = pen.density * pen.density;
+            return 2;
+        }
+        return 1;
+    }
+    if (type == kSmooth) {
+        const SmoothRec s = b.smooth[k];
+        smooth_rows(s, n == (s.ij & 0xFFFF), J, res);
+        lam2[0] = lam2[1] = lam2[2] = pen.smooth * pen.smooth;
+        return 3;
+    }
+    const P2PRec& rec = type == kFeature ? b.feat[k] : b.cross[k];
+    p2p_rows(rec, J, res);
+    const float w = slot_weight(rec.knn, rec.w, n);
+    const float l = type == kFeature ? pen.feature : pen.cross;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) J[ch][q] *= w;
+        lam2[ch] = l * l;
+    }
+    return 3;
+}
+
+// -------------------------------------------------- 6x6 SPD inverse (Cholesky)
+// inv(A) = L^-T L^-1 with A = L L^T; A is symmetric positive definite (the
+// reference adds 1e-2 to every diagonal, PreconditionerBlocked.cu:272-275).
+template <typename T>
+__device__ inline void spd_inverse6(const float* A, float* out) {
+    T L[21];  // packed lower triangle, row-major
+#define LI(r, c) L[(r) * ((r) + 1) / 2 + (c)]
+    for (int r = 0; r < 6; ++r) {
+        for (int col = 0; col <= r; ++col) {
+            T s = (T)A[6 * r + col];
+            for (int k = 0; k < col; ++k) s -= LI(r, k) * LI(col, k);
+            if (r == col) LI(r, r) = s > (T)0 ? sqrt(s) : (T)1e-30;
+            else LI(r, col) = s / LI(col, col);
+        }
+    }
+    // X = L^-1 (lower), in place of a dense 6x6
+    T X[36];
+    for (int c2 = 0; c2 < 6; ++c2)
+        for (int r = 0; r < 6; ++r) {
+            T s = (r == c2) ? (T)1 : (T)0;
+            for (int k = 0; k < r; ++k) s -= LI(r, k) * X[6 * k + c2];
+            X[6 * r + c2] = s / LI(r, r);
+        }
+#undef LI
+    // inv = X^T X
+    for (int r = 0; r < 6; ++r)
+        for (int col = 0; col < 6; ++col) {
+            T s = 0;
+            for (int k = 0; k < 6; ++k) s += X[6 * k + r] * X[6 * k + col];
+            out[6 * r + col] = (float)s;
+        }
+}
+
+// ------------------------------------------- diagonal blocks, J^T r, inverse
+// One wave per node over its Node2Term list: D_n = sum lambda^2 J^T J + 1e-2 I
+// (computeBlockDiagonalPreconditionerKernel, PreconditionerBlocked.cu:172-284),
+// b_n = -sum lambda^2 J^T r (computeJtResidualWithIndexKernel,
+// JtDotResidualIndexed.cu:116-224), then the block-Jacobi inverse
+// (blockDiagonalInverseKernel, BlockDiagonalPreconditionerInverse.cu:6-41).
+template <typename InvT>
+__global__ void __launch_bounds__(256) node_blocks_kernel(Buffers b, Penalty pen) {
+    const FrameDev& f = *b.frame;
+    const Counts& c = *b.counts;
+    if (c.error) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); n < f.N; n += waves) {
+        float acc[27];
+#pragma unroll
+        for (int q = 0; q < 27; ++q) acc[q] = 0.f;
+        const uint32_t beg = b.n2t_offset[n], end = b.n2t_offset[n + 1];
+        for (uint32_t p = beg + lane; p < end; p += 64) {
+            float J[4][6], res[4], lam2[4];
+            const int rows = term_rows(b, c, pen, b.n2t_index[p], n, J, res, lam2);
+            for (int ch = 0; ch < rows; ++ch) {
+                int o = 0;
+#pragma unroll
+                for (int a = 0; a < 6; ++a) {
+#pragma unroll
+                    for (int k = a; k < 6; ++k) acc[o++] += lam2[ch] * J[ch][a] * J[ch][k];
+                }
+#pragma unroll
+                for (int a = 0; a < 6; ++a) acc[21 + a] -= lam2[ch] * J[ch][a] * res[ch];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 27; ++q) acc[q] = wave_sum(acc[q]);
+        float D[36];
+        {
+            int o = 0;
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+#pragma unroll
+                for (int k = a; k < 6; ++k) {
+                    D[6 * a + k] = acc[o];
+                    D[6 * k + a] = acc[o];
+                    ++o;
+                }
+#pragma unroll
+            for (int a = 0; a < 6; ++a) D[7 * a] += 1e-2f;
+        }
+        float* Dout = b.diag + 36 * (size_t)n;
+        if (lane < 36) {
+            float val = 0.f;
+#pragma unroll
+            for (int q = 0; q < 36; ++q) val = (q == (int)lane) ? D[q] : val;
+            Dout[lane] = val;
+        }
+        if (lane < 6) {
+            float val = 0.f;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) val = (q == (int)lane) ? acc[21 + q] : val;
+            b.jtr[6 * (size_t)n + lane] = val;
+        }
+        if (lane == 0) spd_inverse6<InvT>(D, b.diag_inv + 36 * (size_t)n);
+    }
+}
+
+// ----------------------------------------------------- off-diagonal blocks
+// kPairLanes lanes per unique pair (i < j): B_ij[a][b] = sum lambda^2 J_i[a] J_j[b]
+// over the pair's term list (computeJtJNonDiagonalBlockKernel,
+// JtJMaterializer.cu:270-365).  Stored once, row-major, row node i.
+constexpr int kPairLanes = 16;
+
+__global__ void __launch_bounds__(256) pair_blocks_kernel(Buffers b, Penalty pen) {
+    const Counts& c = *b.counts;
+    if (c.error) return;
+    const uint32_t sub = threadIdx.x % kPairLanes;
+    const uint32_t groups = gridDim.x * (blockDim.x / kPairLanes);
+    const uint32_t P = c.pairs;
+    for (uint32_t base = blockIdx.x * (blockDim.x / kPairLanes); base < P + 0; base += groups) {
+        const uint32_t p = base + threadIdx.x / kPairLanes;
+        float acc[36];
+#pragma unroll
+        for (int q = 0; q < 36; ++q) acc[q] = 0.f;
+        if (p < P) {
+            const uint32_t key = b.pair_key[p];
+            const uint32_t i = key >> 16, j = key & 0xFFFF;
+            const uint2 range = b.pair_range[p];
+            for (uint32_t q = range.x + sub; q < range.y; q += kPairLanes) {
+                const uint32_t t = b.pair_terms[q];
+                float Ji[4][6], Jj[4][6], ri[4], rj[4], li[4], lj[4];
+                const int rows = term_rows(b, c, pen, t, i, Ji, ri, li);
+                term_rows(b, c, pen, t, j, Jj, rj, lj);
+                for (int ch = 0; ch < rows; ++ch)
+#pragma unroll
+                    for (int a = 0; a < 6; ++a)
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) acc[6 * a + k] += li[ch] * Ji[ch][a] * Jj[ch][k];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 36; ++q) {
+#pragma unroll
+            for (int o = kPairLanes / 2; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
+        }
+        if (p < P) {
+            float* out = b.offdiag + 36 * (size_t)p;
+#pragma unroll
+            for (int q = 0; q < 36; ++q)
+                if ((uint32_t)(q % kPairLanes) == sub) out[q] = acc[q];
+        }
+    }
+}
+
+// ---------------------------------------------------------- total residual
+// computeTermResidualKernel (ResidualEvaluator.cu:71-142): sum of squared
+// lambda-weighted residuals, accumulated in double.
+__global__ void __launch_bounds__(256) cost_kernel(Buffers b, Penalty pen, int density_in_jtj) {
+    const Counts& c = *b.counts;
+    double acc = 0.0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < c.off[4] && !c.error; t += stride) {
+        uint32_t k;
+        const int type = type_of(c, t, k);
+        if (type == kDense) {
+            const double r = b.dense[k].r;
+            acc += r * r;
+            if (density_in_jtj && b.density) {
+                const double d = (double)pen.density * b.density[k].r;
+                acc += d * d;
+            }
+        } else if (type == kForeground) {
+            const double r = (double)pen.foreground * b.fg[k].r;
+            acc += r * r;
+        } else if (type == kSmooth) {
+            float J[3][6], res[3];
+            smooth_rows(b.smooth[k], true, J, res);
+            for (int ch = 0; ch < 3; ++ch) acc += (double)pen.smooth * pen.smooth * res[ch] * res[ch];
+        } else {
+            float J[3][6], res[3];
+            p2p_rows(type == kFeature ? b.feat[k] : b.cross[k], J, res);
+            const double l = type == kFeature ? pen.feature : pen.cross;
+            for (int ch = 0; ch < 3; ++ch) acc += l * l * res[ch] * res[ch];
+        }
+    }
+    acc = wave_sum(acc);
+    __shared__ double part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        b.partial[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+        __threadfence();
+        const uint32_t ticket = atomicAdd(&b.ticket[1], 1u);
+        if (ticket == gridDim.x - 1) {
+            __threadfence();
+            double s = 0.0;
+            for (uint32_t q = 0; q < gridDim.x; ++q) s += b.partial[q];
+            *b.cost = s;
+            b.ticket[1] = 0;
+        }
+    }
+}
+
+// ------------------------------------------------------------- host launch
+void launch_eval_terms(const Buffers& b, const DQ* se3, hipStream_t stream) {
+    hipLaunchKernelGGL(eval_terms_kernel, dim3(1024), dim3(256), 0, stream, b, se3);
+}
+
+void launch_assemble(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream) {
+    const uint32_t node_blocks = (cap_nodes + 3) / 4;
+    if (fp64_inverse)
+        hipLaunchKernelGGL(node_blocks_kernel<double>, dim3(node_blocks), dim3(256), 0, stream, b, pen);
+    else
+        hipLaunchKernelGGL(node_blocks_kernel<float>, dim3(node_blocks), dim3(256), 0, stream, b, pen);
+    const uint32_t pair_blocks = std::min<uint32_t>((b.cap_pairs + 15) / 16, 4096u);
+    hipLaunchKernelGGL(pair_blocks_kernel, dim3(pair_blocks), dim3(256), 0, stream, b, pen);
+}
+
+void launch_cost(const Buffers& b, const Penalty& pen, bool density_in_jtj, hipStream_t stream) {
+    hipLaunchKernelGGL(cost_kernel, dim3(256), dim3(256), 0, stream, b, pen, density_in_jtj ? 1 : 0);
+}
+
+}  // namespace mvdr

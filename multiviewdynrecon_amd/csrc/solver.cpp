@@ -1,0 +1,506 @@
+// Host side of the C ABI (include/mvdr/solver.h): buffer ownership, input
+// validation, the GN schedule and stage launches.  Mirrors
+// SparseSurfelFusion::NonRigidSolver (core/NonRigidSolver/NonRigidSolver.h:78-541).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace mvdr {
+void launch_index_build(const Buffers& b, uint32_t cap_tiles, uint32_t cap_nodes, bool use_foreground, bool use_cross,
+                        hipStream_t stream);
+void launch_eval_terms(const Buffers& b, const DQ* se3, hipStream_t stream);
+void launch_assemble(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream);
+void launch_cost(const Buffers& b, const Penalty& pen, bool density_in_jtj, hipStream_t stream);
+void launch_pcg(const Buffers& b, uint32_t iters, uint32_t refine, bool fp64, uint32_t cap_nodes, hipStream_t stream);
+void launch_apply_update(const Buffers& b, const DQ* in, DQ* out, hipStream_t stream);
+void launch_copy_se3(const Buffers& b, DQ* dst, hipStream_t stream);
+}  // namespace mvdr
+
+using namespace mvdr;
+
+struct mvdr_solver {
+    mvdr_solver_config cfg{};
+    Buffers b{};
+    FrameDev* frame_dev = nullptr;      // device copy of the pointer table
+    FrameDev frame_host{};
+    std::vector<void*> allocations;
+    uint32_t cap_tiles = 0;
+    bool has_input = false;
+    bool indexed = false;
+    uint32_t iter = 0;                  // GN iterations done on the current frame
+    hipEvent_t done = nullptr;          // end of the last enqueued work
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+    std::string error;
+
+    ~mvdr_solver() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (done) (void)hipEventDestroy(done);
+        for (void* p : allocations) (void)hipFree(p);
+    }
+};
+
+namespace {
+
+mvdr_status fail(mvdr_solver* s, mvdr_status st, const std::string& msg) {
+    if (s) s->error = msg;
+    return st;
+}
+
+#define HIP_TRY(s, call)                                                                            \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) return fail((s), MVDR_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+hipError_t alloc(mvdr_solver* s, T*& ptr, size_t count) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) s->allocations.push_back(p);
+    ptr = static_cast<T*>(p);
+    return e;
+}
+
+// setElasticPenaltyValue (SolverIterationData.cpp:104-125) with the constants
+// of PenaltyConstants.cpp:9-51.  Iteration 0 uses the reference's default flags
+// (SolverIterationData.cpp:49): no foreground, density or cross-view penalty.
+Penalty penalty_for(const mvdr_solver_config& c, uint32_t iter) {
+    if (!c.use_elastic_penalty) return Penalty{2.5f, 0.0f, 0.0f, 0.7f, 0.1f};
+    const bool density = iter > 0 && c.use_density;
+    const bool foreground = iter > 0 && c.use_foreground;
+    const bool cross = iter > 0 && c.use_cross_view;
+    if (iter < c.global_iterations) return Penalty{5.0f, 0.0f, foreground ? 2e-3f : 0.0f, 1.0f, cross ? 1.0f : 0.0f};
+    return Penalty{5.0f, density ? 1e-2f : 0.0f, 0.0f, 0.0f, cross ? 1.0f : 0.0f};
+}
+
+// The density channel enters J^T J only under USE_DENSE_IMAGE_DENSITY_TERM
+// (PreconditionerBlocked.cu:216-219), which the reference build leaves off.
+Penalty kernel_penalty(const mvdr_solver_config& c, uint32_t iter) {
+    Penalty p = penalty_for(c, iter);
+    if (!c.density_in_jtj) p.density = 0.f;
+    return p;
+}
+
+const char* error_text(uint32_t bits) {
+    if (bits & kErrDense) return "dense term count exceeds max_views*max_width*max_height";
+    if (bits & kErrFg) return "foreground term count exceeds capacity";
+    if (bits & kErrFeat) return "feature term count exceeds max_feature_pairs";
+    if (bits & kErrCross) return "cross-view term count exceeds max_cross_pairs";
+    if (bits & kErrKv1) return "node-to-term entries exceed capacity";
+    if (bits & kErrKv2) return "node-pair term entries exceed capacity";
+    if (bits & kErrPairs) return "unique node pairs exceed 64 * max_nodes";
+    if (bits & kErrSeg) return "a node's term list exceeds the 8192-entry sort capacity";
+    if (bits & kErrRow) return "a node-pair row exceeds the 4096-entry sort capacity";
+    if (bits & kErrNodes) return "an input node / surfel index is out of range";
+    return "capacity exceeded";
+}
+
+void enqueue_solve(mvdr_solver* s, hipStream_t stream, bool index, uint32_t first_iter, uint32_t iters) {
+    const mvdr_solver_config& c = s->cfg;
+    const bool fp64 = c.precision == MVDR_PRECISION_MIXED;
+    if (index) {
+        launch_copy_se3(s->b, s->b.se3[0], stream);
+        launch_index_build(s->b, s->cap_tiles, c.max_nodes, c.use_foreground != 0, c.use_cross_view != 0, stream);
+    }
+    for (uint32_t k = first_iter; k < first_iter + iters; ++k) {
+        const Penalty pen = kernel_penalty(c, k);
+        const DQ* cur = s->b.se3[k & 1];
+        DQ* next = s->b.se3[(k + 1) & 1];
+        launch_eval_terms(s->b, cur, stream);
+        launch_assemble(s->b, pen, fp64, c.max_nodes, stream);
+        launch_pcg(s->b, c.pcg_iterations, fp64 ? c.refine_steps : 0, fp64, c.max_nodes, stream);
+        launch_apply_update(s->b, cur, next, stream);
+    }
+}
+
+mvdr_status check_counts(mvdr_solver* s, Counts& out) {
+    HIP_TRY(s, hipMemcpy(&out, s->b.counts, sizeof(Counts), hipMemcpyDeviceToHost));
+    if (out.error) return fail(s, MVDR_E_CAPACITY, error_text(out.error));
+    return MVDR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mvdr_status_string(mvdr_status st) {
+    switch (st) {
+        case MVDR_OK: return "MVDR_OK";
+        case MVDR_E_INVALID: return "MVDR_E_INVALID";
+        case MVDR_E_SHAPE: return "MVDR_E_SHAPE";
+        case MVDR_E_CAPACITY: return "MVDR_E_CAPACITY";
+        case MVDR_E_HIP: return "MVDR_E_HIP";
+        case MVDR_E_STATE: return "MVDR_E_STATE";
+    }
+    return "MVDR_E_UNKNOWN";
+}
+
+void mvdr_solver_default_config(mvdr_solver_config* c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->max_views = 3;
+    c->max_width = 640;
+    c->max_height = 480;
+    c->max_nodes = 8192;
+    c->max_graph_edges = 8 * 8192;
+    c->max_feature_pairs = 3 * 1024;
+    c->max_cross_pairs = 4096;
+    c->gn_iterations = 5;
+    c->pcg_iterations = 10;
+    c->global_iterations = 1;
+    c->use_elastic_penalty = 1;
+    c->use_density = 1;
+    c->use_foreground = 0;
+    c->use_cross_view = 1;
+    c->density_in_jtj = 0;
+    c->precision = MVDR_PRECISION_FP32;
+    c->jtj_mode = MVDR_JTJ_MATERIALIZED;
+    c->refine_steps = 0;
+    c->use_graph = 0;
+}
+
+const char* mvdr_solver_last_error(const mvdr_solver* s) { return s ? s->error.c_str() : "null solver"; }
+
+mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** out) {
+    if (!config || !out) return MVDR_E_INVALID;
+    *out = nullptr;
+    const mvdr_solver_config& c = *config;
+    if (c.max_views < 1 || c.max_views > MVDR_MAX_VIEWS || c.max_width < 1 || c.max_height < 1 || c.max_width > 65535 ||
+        c.max_height > 65535 || c.max_nodes < 1 || c.max_nodes > 65535 || c.precision > 1 || c.jtj_mode > 1)
+        return MVDR_E_SHAPE;
+    if (c.jtj_mode == MVDR_JTJ_MATRIX_FREE) return MVDR_E_INVALID;  // not available in this build
+    auto* s = new mvdr_solver();
+    s->cfg = c;
+    Buffers& b = s->b;
+    const uint64_t pixels = (uint64_t)c.max_views * c.max_width * c.max_height;
+    if (pixels > 0x7FFFFFFFull) { delete s; return MVDR_E_SHAPE; }
+    b.cap_pixels = (uint32_t)pixels;
+    b.cap_feat = std::max<uint32_t>(c.max_feature_pairs, 1);
+    b.cap_cross = std::max<uint32_t>(c.max_cross_pairs, 1);
+    b.cap_edges = std::max<uint32_t>(c.max_graph_edges, 1);
+    b.cap_nodes = c.max_nodes;
+    const uint64_t fg = c.use_foreground ? pixels : 0;
+    const uint64_t knn_terms = pixels + fg + b.cap_feat + b.cap_cross;
+    const uint64_t kv1 = 4 * knn_terms + 2ull * b.cap_edges;
+    const uint64_t kv2 = 6 * knn_terms + b.cap_edges;
+    if (kv2 > 0xFFFFFFFFull) { delete s; return MVDR_E_SHAPE; }
+    b.cap_kv1 = (uint32_t)kv1;
+    b.cap_kv2 = (uint32_t)kv2;
+    b.cap_pairs = (uint32_t)std::min<uint64_t>(kv2, 64ull * c.max_nodes);
+    s->cap_tiles = (uint32_t)((pixels + 4095) / 4096);
+    const uint32_t N1 = c.max_nodes + 1;
+
+    hipError_t e = hipSuccess;
+    auto A = [&](auto*& p, size_t n) { if (e == hipSuccess) e = alloc(s, p, n); };
+    A(s->frame_dev, 1);
+    A(b.counts, 1);
+    A(b.dense_px, b.cap_pixels);
+    A(b.dense, b.cap_pixels);
+    b.density = nullptr;
+    if (c.density_in_jtj) A(b.density, b.cap_pixels);
+    b.fg_px = nullptr;
+    b.fg = nullptr;
+    if (c.use_foreground) { A(b.fg_px, b.cap_pixels); A(b.fg, b.cap_pixels); }
+    A(b.smooth, b.cap_edges);
+    A(b.feat, b.cap_feat);
+    A(b.feat_canon, b.cap_feat);
+    A(b.cross, b.cap_cross);
+    A(b.cross_canon, b.cap_cross);
+    A(b.tile_count, s->cap_tiles);
+    A(b.tile_offset, s->cap_tiles);
+    A(b.feat_flag, 1);
+    A(b.n2t_count, N1);
+    A(b.n2t_offset, N1);
+    A(b.n2t_cursor, N1);
+    A(b.n2t_index, b.cap_kv1);
+    A(b.row_count, N1);
+    A(b.row_offset, N1);
+    A(b.row_pairs, N1);
+    A(b.pair_offset, N1);
+    A(b.uniq_j, b.cap_kv2);
+    A(b.uniq_range, b.cap_kv2);
+    A(b.pair_terms, b.cap_kv2);
+    A(b.pair_key, b.cap_pairs);
+    A(b.pair_range, b.cap_pairs);
+    A(b.low_count, N1);
+    A(b.low_offset, N1);
+    A(b.low_cursor, N1);
+    A(b.low_ent, b.cap_pairs);
+    A(b.diag, 36 * (size_t)c.max_nodes);
+    A(b.diag_inv, 36 * (size_t)c.max_nodes);
+    A(b.jtr, 6 * (size_t)c.max_nodes);
+    A(b.offdiag, 36 * (size_t)b.cap_pairs);
+    for (float** v : {&b.x, &b.r, &b.s, &b.q, &b.d0, &b.d1}) A(*v, 6 * (size_t)c.max_nodes);
+    A(b.partial, 4096);
+    A(b.scalars, 16);
+    A(b.ticket, 4);
+    A(b.se3[0], c.max_nodes);
+    A(b.se3[1], c.max_nodes);
+    A(b.cost, 1);
+    if (e == hipSuccess) e = hipMemset(b.ticket, 0, 4 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(b.counts, 0, sizeof(Counts));
+    if (e == hipSuccess) e = hipMemset(b.scalars, 0, 16 * sizeof(double));
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "mvdr_solver_create: %s\n", hipGetErrorString(e));
+        delete s;
+        return MVDR_E_HIP;
+    }
+    b.frame = s->frame_dev;
+    *out = s;
+    return MVDR_OK;
+}
+
+void mvdr_solver_destroy(mvdr_solver* s) {
+    if (!s) return;
+    if (s->done) (void)hipEventSynchronize(s->done);
+    delete s;
+}
+
+mvdr_status mvdr_solver_set_input(mvdr_solver* s, const mvdr_frame_input* in) {
+    if (!s || !in) return MVDR_E_INVALID;
+    const mvdr_solver_config& c = s->cfg;
+    if (in->num_views < 1 || in->num_views > c.max_views || in->width < 1 || in->height < 1 || in->width > c.max_width ||
+        in->height > c.max_height)
+        return fail(s, MVDR_E_SHAPE, "views / image size outside the configured maximum");
+    if (in->num_nodes < 1 || in->num_nodes > c.max_nodes)
+        return fail(s, MVDR_E_SHAPE, "num_nodes outside [1, max_nodes]");
+    if (in->num_graph_edges > c.max_graph_edges) return fail(s, MVDR_E_SHAPE, "num_graph_edges > max_graph_edges");
+    uint64_t feat_total = 0;
+    for (uint32_t v = 0; v < in->num_views; ++v) feat_total += in->num_feature_pairs[v];
+    if (feat_total > c.max_feature_pairs) return fail(s, MVDR_E_SHAPE, "feature pairs > max_feature_pairs");
+    if (in->num_cross_pairs > c.max_cross_pairs) return fail(s, MVDR_E_SHAPE, "cross pairs > max_cross_pairs");
+    if (!in->surfel_knn || !in->surfel_knn_weight || !in->node_se3 || !in->canonical_nodes || in->num_surfels == 0)
+        return fail(s, MVDR_E_INVALID, "surfel knn / node arrays must be set");
+    if (in->num_graph_edges && !in->node_graph) return fail(s, MVDR_E_INVALID, "node_graph is null");
+    if (in->num_cross_pairs && c.use_cross_view && !in->cross_pairs) return fail(s, MVDR_E_INVALID, "cross_pairs is null");
+    for (uint32_t v = 0; v < in->num_views; ++v) {
+        if (!in->vertex_confid[v] || !in->normal_radius[v] || !in->reference_vertex[v] || !in->reference_normal[v] ||
+            !in->warp_vertex[v] || !in->index_map[v])
+            return fail(s, MVDR_E_INVALID, "a required per-view map is null");
+        if ((in->num_feature_pairs[v] || (in->num_cross_pairs && c.use_cross_view)) && !in->edge_mask[v])
+            return fail(s, MVDR_E_INVALID, "edge_mask is required with feature or cross-view pairs");
+        if (in->num_feature_pairs[v] && !in->feature_pairs[v]) return fail(s, MVDR_E_INVALID, "feature_pairs is null");
+        if (c.use_foreground && (!in->filtered_foreground[v] || !in->foreground_gradient[v]))
+            return fail(s, MVDR_E_INVALID, "foreground maps are required when use_foreground is set");
+        if (c.density_in_jtj && (!in->gray[v] || !in->gray_gradient[v] || !in->normalized_rgb[v]))
+            return fail(s, MVDR_E_INVALID, "gray / rgb maps are required when density_in_jtj is set");
+    }
+    FrameDev& f = s->frame_host;
+    std::memset(&f, 0, sizeof(f));
+    f.V = in->num_views;
+    f.W = in->width;
+    f.H = in->height;
+    f.N = in->num_nodes;
+    f.E = in->num_graph_edges;
+    f.num_surfels = in->num_surfels;
+    f.feat_base[0] = 0;
+    for (uint32_t v = 0; v < MVDR_MAX_VIEWS; ++v) {
+        f.num_feat[v] = v < in->num_views ? in->num_feature_pairs[v] : 0;
+        f.feat_base[v + 1] = f.feat_base[v] + f.num_feat[v];
+    }
+    f.num_cross = c.use_cross_view ? in->num_cross_pairs : 0;
+    for (uint32_t v = 0; v < in->num_views; ++v) {
+        f.fx[v] = in->intrinsic[v].fx;
+        f.fy[v] = in->intrinsic[v].fy;
+        f.cx[v] = in->intrinsic[v].cx;
+        f.cy[v] = in->intrinsic[v].cy;
+        f.init[v] = to_mat34(in->initial_camera_se3[v]);
+        f.init_inv[v] = inverse(f.init[v]);
+        f.w2c[v] = to_mat34(in->world2camera[v]);
+        f.c2w[v] = inverse(f.w2c[v]);
+        f.vertex[v] = static_cast<const float4*>(in->vertex_confid[v]);
+        f.normal[v] = static_cast<const float4*>(in->normal_radius[v]);
+        f.ffg[v] = static_cast<const float*>(in->filtered_foreground[v]);
+        f.fg_grad[v] = static_cast<const float2*>(in->foreground_gradient[v]);
+        f.gray[v] = static_cast<const float*>(in->gray[v]);
+        f.gray_grad[v] = static_cast<const float2*>(in->gray_gradient[v]);
+        f.edge[v] = static_cast<const uint8_t*>(in->edge_mask[v]);
+        f.ref_v[v] = static_cast<const float4*>(in->reference_vertex[v]);
+        f.ref_n[v] = static_cast<const float4*>(in->reference_normal[v]);
+        f.warp_v[v] = static_cast<const float4*>(in->warp_vertex[v]);
+        f.index[v] = static_cast<const uint32_t*>(in->index_map[v]);
+        f.rgb[v] = static_cast<const float4*>(in->normalized_rgb[v]);
+        f.feat[v] = static_cast<const ushort4*>(in->feature_pairs[v]);
+    }
+    f.surfel_knn = static_cast<const ushort4*>(in->surfel_knn);
+    f.surfel_w = static_cast<const float4*>(in->surfel_knn_weight);
+    f.node_se3_in = static_cast<const DQ*>(in->node_se3);
+    f.nodes = static_cast<const float4*>(in->canonical_nodes);
+    f.graph = static_cast<const ushort2*>(in->node_graph);
+    f.cross = static_cast<const mvdr_cross_pair*>(in->cross_pairs);
+    // the previous solve may still read the pointer table
+    HIP_TRY(s, hipEventSynchronize(s->done));
+    HIP_TRY(s, hipMemcpy(s->frame_dev, &f, sizeof(FrameDev), hipMemcpyHostToDevice));
+    s->has_input = true;
+    s->indexed = false;
+    s->iter = 0;
+    return MVDR_OK;
+}
+
+mvdr_status mvdr_solver_build_index(mvdr_solver* s, void* stream) {
+    if (!s) return MVDR_E_INVALID;
+    if (!s->has_input) return fail(s, MVDR_E_STATE, "build_index before set_input");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    enqueue_solve(s, st, true, 0, 0);
+    HIP_TRY(s, hipGetLastError());
+    HIP_TRY(s, hipEventRecord(s->done, st));
+    s->indexed = true;
+    s->iter = 0;
+    return MVDR_OK;
+}
+
+mvdr_status mvdr_solver_gn_iteration(mvdr_solver* s, void* stream) {
+    if (!s) return MVDR_E_INVALID;
+    if (!s->indexed) return fail(s, MVDR_E_STATE, "gn_iteration before build_index");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    enqueue_solve(s, st, false, s->iter, 1);
+    HIP_TRY(s, hipGetLastError());
+    HIP_TRY(s, hipEventRecord(s->done, st));
+    ++s->iter;
+    return MVDR_OK;
+}
+
+mvdr_status mvdr_solver_solve(mvdr_solver* s, void* stream) {
+    if (!s) return MVDR_E_INVALID;
+    if (!s->has_input) return fail(s, MVDR_E_STATE, "solve before set_input");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const uint32_t gn = s->cfg.gn_iterations;
+    if (s->cfg.use_graph) {
+        // The pointer table and every buffer are device-resident, so one
+        // captured graph serves every frame of this solver.
+        if (!s->graph_exec) {
+            hipStream_t cap = nullptr;
+            HIP_TRY(s, hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+            hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                enqueue_solve(s, cap, true, 0, gn);
+                e = hipStreamEndCapture(cap, &s->graph);
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0);
+            (void)hipStreamDestroy(cap);
+            if (e != hipSuccess) return fail(s, MVDR_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        }
+        HIP_TRY(s, hipGraphLaunch(s->graph_exec, st));
+    } else {
+        enqueue_solve(s, st, true, 0, gn);
+        HIP_TRY(s, hipGetLastError());
+    }
+    HIP_TRY(s, hipEventRecord(s->done, st));
+    s->indexed = true;
+    s->iter = gn;
+    return MVDR_OK;
+}
+
+mvdr_status mvdr_solver_get_node_se3(mvdr_solver* s, const void** dq, uint32_t* n) {
+    if (!s || !dq || !n) return MVDR_E_INVALID;
+    if (!s->has_input) return fail(s, MVDR_E_STATE, "no input");
+    *dq = s->indexed ? (const void*)s->b.se3[s->iter & 1] : (const void*)s->frame_host.node_se3_in;
+    *n = s->frame_host.N;
+    return MVDR_OK;
+}
+
+mvdr_status mvdr_solver_total_residual(mvdr_solver* s, void* stream, double* cost) {
+    if (!s || !cost) return MVDR_E_INVALID;
+    if (!s->indexed) return fail(s, MVDR_E_STATE, "total_residual before build_index");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const Penalty pen = penalty_for(s->cfg, s->iter);
+    launch_eval_terms(s->b, s->b.se3[s->iter & 1], st);
+    Penalty kp = pen;
+    if (!s->cfg.density_in_jtj) kp.density = 0.f;
+    launch_cost(s->b, kp, s->cfg.density_in_jtj != 0, st);
+    HIP_TRY(s, hipGetLastError());
+    HIP_TRY(s, hipEventRecord(s->done, st));
+    HIP_TRY(s, hipStreamSynchronize(st));
+    Counts cnt;
+    mvdr_status rc = check_counts(s, cnt);
+    if (rc != MVDR_OK) return rc;
+    HIP_TRY(s, hipMemcpy(cost, s->b.cost, sizeof(double), hipMemcpyDeviceToHost));
+    return MVDR_OK;
+}
+
+mvdr_status mvdr_solver_get_sizes(mvdr_solver* s, mvdr_solver_sizes* out) {
+    if (!s || !out) return MVDR_E_INVALID;
+    HIP_TRY(s, hipEventSynchronize(s->done));
+    Counts c;
+    HIP_TRY(s, hipMemcpy(&c, s->b.counts, sizeof(Counts), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(*out));
+    out->num_dense_terms = c.dense;
+    out->num_smooth_terms = c.smooth;
+    out->num_foreground_terms = c.fg;
+    out->num_feature_terms = c.feat;
+    out->num_cross_terms = c.cross;
+    out->num_node2term = c.kv1;
+    out->num_nodepair_terms = c.kv2;
+    out->num_node_pairs = c.pairs;
+    out->num_nodes = s->frame_host.N;
+    out->gn_iterations_done = s->iter;
+    if (c.error) return fail(s, MVDR_E_CAPACITY, error_text(c.error));
+    return MVDR_OK;
+}
+
+mvdr_status mvdr_solver_debug_get(mvdr_solver* s, int tap, void* dst, size_t bytes, size_t* needed) {
+    if (!s) return MVDR_E_INVALID;
+    if (!s->has_input) return fail(s, MVDR_E_STATE, "no input");
+    HIP_TRY(s, hipEventSynchronize(s->done));
+    Counts c;
+    HIP_TRY(s, hipMemcpy(&c, s->b.counts, sizeof(Counts), hipMemcpyDeviceToHost));
+    const Buffers& b = s->b;
+    const uint32_t N = s->frame_host.N;
+    const void* src = nullptr;
+    size_t size = 0;
+    float pen[5];
+    switch (tap) {
+        case MVDR_TAP_NODE2TERM_OFFSET: src = b.n2t_offset; size = 4ull * (N + 1); break;
+        case MVDR_TAP_NODE2TERM_INDEX: src = b.n2t_index; size = 4ull * c.kv1; break;
+        case MVDR_TAP_NODEPAIR_KEY: src = b.pair_key; size = 4ull * c.pairs; break;
+        case MVDR_TAP_NODEPAIR_RANGE: src = b.pair_range; size = 8ull * c.pairs; break;
+        case MVDR_TAP_NODEPAIR_TERM_INDEX: src = b.pair_terms; size = 4ull * c.kv2; break;
+        case MVDR_TAP_DENSE_TERMS: src = b.dense; size = sizeof(ScalarRec) * (size_t)c.dense; break;
+        case MVDR_TAP_SMOOTH_TERMS: src = b.smooth; size = sizeof(SmoothRec) * (size_t)c.smooth; break;
+        case MVDR_TAP_FEATURE_TERMS: src = b.feat; size = sizeof(P2PRec) * (size_t)c.feat; break;
+        case MVDR_TAP_CROSS_TERMS: src = b.cross; size = sizeof(P2PRec) * (size_t)c.cross; break;
+        case MVDR_TAP_FOREGROUND_TERMS: src = b.fg; size = b.fg ? sizeof(ScalarRec) * (size_t)c.fg : 0; break;
+        case MVDR_TAP_DENSE_PIXELS: src = b.dense_px; size = 8ull * c.dense; break;
+        case MVDR_TAP_JTJ_DIAG: src = b.diag; size = 144ull * N; break;
+        case MVDR_TAP_JTJ_DIAG_INV: src = b.diag_inv; size = 144ull * N; break;
+        case MVDR_TAP_JTR: src = b.jtr; size = 24ull * N; break;
+        case MVDR_TAP_JTJ_OFFDIAG: src = b.offdiag; size = 144ull * c.pairs; break;
+        case MVDR_TAP_TWIST: src = b.x; size = 24ull * N; break;
+        case MVDR_TAP_NODE_SE3: {
+            const void* p;
+            uint32_t n;
+            mvdr_solver_get_node_se3(s, &p, &n);
+            src = p;
+            size = 32ull * n;
+            break;
+        }
+        case MVDR_TAP_PENALTY: {
+            // constants of the most recent iteration (iteration 0 before any)
+            const Penalty p = penalty_for(s->cfg, s->iter > 0 ? s->iter - 1 : 0);
+            pen[0] = p.smooth; pen[1] = p.density; pen[2] = p.foreground; pen[3] = p.feature; pen[4] = p.cross;
+            if (needed) *needed = sizeof(pen);
+            if (!dst || bytes < sizeof(pen)) return fail(s, MVDR_E_CAPACITY, "buffer too small");
+            std::memcpy(dst, pen, sizeof(pen));
+            return MVDR_OK;
+        }
+        case MVDR_TAP_DENSITY_TERMS: src = b.density; size = b.density ? sizeof(ScalarRec) * (size_t)c.dense : 0; break;
+        default: return fail(s, MVDR_E_INVALID, "unknown tap");
+    }
+    if (needed) *needed = size;
+    if (size && (!dst || bytes < size)) return fail(s, MVDR_E_CAPACITY, "buffer too small");
+    if (size) HIP_TRY(s, hipMemcpy(dst, src, size, hipMemcpyDeviceToHost));
+    if (c.error) return fail(s, MVDR_E_CAPACITY, error_text(c.error));
+    return MVDR_OK;
+}
+
+}  // extern "C"

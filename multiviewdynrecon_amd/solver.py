@@ -1,0 +1,174 @@
+"""Python mirror of SparseSurfelFusion::NonRigidSolver over the C ABI.
+
+Method names follow the reference class (core/NonRigidSolver/NonRigidSolver.h):
+SetSolverInput, SolveNonRigidAlignment, SolvedNodeSE3,
+ComputeTotalResidualSynced, JtResidualValue, JtJDiagonalBlockValue, plus the
+two halves of the streamed solve (BuildSolverIndex, SolverIteration) used by
+stage-wise parity checks.  Every call goes to libmvdr_solver.so; there is no
+other implementation behind it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+from .config import default_config
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+# tap -> (dtype, record width in elements of dtype)
+TAP_FORMAT = {
+    "node2term_offset": (np.uint32, 1), "node2term_index": (np.uint32, 1), "nodepair_key": (np.uint32, 1),
+    "nodepair_range": (np.uint32, 2), "nodepair_term_index": (np.uint32, 1), "dense_terms": (np.uint32, 16),
+    "smooth_terms": (np.uint32, 8), "feature_terms": (np.uint32, 16), "cross_terms": (np.uint32, 16),
+    "foreground_terms": (np.uint32, 16), "dense_pixels": (np.uint16, 4), "jtj_diag": (np.float32, 36),
+    "jtj_diag_inv": (np.float32, 36), "jtr": (np.float32, 6), "jtj_offdiag": (np.float32, 36),
+    "twist": (np.float32, 6), "node_se3": (np.float32, 8), "penalty": (np.float32, 1),
+    "density_terms": (np.uint32, 16),
+}
+
+
+class SolverError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{_abi.STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+def lib():
+    """Load libmvdr_solver.so (built in-tree by __graft_entry__.build())."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libmvdr_solver.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: the HIP solver library must be built (__graft_entry__.build())")
+        L = C.CDLL(path)
+        L.mvdr_solver_create.argtypes = [C.POINTER(_abi.SolverConfig), C.POINTER(C.c_void_p)]
+        L.mvdr_solver_destroy.argtypes = [C.c_void_p]
+        L.mvdr_solver_last_error.argtypes = [C.c_void_p]
+        L.mvdr_solver_last_error.restype = C.c_char_p
+        L.mvdr_solver_set_input.argtypes = [C.c_void_p, C.POINTER(_abi.FrameInput)]
+        for name in ("solve", "build_index", "gn_iteration"):
+            getattr(L, "mvdr_solver_" + name).argtypes = [C.c_void_p, C.c_void_p]
+        L.mvdr_solver_get_node_se3.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)]
+        L.mvdr_solver_total_residual.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
+        L.mvdr_solver_get_sizes.argtypes = [C.c_void_p, C.POINTER(_abi.SolverSizes)]
+        L.mvdr_solver_debug_get.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.mvdr_solver_default_config.argtypes = [C.POINTER(_abi.SolverConfig)]
+        L.mvdr_status_string.restype = C.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def _stream_handle(stream):
+    if stream is None:
+        import torch
+
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return C.c_void_p(stream)
+    return C.c_void_p(stream.cuda_stream)
+
+
+class NonRigidSolver:
+    """One solver instance (owns all device scratch memory, like the reference)."""
+
+    def __init__(self, config=None, **overrides):
+        self.config = config if config is not None else default_config(**overrides)
+        self.h = C.c_void_p()
+        rc = lib().mvdr_solver_create(C.byref(self.config), C.byref(self.h))
+        if rc != 0:
+            raise SolverError(rc, "mvdr_solver_create failed")
+        self._frame = None
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().mvdr_solver_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise SolverError(rc, lib().mvdr_solver_last_error(self.h).decode())
+
+    # -- NonRigidSolver::SetSolverInput
+    def SetSolverInput(self, frame):
+        """frame: a DeviceFrame (or anything with a `.struct` mvdr_frame_input)."""
+        self._frame = frame  # keeps device buffers alive
+        self._check(lib().mvdr_solver_set_input(self.h, C.byref(frame.struct)))
+
+    # -- NonRigidSolver::SolveNonRigidAlignment
+    def SolveNonRigidAlignment(self, stream=None):
+        self._check(lib().mvdr_solver_solve(self.h, _stream_handle(stream)))
+
+    # -- buildSolverIndexStreamed / solverIteration*Streamed
+    def BuildSolverIndex(self, stream=None):
+        self._check(lib().mvdr_solver_build_index(self.h, _stream_handle(stream)))
+
+    def SolverIteration(self, stream=None):
+        self._check(lib().mvdr_solver_gn_iteration(self.h, _stream_handle(stream)))
+
+    # -- NonRigidSolver::SolvedNodeSE3: (device pointer, node count)
+    def SolvedNodeSE3(self):
+        p = C.c_void_p()
+        n = C.c_uint32()
+        self._check(lib().mvdr_solver_get_node_se3(self.h, C.byref(p), C.byref(n)))
+        return p.value, n.value
+
+    def ComputeTotalResidualSynced(self, stream=None):
+        out = C.c_double()
+        self._check(lib().mvdr_solver_total_residual(self.h, _stream_handle(stream), C.byref(out)))
+        return out.value
+
+    def sizes(self):
+        s = _abi.SolverSizes()
+        self._check(lib().mvdr_solver_get_sizes(self.h, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in _abi.SolverSizes._fields_}
+
+    def debug_get(self, tap):
+        """Copy a stage buffer to a host numpy array (synchronises)."""
+        code = _abi.TAP[tap]
+        need = C.c_size_t()
+        rc = lib().mvdr_solver_debug_get(self.h, code, None, 0, C.byref(need))
+        if rc not in (0, 3):
+            self._check(rc)
+        dtype, width = TAP_FORMAT[tap]
+        out = np.empty(need.value // np.dtype(dtype).itemsize, dtype=dtype)
+        self._check(lib().mvdr_solver_debug_get(self.h, code, out.ctypes.data if out.size else None, need.value,
+                                                C.byref(need)))
+        return out.reshape(-1, width) if width > 1 else out
+
+    # reference accessors (NonRigidSolver.h:329-330)
+    def JtResidualValue(self):
+        return self.debug_get("jtr").reshape(-1)
+
+    def JtJDiagonalBlockValue(self):
+        return self.debug_get("jtj_diag").reshape(-1)
+
+    def solved_node_se3_host(self):
+        return self.debug_get("node_se3")
+
+
+def decode_scalar_terms(raw):
+    """Split MVDR_TAP_{DENSE,FOREGROUND,DENSITY}_TERMS records (16 x u32 each)."""
+    raw = np.asarray(raw).reshape(-1, 16)
+    f = raw.view(np.float32)
+    knn = np.stack([raw[:, 0] & 0xFFFF, raw[:, 0] >> 16, raw[:, 1] & 0xFFFF, raw[:, 1] >> 16], 1).astype(np.uint16)
+    return {"knn": knn, "r": f[:, 2].copy(), "w": f[:, 4:8].copy(), "J": f[:, 8:14].copy()}
+
+
+def decode_smooth_terms(raw):
+    raw = np.asarray(raw).reshape(-1, 8)
+    f = raw.view(np.float32)
+    return {"i": (raw[:, 0] & 0xFFFF).astype(np.uint16), "j": (raw[:, 0] >> 16).astype(np.uint16),
+            "valid": raw[:, 1].astype(np.uint8), "Ti_xj": f[:, 2:5].copy(), "Tj_xj": f[:, 5:8].copy()}
+
+
+def decode_p2p_terms(raw):
+    raw = np.asarray(raw).reshape(-1, 16)
+    f = raw.view(np.float32)
+    knn = np.stack([raw[:, 0] & 0xFFFF, raw[:, 0] >> 16, raw[:, 1] & 0xFFFF, raw[:, 1] >> 16], 1).astype(np.uint16)
+    return {"knn": knn, "w": f[:, 4:8].copy(), "target": f[:, 8:11].copy(), "warped": f[:, 12:15].copy()}
