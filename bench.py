@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark: non-rigid GN + block-PCG solve, ms per frame, on one MI355X.
+
+Workload (default): the three-view body scene -- ~150k surfels, ~4k nodes,
+3 x 640x480 depth + foreground, ~300 feature pairs per view, ~2000 cross-view
+pairs -- solved with 5 Gauss-Newton x 10 PCG iterations in the mixed-precision
+mode (FP32 Jacobians / J^T J blocks, FP64 dots, alpha/beta and 6x6 inverses,
+one residual-refinement step per GN iteration).  A "step" is one full solve of
+one frame: index build + all GN iterations + warp updates, with every input
+already resident in HBM.
+
+The JSON line carries the contract fields plus:
+  roofline      the dominant kernel's algorithmic bytes / launch time vs 8 TB/s
+  cpu_baseline  the CPU restatement of the solve (oracle/, FP32, 1 thread)
+                timed on a bounded sample of frames of the same workload
+  twist_rel_err max over GN iterations of |dx - dx_fp64| / |dx_fp64| against the
+                FP64 host oracle from the same linearisation point
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOAD_NAMES = {
+    0: "sphere_1view_320x240_1k_nodes",
+    1: "body_3view_640x480_4k_nodes",
+    3: "body_3view_640x480_8k_nodes",
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", type=int, default=1, choices=[0, 1, 3])
+    ap.add_argument("--precision", default="mixed", choices=["fp32", "mixed"])
+    ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--cpu-frames", type=int, default=8, help="frames in the CPU-baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--stage-reps", type=int, default=50)
+    return ap.parse_args()
+
+
+def algorithmic_bytes(sz):
+    """Minimum compulsory bytes per launch of each stage (format independent)."""
+    N, P = sz["num_nodes"], sz["num_node_pairs"]
+    Td, Tfg, E = sz["num_dense_terms"], sz["num_foreground_terms"], sz["num_smooth_terms"]
+    Tf, Tc = sz["num_feature_terms"], sz["num_cross_terms"]
+    kv1, kv2 = sz["num_node2term"], sz["num_nodepair_terms"]
+    r_terms = 52 * (Td + Tfg) + 29 * E + 56 * (Tf + Tc)
+    return {
+        "pair_blocks": r_terms + 4 * kv2 + 12 * P + 144 * P,
+        "node_blocks": r_terms + 4 * kv1 + 4 * (N + 1) + 144 * N + 24 * N + 144 * N,
+        "spmv": 144 * (N + P) + 4 * (N + P) + 4 * (N + 1) + 24 * N + 24 * N,
+    }
+
+
+STAGE_IDS = {"eval_terms": 0, "node_blocks": 1, "pair_blocks": 2, "spmv": 3, "index": 4}
+
+
+def stage_time_ms(solver, stage, reps):
+    import ctypes as C
+
+    from multiviewdynrecon_amd.solver import _stream_handle, lib
+
+    out = C.c_float()
+    solver._check(lib().mvdr_solver_bench_stage(solver.h, STAGE_IDS[stage], reps, _stream_handle(None), C.byref(out)))
+    return float(out.value)
+
+
+def cpu_baseline_leg(scene, cfg_kwargs, frames, gpu_solver_factory, frame_dev):
+    """CPU restatement timed on a bounded sample, and the FP64 oracle as the checker
+    of the GPU twist updates.  The oracle is test infrastructure: it is only the
+    timed baseline and the checker here, never part of the measured GPU path."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+
+    from multiviewdynrecon_amd.config import workload_config
+    from multiviewdynrecon_amd.frame import HostFrame
+
+    hf = HostFrame(scene)
+    cfg32 = workload_config(cfg_kwargs["workload"], precision=0, refine_steps=0)
+    times = []
+    t_all = time.perf_counter()
+    for _ in range(frames):
+        o = Oracle(cfg32, hf, "fp32")
+        t0 = time.perf_counter()
+        o.solve()
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_all > 30.0:
+            break
+    cpu = {"value": float(np.median(times) * 1e3), "unit": "ms/frame", "cores": 1, "kind": "port",
+           "sample": f"{len(times)} full solves (index + GN x PCG) of the benchmark frame, FP32, single thread, median"}
+
+    # checker: per-GN-iteration twist error of the GPU path against the FP64 oracle
+    cfg = workload_config(cfg_kwargs["workload"], **cfg_kwargs["overrides"])
+    solver = gpu_solver_factory(cfg)
+    solver.SetSolverInput(frame_dev)
+    solver.BuildSolverIndex()
+    o64 = Oracle(cfg, hf, "fp64")
+    o64.build_index()
+    errs = []
+    for k in range(cfg.gn_iterations):
+        se3 = solver.solved_node_se3_host().astype(np.float64)
+        solver.SolverIteration()
+        tw = solver.debug_get("twist").astype(np.float64).ravel()
+        o64.set("node_se3", se3)
+        o64.set_iteration(k)
+        o64.eval_terms()
+        o64.assemble()
+        o64.pcg(cfg.pcg_iterations)
+        ref = o64.get("twist").ravel()
+        errs.append(float(np.linalg.norm(tw - ref) / max(np.linalg.norm(ref), 1e-30)))
+    return cpu, errs
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build solves on a single GPU; --gpus {args.gpus} is not supported "
+              "(independent frames shard one per GPU, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from multiviewdynrecon_amd.config import workload_config
+    from multiviewdynrecon_amd.frame import DeviceFrame
+    from multiviewdynrecon_amd.scene import make_scene
+    from multiviewdynrecon_amd.solver import NonRigidSolver
+
+    overrides = dict(use_graph=args.graph)
+    if args.precision == "mixed":
+        overrides.update(precision=1, refine_steps=1)
+    else:
+        overrides.update(precision=0, refine_steps=0)
+    cfg = workload_config(args.workload, **overrides)
+
+    scene = make_scene(args.workload)
+    frame = DeviceFrame(scene)
+    solver = NonRigidSolver(cfg)
+    solver.SetSolverInput(frame)
+    for _ in range(args.warmup):
+        solver.SolveNonRigidAlignment()
+    torch.cuda.synchronize()
+    print(f"[bench] warmup done ({args.warmup} solves)", file=sys.stderr, flush=True)
+
+    # timed region: K full solves, synchronised on both sides
+    per = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        solver.SolveNonRigidAlignment()
+        e1.record()
+        per.append((e0, e1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    event_ms = [a.elapsed_time(b) for a, b in per]
+    ms_per_step = wall * 1e3 / args.steps
+    sizes = solver.sizes()
+    print(f"[bench] {args.steps} solves: wall {ms_per_step:.3f} ms/frame, event median {np.median(event_ms):.3f}",
+          file=sys.stderr, flush=True)
+
+    # per-stage kernel timing on the solver's stream (hipEvents around repeated launches)
+    stages = {}
+    for st in ("eval_terms", "node_blocks", "pair_blocks", "spmv", "index"):
+        stages[st] = stage_time_ms(solver, st, args.stage_reps)
+    solver.SolveNonRigidAlignment()  # restore a consistent state after the stage loops
+    torch.cuda.synchronize()
+    ab = algorithmic_bytes(sizes)
+    per_solve = {  # launches of each stage in one solve
+        "eval_terms": cfg.gn_iterations, "node_blocks": cfg.gn_iterations, "pair_blocks": cfg.gn_iterations,
+        "spmv": cfg.gn_iterations * (cfg.pcg_iterations + 2 * cfg.refine_steps), "index": 1,
+    }
+    share = {k: stages[k] * per_solve[k] for k in stages}
+    print("[bench] stage ms/launch " + json.dumps({k: round(v, 4) for k, v in stages.items()}), file=sys.stderr)
+    print("[bench] stage ms/solve  " + json.dumps({k: round(v, 4) for k, v in share.items()}), file=sys.stderr)
+    dominant = max(("pair_blocks", "node_blocks", "spmv"), key=lambda k: share[k])
+    achieved = ab[dominant] / (stages[dominant] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "bytes_per_launch": int(ab[dominant]), "launch_ms": round(stages[dominant], 5)}
+    spmv_gbs = ab["spmv"] / (stages["spmv"] * 1e-3) / 1e9
+    jtj_gbs = (ab["pair_blocks"] + ab["node_blocks"]) / ((stages["pair_blocks"] + stages["node_blocks"]) * 1e-3) / 1e9
+
+    cpu = None
+    errs = None
+    if not args.no_cpu_baseline:
+        cpu, errs = cpu_baseline_leg(scene, {"workload": args.workload, "overrides": overrides},
+                                     args.cpu_frames, lambda c: NonRigidSolver(c), frame)
+        print(f"[bench] cpu baseline {cpu['value']:.1f} ms/frame; twist rel err per GN iteration {errs}",
+              file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "non-rigid solve ms/frame (3 views, ~4k nodes) at <=1e-4 rel err vs FP64",
+        "value": round(ms_per_step, 4),
+        "unit": "ms/frame",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 reductions / 6x6 inverses)" if args.precision == "mixed" else "f32",
+        "data": "synthetic (seeded scene generator)",
+        "config": {"workload": WORKLOAD_NAMES[args.workload], "views": scene.num_views, "width": scene.width,
+                   "height": scene.height, "nodes": scene.num_nodes, "surfels": scene.num_surfels,
+                   "gn_iterations": cfg.gn_iterations, "pcg_iterations": cfg.pcg_iterations,
+                   "precision": args.precision, "hip_graph": bool(args.graph), "dense_terms": sizes["num_dense_terms"],
+                   "node_pairs": sizes["num_node_pairs"], "parallelism": "single GPU"},
+        "event_ms_median": round(float(np.median(event_ms)), 4),
+        "roofline": roofline,
+        "apply_jtj_gbs": round(spmv_gbs, 2),
+        "jtj_assembly_gbs": round(jtj_gbs, 2),
+        "stage_ms": {k: round(v, 5) for k, v in stages.items()},
+        "cpu_baseline": cpu,
+        "twist_rel_err": max(errs) if errs else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -203,6 +203,22 @@ mvdr_status mvdr_solver_get_sizes(mvdr_solver* solver, mvdr_solver_sizes* sizes)
  * returns MVDR_E_CAPACITY without copying (needed may be NULL). */
 mvdr_status mvdr_solver_debug_get(mvdr_solver* solver, int tap, void* dst, size_t bytes, size_t* needed);
 
+typedef enum {
+    MVDR_STAGE_EVAL_TERMS = 0,     /* term residuals / Jacobians                      */
+    MVDR_STAGE_NODE_BLOCKS = 1,    /* J^T J diagonal + J^T r + block inverse          */
+    MVDR_STAGE_PAIR_BLOCKS = 2,    /* J^T J off-diagonal blocks                       */
+    MVDR_STAGE_SPMV = 3,           /* one block-sparse ApplyJtJ (q = A d) + dot(d, q) */
+    MVDR_STAGE_INDEX = 4           /* the whole index build                           */
+} mvdr_stage;
+
+/* Kernel micro-benchmark (the reference keeps a commented 1000x loop around
+ * its JtJ kernel, JtJMaterializer.cpp:52-60): re-launch one stage `reps` times
+ * on `stream` on the data of the last solve and return the mean time per
+ * launch measured with hipEvents on that stream.  Needs a completed solve;
+ * stage buffers hold re-computed values afterwards, which the next solve
+ * overwrites.  Synchronises. */
+mvdr_status mvdr_solver_bench_stage(mvdr_solver* solver, int stage, uint32_t reps, void* stream, float* avg_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,6 +15,9 @@ namespace mvdr {
 
 constexpr int kMaxViews = MVDR_MAX_VIEWS;
 constexpr uint32_t kInvalidIndex = 0xFFFFFFFFu;
+constexpr uint32_t kPartialStride = 1024;  // per-block partial sums per reduction slot
+constexpr uint32_t kPartialSlots = 160;
+constexpr uint32_t kSlotCost = 150;        // partials of the total residual
 
 // ---------------------------------------------------------------- vectors
 struct f3 {
@@ -246,17 +249,14 @@ struct Buffers {
     // linear system
     float* diag;             // 36 N
     float* diag_inv;         // 36 N
+    double* diag_inv64;      // 36 N, mixed mode only
     float* jtr;              // 6 N
     float* offdiag;          // 36 P
     // PCG
     float *x, *r, *s, *q, *d0, *d1;
-    double* partial;         // reduction partials
-    double* scalars;         // [0] delta, [1] delta_old, [2] dq, [3] tmp ...
-    uint32_t* ticket;        // last-block tickets
+    double* partial;         // kPartialSlots x kPartialStride per-block partial sums
     // warp field double buffer
     DQ* se3[2];
-    // residual
-    double* cost;
 };
 
 }  // namespace mvdr

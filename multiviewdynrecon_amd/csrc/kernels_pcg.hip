@@ -2,49 +2,64 @@
 //
 // The PCG is BlockPCG<6>::SolveNoConvergeCheck (BlockPCG.cuh:65-181,312-458)
 // restructured into two launches per iteration:
-//   spmv   : d = s + beta * d (beta = delta / delta_old) folded into the gather of
-//            q = A d, plus dot(d, q) -- the reference's third update, SpMV and
-//            first cublasSdot;
-//   update : alpha = delta / dot(d, q); x += alpha d; r -= alpha q; s = M^-1 r;
-//            delta = dot(r, s) -- its second update and second cublasSdot.
-// Dot products end in the last block to arrive (ticket), which sums the block
-// partials in a fixed order, so results are run-to-run deterministic.  Acc is
-// float for the plain-FP32 mode and double for the mixed mode.
+//   spmv   : d = s + beta * d (beta = delta_k / delta_{k-1}) folded into the
+//            gather of q = A d, plus dot(d, q) -- the reference's third update,
+//            SpMV and first cublasSdot;
+//   update : alpha = delta_k / dot(d, q); x += alpha d; r -= alpha q; s = M^-1 r;
+//            delta_{k+1} = dot(r, s) -- its second update and second cublasSdot.
+// No kernel talks across its own blocks: every dot product is left as one
+// partial per block, and the next launch reduces those partials (in a fixed
+// order, identically in every block), so there is no grid-wide fence and the
+// results are run-to-run deterministic.  Acc is float in the plain-FP32 mode
+// and double in the mixed mode, where M^-1 is also kept in FP64.
 #include "common.h"
 #include "device_util.h"
 
 namespace mvdr {
 
-constexpr int kRowLanes = 8;          // lanes per block row in the SpMV
+constexpr int kRowLanes = 16;         // lanes per block row in the SpMV
 constexpr int kRowsPerBlock = 256 / kRowLanes;
+constexpr uint32_t kVecBlocks = 64;   // grid of the per-node vector kernels
 
-// scalar slots
-enum : int { kDelta = 0, kDeltaOld = 1, kDotDQ = 2, kRefineNum = 3, kRefineDen = 4 };
+// partial-sum slots (kPartialStride doubles each)
+constexpr uint32_t kSlotDelta = 0;     // delta_k   : slot k, k <= kMaxIters
+constexpr uint32_t kSlotDQ = 72;       // dot(d, q)_k: slot 72 + k
+constexpr uint32_t kSlotRefNum = 144;  // refinement numerator / denominator
+constexpr uint32_t kSlotRefDen = 145;
+constexpr uint32_t kMaxIters = 64;
 
+__device__ inline double* slot_ptr(const Buffers& b, uint32_t slot) { return b.partial + (size_t)slot * kPartialStride; }
+
+// Sum of `count` partials, computed identically by every block (fixed order).
 template <typename Acc>
-__device__ inline void finish_dot(const Buffers& b, Acc block_value, int slot, int shift_delta) {
-    __shared__ Acc part[4];
-    Acc v = wave_sum(block_value);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+__device__ inline Acc reduce_partials(const double* p, uint32_t count) {
+    __shared__ Acc sh[256];
+    Acc v = 0;
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) v += (Acc)p[i];
+    sh[threadIdx.x] = v;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const Acc s = part[0] + part[1] + part[2] + part[3];
-        b.partial[blockIdx.x] = (double)s;
-        __threadfence();
-        const uint32_t ticket = atomicAdd(&b.ticket[0], 1u);
-        if (ticket == gridDim.x - 1) {
-            __threadfence();
-            Acc total = 0;
-            for (uint32_t q = 0; q < gridDim.x; ++q) total += (Acc)b.partial[q];
-            if (shift_delta) b.scalars[kDeltaOld] = b.scalars[kDelta];
-            b.scalars[slot] = (double)total;
-            b.ticket[0] = 0;
-        }
+    for (uint32_t w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
     }
+    const Acc total = sh[0];
+    __syncthreads();
+    return total;
 }
 
-// 6x6 row-major block times a 6-vector (or its transpose).
-__device__ inline void block_mv(const float* __restrict__ B, bool transposed, const float v[6], float y[6]) {
+// Block sum of per-thread values, stored as this block's partial of `slot`.
+template <typename Acc>
+__device__ inline void write_partial(const Buffers& b, uint32_t slot, Acc value) {
+    __shared__ Acc part[4];
+    value = wave_sum(value);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = value;
+    __syncthreads();
+    if (threadIdx.x == 0) slot_ptr(b, slot)[blockIdx.x] = (double)(part[0] + part[1] + part[2] + part[3]);
+}
+
+// 6x6 row-major block times a 6-vector (or its transpose), accumulated in Acc.
+template <typename Acc>
+__device__ inline void block_mv(const float* __restrict__ B, bool transposed, const float v[6], Acc y[6]) {
     float m[36];
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
@@ -55,12 +70,36 @@ __device__ inline void block_mv(const float* __restrict__ B, bool transposed, co
 #pragma unroll
         for (int a = 0; a < 6; ++a)
 #pragma unroll
-            for (int k = 0; k < 6; ++k) y[a] += m[6 * a + k] * v[k];
+            for (int k = 0; k < 6; ++k) y[a] += (Acc)m[6 * a + k] * (Acc)v[k];
     } else {
 #pragma unroll
         for (int a = 0; a < 6; ++a)
 #pragma unroll
-            for (int k = 0; k < 6; ++k) y[a] += m[6 * k + a] * v[k];
+            for (int k = 0; k < 6; ++k) y[a] += (Acc)m[6 * k + a] * (Acc)v[k];
+    }
+}
+
+// M^-1 v for node n: FP64 inverse in the mixed mode, FP32 otherwise.
+template <typename Acc>
+__device__ inline void apply_minv(const Buffers& b, uint32_t n, const float v[6], float out[6]) {
+    if constexpr (sizeof(Acc) == 8) {
+        const double* M = b.diag_inv64 + 36 * (size_t)n;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) s += M[6 * a + k] * (double)v[k];
+            out[a] = (float)s;
+        }
+    } else {
+        const float* M = b.diag_inv + 36 * (size_t)n;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+            float s = 0.f;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) s += M[6 * a + k] * v[k];
+            out[a] = s;
+        }
     }
 }
 
@@ -75,14 +114,14 @@ __device__ inline void load_dir(const float* __restrict__ prev, const float* __r
     }
 }
 
-// y = (A v)_i over the block row i: diagonal, lower entries (B_ki^T, k < i),
-// upper entries (B_ij, j > i), in ascending column order, split over the
-// kRowLanes lanes of the row group and reduced across them.
+// y = (A v)_i over block row i: diagonal, lower entries (B_ki^T, k < i), upper
+// entries (B_ij, j > i), split over the kRowLanes lanes of the row group and
+// reduced across them.
 template <typename Acc>
 __device__ inline void row_apply(const Buffers& b, uint32_t i, uint32_t sub, const float* prev, const float* s, bool use_beta,
-                                 Acc beta, float y[6]) {
+                                 Acc beta, Acc y[6]) {
 #pragma unroll
-    for (int a = 0; a < 6; ++a) y[a] = 0.f;
+    for (int a = 0; a < 6; ++a) y[a] = 0;
     const uint32_t lo_beg = b.low_offset[i], lo_n = b.low_offset[i + 1] - lo_beg;
     const uint32_t up_beg = b.pair_offset[i], up_n = b.pair_offset[i + 1] - up_beg;
     const uint32_t total = 1 + lo_n + up_n;
@@ -107,7 +146,7 @@ __device__ inline void row_apply(const Buffers& b, uint32_t i, uint32_t sub, con
         }
         float v[6];
         load_dir<Acc>(prev, s, use_beta, beta, j, v);
-        block_mv(B, tr, v, y);
+        block_mv<Acc>(B, tr, v, y);
     }
 #pragma unroll
     for (int a = 0; a < 6; ++a)
@@ -115,8 +154,8 @@ __device__ inline void row_apply(const Buffers& b, uint32_t i, uint32_t sub, con
         for (int o = kRowLanes / 2; o > 0; o >>= 1) y[a] += __shfl_xor(y[a], o, 64);
 }
 
-// r = b, d = M^-1 b, x = 0, delta = dot(r, d) (blockPCGZeroInitKernel,
-// BlockPCG.cuh:65-111, then cublasSdot).
+// r = b, d = M^-1 b, x = 0, partials of delta_0 = dot(r, d)
+// (blockPCGZeroInitKernel, BlockPCG.cuh:65-111, then cublasSdot).
 template <typename Acc>
 __global__ void __launch_bounds__(256) pcg_init_kernel(Buffers b) {
     const FrameDev& f = *b.frame;
@@ -124,95 +163,90 @@ __global__ void __launch_bounds__(256) pcg_init_kernel(Buffers b) {
     if (!b.counts->error) {
         const uint32_t stride = gridDim.x * blockDim.x;
         for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < f.N; n += stride) {
-            float rb[6];
+            float rb[6], dv[6];
 #pragma unroll
             for (int a = 0; a < 6; ++a) rb[a] = b.jtr[6 * n + a];
-            const float* M = b.diag_inv + 36 * (size_t)n;
+            apply_minv<Acc>(b, n, rb, dv);
 #pragma unroll
             for (int a = 0; a < 6; ++a) {
-                float dv = 0.f;
-#pragma unroll
-                for (int k = 0; k < 6; ++k) dv += M[6 * a + k] * rb[k];
                 b.r[6 * n + a] = rb[a];
-                b.d0[6 * n + a] = dv;
+                b.d0[6 * n + a] = dv[a];
                 b.x[6 * n + a] = 0.f;
-                acc += (Acc)rb[a] * (Acc)dv;
+                acc += (Acc)rb[a] * (Acc)dv[a];
             }
         }
     }
-    finish_dot<Acc>(b, acc, kDelta, 0);
+    write_partial<Acc>(b, kSlotDelta + 0, acc);
 }
 
-// q = A d with d = s + beta * d_prev folded in (iterations after the first);
-// writes d to d_out and dot(d, q) to the kDotDQ slot.
+// Iteration k: q = A d with d = s + beta d_prev (k > 0); partials of dot(d, q).
 template <typename Acc>
-__global__ void __launch_bounds__(256) pcg_spmv_kernel(Buffers b, const float* __restrict__ d_prev, float* __restrict__ d_out,
-                                                       int use_beta) {
+__global__ void __launch_bounds__(256) pcg_spmv_kernel(Buffers b, uint32_t k, const float* __restrict__ d_prev,
+                                                       float* __restrict__ d_out) {
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     Acc beta = 0;
-    if (use_beta) {
-        const Acc den = (Acc)b.scalars[kDeltaOld];
-        beta = den != (Acc)0 ? (Acc)b.scalars[kDelta] / den : (Acc)0;
+    if (k > 0) {
+        const Acc num = reduce_partials<Acc>(slot_ptr(b, kSlotDelta + k), kVecBlocks);
+        const Acc den = reduce_partials<Acc>(slot_ptr(b, kSlotDelta + k - 1), kVecBlocks);
+        beta = den != (Acc)0 ? num / den : (Acc)0;
     }
     Acc acc = 0;
     if (!b.counts->error) {
         const uint32_t groups = gridDim.x * kRowsPerBlock;
         for (uint32_t base = blockIdx.x * kRowsPerBlock; base < f.N; base += groups) {
             const uint32_t i = base + threadIdx.x / kRowLanes;
-            float y[6];
-            const uint32_t row = i < f.N ? i : 0;
-            row_apply<Acc>(b, row, sub, d_prev, b.s, use_beta != 0, beta, y);
+            Acc y[6];
+            row_apply<Acc>(b, i < f.N ? i : 0, sub, d_prev, b.s, k > 0, beta, y);
             if (i < f.N && sub == 0) {
                 float v[6];
-                load_dir<Acc>(d_prev, b.s, use_beta != 0, beta, i, v);
+                load_dir<Acc>(d_prev, b.s, k > 0, beta, i, v);
 #pragma unroll
                 for (int a = 0; a < 6; ++a) {
                     d_out[6 * i + a] = v[a];
-                    b.q[6 * i + a] = y[a];
-                    acc += (Acc)v[a] * (Acc)y[a];
+                    b.q[6 * i + a] = (float)y[a];
+                    acc += (Acc)v[a] * y[a];
                 }
             }
         }
     }
-    finish_dot<Acc>(b, acc, kDotDQ, 0);
+    write_partial<Acc>(b, kSlotDQ + k, acc);
 }
 
-// alpha = delta / dot(d, q); x += alpha d; r -= alpha q; s = M^-1 r;
-// delta_old = delta; delta = dot(r, s) (blockPCGSecondUpdateKernel,
+// Iteration k: alpha = delta_k / dot(d, q); x += alpha d; r -= alpha q;
+// s = M^-1 r; partials of delta_{k+1} = dot(r, s) (blockPCGSecondUpdateKernel,
 // BlockPCG.cuh:120-161).  A zero denominator gives a zero step.
 template <typename Acc>
-__global__ void __launch_bounds__(256) pcg_update_kernel(Buffers b, const float* __restrict__ d) {
+__global__ void __launch_bounds__(256) pcg_update_kernel(Buffers b, uint32_t k, uint32_t spmv_blocks, const float* __restrict__ d) {
     const FrameDev& f = *b.frame;
-    const Acc dq = (Acc)b.scalars[kDotDQ];
-    const Acc alpha = dq != (Acc)0 ? (Acc)b.scalars[kDelta] / dq : (Acc)0;
+    const Acc delta = reduce_partials<Acc>(slot_ptr(b, kSlotDelta + k), kVecBlocks);
+    const Acc dq = reduce_partials<Acc>(slot_ptr(b, kSlotDQ + k), spmv_blocks);
+    const Acc alpha = dq != (Acc)0 ? delta / dq : (Acc)0;
     Acc acc = 0;
     if (!b.counts->error) {
         const uint32_t stride = gridDim.x * blockDim.x;
         for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < f.N; n += stride) {
-            float t[6];
+            float t[6], sv[6];
 #pragma unroll
             for (int a = 0; a < 6; ++a) {
-                const uint32_t k = 6 * n + a;
-                b.x[k] = (float)((Acc)b.x[k] + alpha * (Acc)d[k]);
-                t[a] = (float)((Acc)b.r[k] - alpha * (Acc)b.q[k]);
-                b.r[k] = t[a];
+                const uint32_t i = 6 * n + a;
+                b.x[i] = (float)((Acc)b.x[i] + alpha * (Acc)d[i]);
+                t[a] = (float)((Acc)b.r[i] - alpha * (Acc)b.q[i]);
+                b.r[i] = t[a];
             }
-            const float* M = b.diag_inv + 36 * (size_t)n;
+            apply_minv<Acc>(b, n, t, sv);
 #pragma unroll
             for (int a = 0; a < 6; ++a) {
-                float sv = 0.f;
-#pragma unroll
-                for (int k = 0; k < 6; ++k) sv += M[6 * a + k] * t[k];
-                b.s[6 * n + a] = sv;
-                acc += (Acc)t[a] * (Acc)sv;
+                b.s[6 * n + a] = sv[a];
+                acc += (Acc)t[a] * (Acc)sv[a];
             }
         }
     }
-    finish_dot<Acc>(b, acc, kDelta, 1);
+    write_partial<Acc>(b, kSlotDelta + k + 1, acc);
 }
 
-// Residual refinement (mixed mode): r = b - A x, z = M^-1 r, num = dot(r, z).
+// Residual refinement (mixed mode), one preconditioned steepest-descent step
+// on the true residual: r = b - A x, z = M^-1 r, num = dot(r, z) ...
 template <typename Acc>
 __global__ void __launch_bounds__(256) refine_residual_kernel(Buffers b) {
     const FrameDev& f = *b.frame;
@@ -222,31 +256,28 @@ __global__ void __launch_bounds__(256) refine_residual_kernel(Buffers b) {
         const uint32_t groups = gridDim.x * kRowsPerBlock;
         for (uint32_t base = blockIdx.x * kRowsPerBlock; base < f.N; base += groups) {
             const uint32_t i = base + threadIdx.x / kRowLanes;
-            float y[6];
+            Acc y[6];
             row_apply<Acc>(b, i < f.N ? i : 0, sub, b.x, b.s, false, (Acc)0, y);
             if (i < f.N && sub == 0) {
-                float rr[6];
+                float rr[6], z[6];
 #pragma unroll
                 for (int a = 0; a < 6; ++a) {
-                    rr[a] = (float)((Acc)b.jtr[6 * i + a] - (Acc)y[a]);
+                    rr[a] = (float)((Acc)b.jtr[6 * i + a] - y[a]);
                     b.r[6 * i + a] = rr[a];
                 }
-                const float* M = b.diag_inv + 36 * (size_t)i;
+                apply_minv<Acc>(b, i, rr, z);
 #pragma unroll
                 for (int a = 0; a < 6; ++a) {
-                    float z = 0.f;
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) z += M[6 * a + k] * rr[k];
-                    b.d1[6 * i + a] = z;
-                    acc += (Acc)rr[a] * (Acc)z;
+                    b.d1[6 * i + a] = z[a];
+                    acc += (Acc)rr[a] * (Acc)z[a];
                 }
             }
         }
     }
-    finish_dot<Acc>(b, acc, kRefineNum, 0);
+    write_partial<Acc>(b, kSlotRefNum, acc);
 }
 
-// q = A z, den = dot(z, q)
+// ... den = dot(z, A z) ...
 template <typename Acc>
 __global__ void __launch_bounds__(256) refine_curvature_kernel(Buffers b) {
     const FrameDev& f = *b.frame;
@@ -256,33 +287,33 @@ __global__ void __launch_bounds__(256) refine_curvature_kernel(Buffers b) {
         const uint32_t groups = gridDim.x * kRowsPerBlock;
         for (uint32_t base = blockIdx.x * kRowsPerBlock; base < f.N; base += groups) {
             const uint32_t i = base + threadIdx.x / kRowLanes;
-            float y[6];
+            Acc y[6];
             row_apply<Acc>(b, i < f.N ? i : 0, sub, b.d1, b.s, false, (Acc)0, y);
             if (i < f.N && sub == 0) {
 #pragma unroll
-                for (int a = 0; a < 6; ++a) acc += (Acc)b.d1[6 * i + a] * (Acc)y[a];
+                for (int a = 0; a < 6; ++a) acc += (Acc)b.d1[6 * i + a] * y[a];
             }
         }
     }
-    finish_dot<Acc>(b, acc, kRefineDen, 0);
+    write_partial<Acc>(b, kSlotRefDen, acc);
 }
 
-// x += (num / den) z
+// ... x += (num / den) z
 template <typename Acc>
-__global__ void __launch_bounds__(256) refine_step_kernel(Buffers b) {
+__global__ void __launch_bounds__(256) refine_step_kernel(Buffers b, uint32_t spmv_blocks) {
     const FrameDev& f = *b.frame;
+    const Acc num = reduce_partials<Acc>(slot_ptr(b, kSlotRefNum), spmv_blocks);
+    const Acc den = reduce_partials<Acc>(slot_ptr(b, kSlotRefDen), spmv_blocks);
     if (b.counts->error) return;
-    const Acc den = (Acc)b.scalars[kRefineDen];
-    const Acc a = den > (Acc)0 ? (Acc)b.scalars[kRefineNum] / den : (Acc)0;
+    const Acc a = den > (Acc)0 ? num / den : (Acc)0;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < 6 * f.N; k += stride)
-        b.x[k] = (float)((Acc)b.x[k] + a * (Acc)b.d1[k]);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < 6 * f.N; i += stride)
+        b.x[i] = (float)((Acc)b.x[i] + a * (Acc)b.d1[i]);
 }
 
 // ---------------------------------------------------------- warp update
-// applyWarpFieldUpdateKernel (SolverIterationData.cu:9-57): Rodrigues twist
-// (MatUtils.cpp:10-40) left-composed with the node's SE3, back to a dual
-// quaternion (DualQuaternion(const mat34&), DualQuaternion.h:123-130).
+// DualQuaternion(const mat34&) (DualQuaternion.h:123-130) with Quaternion(mat33)
+// (Quaternion.h:46-77).
 __device__ inline DQ dq_from_mat34(const Mat34& T) {
     const float m00 = T.c0.x, m10 = T.c0.y, m20 = T.c0.z;
     const float m01 = T.c1.x, m11 = T.c1.y, m21 = T.c1.z;
@@ -327,6 +358,8 @@ __device__ inline Mat34 mat34_of(DQ d) {
     return m;
 }
 
+// applyWarpFieldUpdateKernel (SolverIterationData.cu:9-57): Rodrigues twist
+// (MatUtils.cpp:10-40) left-composed with the node's SE3.
 __global__ void __launch_bounds__(256) apply_update_kernel(Buffers b, const DQ* __restrict__ se3_in, DQ* __restrict__ se3_out) {
     const FrameDev& f = *b.frame;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -336,8 +369,15 @@ __global__ void __launch_bounds__(256) apply_update_kernel(Buffers b, const DQ* 
             se3_out[n] = se3_in[n];
             continue;
         }
-        const float* xi = b.x + 6 * (size_t)n;
-        const f3 w = mk(xi[0], xi[1], xi[2]), v = mk(xi[3], xi[4], xi[5]);
+        // node-centred solution (w, u) -> reference twist (w, v = u + g x w),
+        // written back so that the twist buffer holds the reference quantity
+        float* xi = b.x + 6 * (size_t)n;
+        const float4 g4 = f.nodes[n];
+        const f3 w = mk(xi[0], xi[1], xi[2]);
+        const f3 v = mk(xi[3], xi[4], xi[5]) + cross3(mk(g4.x, g4.y, g4.z), w);
+        xi[3] = v.x;
+        xi[4] = v.y;
+        xi[5] = v.z;
         Mat34 R;
         R.c0 = mk(1, 0, 0); R.c1 = mk(0, 1, 0); R.c2 = mk(0, 0, 1);
         if (fabsf(w.x) + fabsf(w.y) + fabsf(w.z) >= 1e-4f) {
@@ -367,31 +407,39 @@ __global__ void copy_se3_kernel(Buffers b, DQ* __restrict__ dst) {
 }
 
 // ------------------------------------------------------------- host launch
-constexpr uint32_t kVecBlocks = 64;
+static uint32_t spmv_grid(uint32_t cap_nodes) {
+    return std::max<uint32_t>(1, std::min<uint32_t>((cap_nodes + kRowsPerBlock - 1) / kRowsPerBlock, kPartialStride));
+}
 
 template <typename Acc>
 static void pcg_impl(const Buffers& b, uint32_t iters, uint32_t refine, uint32_t cap_nodes, hipStream_t stream) {
-    const uint32_t spmv_blocks = std::max<uint32_t>(1, std::min<uint32_t>((cap_nodes + kRowsPerBlock - 1) / kRowsPerBlock, 1024u));
+    const uint32_t sg = spmv_grid(cap_nodes);
+    iters = std::min(iters, kMaxIters);
     hipLaunchKernelGGL(pcg_init_kernel<Acc>, dim3(kVecBlocks), dim3(256), 0, stream, b);
-    // d lives in d0 after init; the SpMV writes the folded direction to the other buffer
     float* dbuf[2] = {b.d0, b.d1};
     int cur = 0;
-    for (uint32_t it = 0; it < iters; ++it) {
-        hipLaunchKernelGGL(pcg_spmv_kernel<Acc>, dim3(spmv_blocks), dim3(256), 0, stream, b, dbuf[cur], dbuf[cur ^ 1],
-                           it > 0 ? 1 : 0);
+    for (uint32_t k = 0; k < iters; ++k) {
+        hipLaunchKernelGGL(pcg_spmv_kernel<Acc>, dim3(sg), dim3(256), 0, stream, b, k, dbuf[cur], dbuf[cur ^ 1]);
         cur ^= 1;
-        hipLaunchKernelGGL(pcg_update_kernel<Acc>, dim3(kVecBlocks), dim3(256), 0, stream, b, dbuf[cur]);
+        hipLaunchKernelGGL(pcg_update_kernel<Acc>, dim3(kVecBlocks), dim3(256), 0, stream, b, k, sg, dbuf[cur]);
     }
-    for (uint32_t k = 0; k < refine; ++k) {
-        hipLaunchKernelGGL(refine_residual_kernel<Acc>, dim3(spmv_blocks), dim3(256), 0, stream, b);
-        hipLaunchKernelGGL(refine_curvature_kernel<Acc>, dim3(spmv_blocks), dim3(256), 0, stream, b);
-        hipLaunchKernelGGL(refine_step_kernel<Acc>, dim3(kVecBlocks), dim3(256), 0, stream, b);
+    for (uint32_t r = 0; r < refine; ++r) {
+        hipLaunchKernelGGL(refine_residual_kernel<Acc>, dim3(sg), dim3(256), 0, stream, b);
+        hipLaunchKernelGGL(refine_curvature_kernel<Acc>, dim3(sg), dim3(256), 0, stream, b);
+        hipLaunchKernelGGL(refine_step_kernel<Acc>, dim3(kVecBlocks), dim3(256), 0, stream, b, sg);
     }
 }
 
 void launch_pcg(const Buffers& b, uint32_t iters, uint32_t refine, bool fp64, uint32_t cap_nodes, hipStream_t stream) {
     if (fp64) pcg_impl<double>(b, iters, refine, cap_nodes, stream);
     else pcg_impl<float>(b, iters, refine, cap_nodes, stream);
+}
+
+// One ApplyJtJ (q = A d0, no direction update) for stage timing.
+void launch_spmv_once(const Buffers& b, bool fp64, uint32_t cap_nodes, hipStream_t stream) {
+    const uint32_t sg = spmv_grid(cap_nodes);
+    if (fp64) hipLaunchKernelGGL(pcg_spmv_kernel<double>, dim3(sg), dim3(256), 0, stream, b, 0u, b.d0, b.d1);
+    else hipLaunchKernelGGL(pcg_spmv_kernel<float>, dim3(sg), dim3(256), 0, stream, b, 0u, b.d0, b.d1);
 }
 
 void launch_apply_update(const Buffers& b, const DQ* in, DQ* out, hipStream_t stream) {

@@ -227,11 +227,48 @@ __device__ inline int term_rows(const Buffers& b, const Counts& c, const Penalty
     return 3;
 }
 
+// ------------------------------------------------ node-centred coordinates
+// The reference twist (w, v) moves a point p by w x p + v: its rotation is
+// about the world origin, so every node block has a near-null direction
+// ("rotate about the node, translate back") and FP32 storage of J^T J costs
+// ~1e-3 in the solution.  The solver assembles in the node-centred twist
+// (w, u), u = v + w x g_n, i.e. x_ref = T_n x with T_n = [[I, 0], [[g_n]x, I]].
+// Block-Jacobi PCG is invariant under this block-diagonal congruence
+// (A' = T^T A T, M' = T^T M T, b' = T^T b give x' = T^-1 x in exact arithmetic),
+// so the iterates are the reference's; only their FP32 rounding improves.
+// A row J = (J_w, J_v) becomes J' = (J_w + J_v x g, J_v).
+__device__ inline void to_node_frame(float J[6], const float4 g) {
+    const float a = J[4] * g.z - J[5] * g.y;
+    const float b = J[5] * g.x - J[3] * g.z;
+    const float c = J[3] * g.y - J[4] * g.x;
+    J[0] += a;
+    J[1] += b;
+    J[2] += c;
+}
+
+// The reference's damping 1e-2 I (PreconditionerBlocked.cu:272-275) seen in
+// node-centred coordinates: 1e-2 T^T T = 1e-2 [[I + G^T G, G^T], [G, I]], G = [g]x.
+__device__ inline void add_node_frame_damping(float D[36], const float4 g) {
+    const float l = 1e-2f;
+    const float gg = g.x * g.x + g.y * g.y + g.z * g.z;
+    const float gv[3] = {g.x, g.y, g.z};
+    const float G[3][3] = {{0.f, -g.z, g.y}, {g.z, 0.f, -g.x}, {-g.y, g.x, 0.f}};
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            D[6 * r + c] += l * ((r == c ? 1.f + gg : 0.f) - gv[r] * gv[c]);
+            D[6 * r + 3 + c] += l * G[c][r];        // G^T
+            D[6 * (3 + r) + c] += l * G[r][c];      // G
+            D[6 * (3 + r) + 3 + c] += (r == c) ? l : 0.f;
+        }
+}
+
 // -------------------------------------------------- 6x6 SPD inverse (Cholesky)
 // inv(A) = L^-T L^-1 with A = L L^T; A is symmetric positive definite (the
 // reference adds 1e-2 to every diagonal, PreconditionerBlocked.cu:272-275).
 template <typename T>
-__device__ inline void spd_inverse6(const float* A, float* out) {
+__device__ inline void spd_inverse6(const float* A, float* out, double* out64) {
     T L[21];  // packed lower triangle, row-major
 #define LI(r, c) L[(r) * ((r) + 1) / 2 + (c)]
     for (int r = 0; r < 6; ++r) {
@@ -257,6 +294,7 @@ __device__ inline void spd_inverse6(const float* A, float* out) {
             T s = 0;
             for (int k = 0; k < 6; ++k) s += X[6 * k + r] * X[6 * k + col];
             out[6 * r + col] = (float)s;
+            if (out64) out64[6 * r + col] = (double)s;
         }
 }
 
@@ -277,11 +315,13 @@ __global__ void __launch_bounds__(256) node_blocks_kernel(Buffers b, Penalty pen
         float acc[27];
 #pragma unroll
         for (int q = 0; q < 27; ++q) acc[q] = 0.f;
+        const float4 g = f.nodes[n];
         const uint32_t beg = b.n2t_offset[n], end = b.n2t_offset[n + 1];
         for (uint32_t p = beg + lane; p < end; p += 64) {
             float J[4][6], res[4], lam2[4];
             const int rows = term_rows(b, c, pen, b.n2t_index[p], n, J, res, lam2);
             for (int ch = 0; ch < rows; ++ch) {
+                to_node_frame(J[ch], g);
                 int o = 0;
 #pragma unroll
                 for (int a = 0; a < 6; ++a) {
@@ -305,8 +345,7 @@ __global__ void __launch_bounds__(256) node_blocks_kernel(Buffers b, Penalty pen
                     D[6 * k + a] = acc[o];
                     ++o;
                 }
-#pragma unroll
-            for (int a = 0; a < 6; ++a) D[7 * a] += 1e-2f;
+            add_node_frame_damping(D, g);
         }
         float* Dout = b.diag + 36 * (size_t)n;
         if (lane < 36) {
@@ -321,7 +360,9 @@ __global__ void __launch_bounds__(256) node_blocks_kernel(Buffers b, Penalty pen
             for (int q = 0; q < 6; ++q) val = (q == (int)lane) ? acc[21 + q] : val;
             b.jtr[6 * (size_t)n + lane] = val;
         }
-        if (lane == 0) spd_inverse6<InvT>(D, b.diag_inv + 36 * (size_t)n);
+        if (lane == 0)
+            spd_inverse6<InvT>(D, b.diag_inv + 36 * (size_t)n,
+                               sizeof(InvT) == 8 ? b.diag_inv64 + 36 * (size_t)n : nullptr);
     }
 }
 
@@ -346,11 +387,16 @@ __global__ void __launch_bounds__(256) pair_blocks_kernel(Buffers b, Penalty pen
             const uint32_t key = b.pair_key[p];
             const uint32_t i = key >> 16, j = key & 0xFFFF;
             const uint2 range = b.pair_range[p];
+            const float4 gi = b.frame->nodes[i], gj = b.frame->nodes[j];
             for (uint32_t q = range.x + sub; q < range.y; q += kPairLanes) {
                 const uint32_t t = b.pair_terms[q];
                 float Ji[4][6], Jj[4][6], ri[4], rj[4], li[4], lj[4];
                 const int rows = term_rows(b, c, pen, t, i, Ji, ri, li);
                 term_rows(b, c, pen, t, j, Jj, rj, lj);
+                for (int ch = 0; ch < rows; ++ch) {
+                    to_node_frame(Ji[ch], gi);
+                    to_node_frame(Jj[ch], gj);
+                }
                 for (int ch = 0; ch < rows; ++ch)
 #pragma unroll
                     for (int a = 0; a < 6; ++a)
@@ -407,18 +453,8 @@ __global__ void __launch_bounds__(256) cost_kernel(Buffers b, Penalty pen, int d
     __shared__ double part[4];
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        b.partial[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
-        __threadfence();
-        const uint32_t ticket = atomicAdd(&b.ticket[1], 1u);
-        if (ticket == gridDim.x - 1) {
-            __threadfence();
-            double s = 0.0;
-            for (uint32_t q = 0; q < gridDim.x; ++q) s += b.partial[q];
-            *b.cost = s;
-            b.ticket[1] = 0;
-        }
-    }
+    // one partial per block; the host sums them in block order
+    if (threadIdx.x == 0) b.partial[(size_t)kSlotCost * kPartialStride + blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
 // ------------------------------------------------------------- host launch
@@ -426,18 +462,29 @@ void launch_eval_terms(const Buffers& b, const DQ* se3, hipStream_t stream) {
     hipLaunchKernelGGL(eval_terms_kernel, dim3(1024), dim3(256), 0, stream, b, se3);
 }
 
-void launch_assemble(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream) {
+void launch_node_blocks(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream) {
     const uint32_t node_blocks = (cap_nodes + 3) / 4;
     if (fp64_inverse)
         hipLaunchKernelGGL(node_blocks_kernel<double>, dim3(node_blocks), dim3(256), 0, stream, b, pen);
     else
         hipLaunchKernelGGL(node_blocks_kernel<float>, dim3(node_blocks), dim3(256), 0, stream, b, pen);
+}
+
+void launch_pair_blocks(const Buffers& b, const Penalty& pen, hipStream_t stream) {
     const uint32_t pair_blocks = std::min<uint32_t>((b.cap_pairs + 15) / 16, 4096u);
     hipLaunchKernelGGL(pair_blocks_kernel, dim3(pair_blocks), dim3(256), 0, stream, b, pen);
 }
 
-void launch_cost(const Buffers& b, const Penalty& pen, bool density_in_jtj, hipStream_t stream) {
-    hipLaunchKernelGGL(cost_kernel, dim3(256), dim3(256), 0, stream, b, pen, density_in_jtj ? 1 : 0);
+void launch_assemble(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream) {
+    launch_node_blocks(b, pen, fp64_inverse, cap_nodes, stream);
+    launch_pair_blocks(b, pen, stream);
+}
+
+// Returns the number of partials written to slot kSlotCost.
+uint32_t launch_cost(const Buffers& b, const Penalty& pen, bool density_in_jtj, hipStream_t stream) {
+    const uint32_t blocks = 256;
+    hipLaunchKernelGGL(cost_kernel, dim3(blocks), dim3(256), 0, stream, b, pen, density_in_jtj ? 1 : 0);
+    return blocks;
 }
 
 }  // namespace mvdr

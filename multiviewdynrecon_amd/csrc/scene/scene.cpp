@@ -13,6 +13,7 @@
 #include "../../../include/mvdr/scene.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -382,6 +383,9 @@ void splat(const std::vector<V3>& pos, const std::vector<V3>& nrm, const std::ve
                 if (std::fabs(den) < 0.15) continue;
                 double t = dot(n, p) / den;
                 if (t <= 0.1 || std::fabs(t - p.z) > 3 * radius[s] + 0.01) continue;
+                // disc footprint: the hit must lie within the surfel's radius
+                const V3 hit = d * t - p;
+                if (dot(hit, hit) > radius[s] * radius[s]) continue;
                 size_t o = (size_t)y * W + x;
                 if (t < rt.depth[o]) { rt.depth[o] = t; rt.id[o] = (int32_t)s; rt.vtx[o] = d * t; }
             }
@@ -552,18 +556,82 @@ int mvdr_scene_create(const mvdr_scene_params* params, mvdr_scene** out) {
     // live (ground truth) and initial-warp positions
     std::vector<V3> live_p(S), live_n(S), init_p(S), init_n(S);
     std::vector<double> radius(S);
+    // ground-truth node dual quaternions in double for the blend below
+    std::vector<std::array<double, 8>> gt_dq(N);
+    for (uint32_t i = 0; i < N; ++i) {
+        double q0[4];
+        quat_from_rot(gt[i], q0);
+        const double tq[4] = {0.5 * gt[i].t.x, 0.5 * gt[i].t.y, 0.5 * gt[i].t.z, 0.0};
+        double q1[4];
+        qmul(tq, q0, q1);
+        for (int k = 0; k < 4; ++k) { gt_dq[i][k] = q0[k]; gt_dq[i][4 + k] = q1[k]; }
+    }
     for (uint32_t s = 0; s < S; ++s) {
-        // blend the four node transforms linearly (small motions: adequate for synthesis)
-        Rigid b; for (double& x : b.R) x = 0; b.t = V3();
+        // dual-quaternion blend, the warp model of the solver (averageDualQuaternion
+        // then normalise and convert, DualQuaternion.h:281-335)
+        double d[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int k = 0; k < 4; ++k) {
-            const Rigid& g = gt[sc->surfel_knn[4 * s + k]];
-            double w = sc->surfel_w[4 * s + k];
-            for (int j = 0; j < 9; ++j) b.R[j] += w * g.R[j];
-            b.t += g.t * w;
+            const auto& g = gt_dq[sc->surfel_knn[4 * s + k]];
+            const double w = sc->surfel_w[4 * s + k];
+            for (int j = 0; j < 8; ++j) d[j] += w * g[j];
+        }
+        const double inv = 1.0 / std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + d[3] * d[3]);
+        for (double& x : d) x *= inv;
+        const double dd = d[0] * d[4] + d[1] * d[5] + d[2] * d[6] + d[3] * d[7];
+        for (int j = 0; j < 4; ++j) d[4 + j] -= dd * d[j];
+        Rigid b;
+        {
+            const double x = d[0], y = d[1], z = d[2], w = d[3];
+            b.R[0] = 1 - 2 * y * y - 2 * z * z; b.R[1] = 2 * x * y - 2 * z * w;     b.R[2] = 2 * x * z + 2 * y * w;
+            b.R[3] = 2 * x * y + 2 * z * w;     b.R[4] = 1 - 2 * x * x - 2 * z * z; b.R[5] = 2 * y * z - 2 * x * w;
+            b.R[6] = 2 * x * z - 2 * y * w;     b.R[7] = 2 * y * z + 2 * x * w;     b.R[8] = 1 - 2 * x * x - 2 * y * y;
+            const double a[4] = {2 * d[4], 2 * d[5], 2 * d[6], 2 * d[7]};
+            const double c[4] = {-x, -y, -z, w};
+            double t[4];
+            qmul(a, c, t);
+            b.t = V3(t[0], t[1], t[2]);
         }
         const Surfel& sf = sc->surfels[s];
         live_p[s] = W2C.apply(b.apply(sf.p));
-        live_n[s] = normalized(W2C.rot(b.rot(sf.n)));
+        // normal of the deformed surface: finite-difference tangents of the warp
+        // with the skinning weights re-evaluated along the surface (the rotated
+        // canonical normal misses the warp's stretch and shear)
+        {
+            const uint16_t* kn = &sc->surfel_knn[4 * (size_t)s];
+            auto warp_at = [&](const V3& x) {
+                double wk[4], ws = 0, dsum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                for (int k = 0; k < 4; ++k) {
+                    const V3 dv = x - node_pos[kn[k]];
+                    wk[k] = std::exp(-dot(dv, dv) / (2 * node_r2));
+                    ws += wk[k];
+                }
+                for (int k = 0; k < 4; ++k)
+                    for (int j = 0; j < 8; ++j) dsum[j] += (wk[k] / ws) * gt_dq[kn[k]][j];
+                const double iv = 1.0 / std::sqrt(dsum[0] * dsum[0] + dsum[1] * dsum[1] + dsum[2] * dsum[2] + dsum[3] * dsum[3]);
+                for (double& v : dsum) v *= iv;
+                const double pr = dsum[0] * dsum[4] + dsum[1] * dsum[5] + dsum[2] * dsum[6] + dsum[3] * dsum[7];
+                for (int j = 0; j < 4; ++j) dsum[4 + j] -= pr * dsum[j];
+                Rigid m;
+                const double x0 = dsum[0], y0 = dsum[1], z0 = dsum[2], w0 = dsum[3];
+                m.R[0] = 1 - 2 * y0 * y0 - 2 * z0 * z0; m.R[1] = 2 * x0 * y0 - 2 * z0 * w0; m.R[2] = 2 * x0 * z0 + 2 * y0 * w0;
+                m.R[3] = 2 * x0 * y0 + 2 * z0 * w0; m.R[4] = 1 - 2 * x0 * x0 - 2 * z0 * z0; m.R[5] = 2 * y0 * z0 - 2 * x0 * w0;
+                m.R[6] = 2 * x0 * z0 - 2 * y0 * w0; m.R[7] = 2 * y0 * z0 + 2 * x0 * w0; m.R[8] = 1 - 2 * x0 * x0 - 2 * y0 * y0;
+                const double a2[4] = {2 * dsum[4], 2 * dsum[5], 2 * dsum[6], 2 * dsum[7]};
+                const double c2[4] = {-x0, -y0, -z0, w0};
+                double tt[4];
+                qmul(a2, c2, tt);
+                m.t = V3(tt[0], tt[1], tt[2]);
+                return m.apply(x);
+            };
+            const V3 helper = std::fabs(sf.n.x) < 0.9 ? V3(1, 0, 0) : V3(0, 1, 0);
+            const V3 t1 = normalized(cross(sf.n, helper)), t2 = cross(sf.n, t1);
+            const double hs = 1e-3;
+            const V3 d1 = warp_at(sf.p + t1 * hs) - warp_at(sf.p - t1 * hs);
+            const V3 d2 = warp_at(sf.p + t2 * hs) - warp_at(sf.p - t2 * hs);
+            V3 nl = normalized(cross(d1, d2));
+            if (dot(nl, b.rot(sf.n)) < 0) nl = nl * -1.0;
+            live_n[s] = normalized(W2C.rot(nl));
+        }
         init_p[s] = W2C.apply(sf.p);
         init_n[s] = W2C.rot(sf.n);
         radius[s] = sf.radius;

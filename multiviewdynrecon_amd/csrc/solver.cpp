@@ -16,7 +16,10 @@ void launch_index_build(const Buffers& b, uint32_t cap_tiles, uint32_t cap_nodes
                         hipStream_t stream);
 void launch_eval_terms(const Buffers& b, const DQ* se3, hipStream_t stream);
 void launch_assemble(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream);
-void launch_cost(const Buffers& b, const Penalty& pen, bool density_in_jtj, hipStream_t stream);
+void launch_node_blocks(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream);
+void launch_pair_blocks(const Buffers& b, const Penalty& pen, hipStream_t stream);
+void launch_spmv_once(const Buffers& b, bool fp64, uint32_t cap_nodes, hipStream_t stream);
+uint32_t launch_cost(const Buffers& b, const Penalty& pen, bool density_in_jtj, hipStream_t stream);
 void launch_pcg(const Buffers& b, uint32_t iters, uint32_t refine, bool fp64, uint32_t cap_nodes, hipStream_t stream);
 void launch_apply_update(const Buffers& b, const DQ* in, DQ* out, hipStream_t stream);
 void launch_copy_se3(const Buffers& b, DQ* dst, hipStream_t stream);
@@ -120,6 +123,79 @@ void enqueue_solve(mvdr_solver* s, hipStream_t stream, bool index, uint32_t firs
         launch_pcg(s->b, c.pcg_iterations, fp64 ? c.refine_steps : 0, fp64, c.max_nodes, stream);
         launch_apply_update(s->b, cur, next, stream);
     }
+}
+
+// Node-centred <-> reference coordinates (see to_node_frame in kernels_solve.hip):
+// x_ref = T x_node with T = [[I, 0], [[g]x, I]].  Row-major 6x6 in double.
+void node_transform(const float4& g, bool inverse, double T[36]) {
+    const double s = inverse ? -1.0 : 1.0;
+    for (int q = 0; q < 36; ++q) T[q] = (q % 7 == 0) ? 1.0 : 0.0;
+    const double G[3][3] = {{0, -g.z, g.y}, {g.z, 0, -g.x}, {-g.y, g.x, 0}};
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) T[6 * (3 + r) + c] = s * G[r][c];
+}
+
+// out = X^T M Y for row-major 6x6 matrices, M / out in float.
+void xt_m_y(const double X[36], const float* M, const double Y[36], float* out) {
+    double tmp[36];
+    for (int p = 0; p < 6; ++p)
+        for (int k = 0; k < 6; ++k) {
+            double acc = 0;
+            for (int q = 0; q < 6; ++q) acc += (double)M[6 * p + q] * Y[6 * q + k];
+            tmp[6 * p + k] = acc;
+        }
+    for (int a = 0; a < 6; ++a)
+        for (int k = 0; k < 6; ++k) {
+            double acc = 0;
+            for (int p = 0; p < 6; ++p) acc += X[6 * p + a] * tmp[6 * p + k];
+            out[6 * a + k] = (float)acc;
+        }
+}
+
+void transpose6(const double A[36], double At[36]) {
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) At[6 * c + r] = A[6 * r + c];
+}
+
+// Map the node-centred system buffers copied from the device back to the
+// reference coordinates of JtJDiagonalBlockValue / JtResidualValue.
+mvdr_status to_reference_frame(mvdr_solver* s, int tap, float* data, uint32_t pairs) {
+    const uint32_t N = s->frame_host.N;
+    std::vector<float4> g(N);
+    HIP_TRY(s, hipMemcpy(g.data(), s->frame_host.nodes, sizeof(float4) * N, hipMemcpyDeviceToHost));
+    double Ti[36], Tinv[36], Tt[36];
+    if (tap == MVDR_TAP_JTJ_DIAG || tap == MVDR_TAP_JTJ_DIAG_INV || tap == MVDR_TAP_JTR) {
+        for (uint32_t n = 0; n < N; ++n) {
+            node_transform(g[n], true, Tinv);
+            if (tap == MVDR_TAP_JTJ_DIAG) {
+                xt_m_y(Tinv, data + 36 * (size_t)n, Tinv, data + 36 * (size_t)n);  // T^-T D' T^-1
+            } else if (tap == MVDR_TAP_JTJ_DIAG_INV) {
+                node_transform(g[n], false, Ti);
+                transpose6(Ti, Tt);
+                xt_m_y(Tt, data + 36 * (size_t)n, Tt, data + 36 * (size_t)n);      // T inv(D') T^T
+            } else {
+                float* bn = data + 6 * (size_t)n;
+                double out[6];
+                for (int a = 0; a < 6; ++a) {
+                    double acc = 0;
+                    for (int p = 0; p < 6; ++p) acc += Tinv[6 * p + a] * bn[p];   // T^-T b'
+                    out[a] = acc;
+                }
+                for (int a = 0; a < 6; ++a) bn[a] = (float)out[a];
+            }
+        }
+        return MVDR_OK;
+    }
+    // off-diagonal: B = T_i^-T B' T_j^-1
+    std::vector<uint32_t> key(pairs);
+    if (pairs) HIP_TRY(s, hipMemcpy(key.data(), s->b.pair_key, 4ull * pairs, hipMemcpyDeviceToHost));
+    double Tj[36];
+    for (uint32_t p = 0; p < pairs; ++p) {
+        node_transform(g[key[p] >> 16], true, Tinv);
+        node_transform(g[key[p] & 0xFFFF], true, Tj);
+        xt_m_y(Tinv, data + 36 * (size_t)p, Tj, data + 36 * (size_t)p);
+    }
+    return MVDR_OK;
 }
 
 mvdr_status check_counts(mvdr_solver* s, Counts& out) {
@@ -237,18 +313,16 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     A(b.low_ent, b.cap_pairs);
     A(b.diag, 36 * (size_t)c.max_nodes);
     A(b.diag_inv, 36 * (size_t)c.max_nodes);
+    b.diag_inv64 = nullptr;
+    if (c.precision == MVDR_PRECISION_MIXED) A(b.diag_inv64, 36 * (size_t)c.max_nodes);
     A(b.jtr, 6 * (size_t)c.max_nodes);
     A(b.offdiag, 36 * (size_t)b.cap_pairs);
     for (float** v : {&b.x, &b.r, &b.s, &b.q, &b.d0, &b.d1}) A(*v, 6 * (size_t)c.max_nodes);
-    A(b.partial, 4096);
-    A(b.scalars, 16);
-    A(b.ticket, 4);
+    A(b.partial, (size_t)kPartialSlots * kPartialStride);
     A(b.se3[0], c.max_nodes);
     A(b.se3[1], c.max_nodes);
-    A(b.cost, 1);
-    if (e == hipSuccess) e = hipMemset(b.ticket, 0, 4 * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(b.counts, 0, sizeof(Counts));
-    if (e == hipSuccess) e = hipMemset(b.scalars, 0, 16 * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(b.partial, 0, (size_t)kPartialSlots * kPartialStride * sizeof(double));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->done, hipEventDisableTiming);
     if (e != hipSuccess) {
         std::fprintf(stderr, "mvdr_solver_create: %s\n", hipGetErrorString(e));
@@ -417,14 +491,19 @@ mvdr_status mvdr_solver_total_residual(mvdr_solver* s, void* stream, double* cos
     launch_eval_terms(s->b, s->b.se3[s->iter & 1], st);
     Penalty kp = pen;
     if (!s->cfg.density_in_jtj) kp.density = 0.f;
-    launch_cost(s->b, kp, s->cfg.density_in_jtj != 0, st);
+    const uint32_t parts = launch_cost(s->b, kp, s->cfg.density_in_jtj != 0, st);
     HIP_TRY(s, hipGetLastError());
     HIP_TRY(s, hipEventRecord(s->done, st));
     HIP_TRY(s, hipStreamSynchronize(st));
     Counts cnt;
     mvdr_status rc = check_counts(s, cnt);
     if (rc != MVDR_OK) return rc;
-    HIP_TRY(s, hipMemcpy(cost, s->b.cost, sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> partials(parts);
+    HIP_TRY(s, hipMemcpy(partials.data(), s->b.partial + (size_t)kSlotCost * kPartialStride, parts * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    double total = 0.0;
+    for (double p : partials) total += p;
+    *cost = total;
     return MVDR_OK;
 }
 
@@ -499,7 +578,49 @@ mvdr_status mvdr_solver_debug_get(mvdr_solver* s, int tap, void* dst, size_t byt
     if (needed) *needed = size;
     if (size && (!dst || bytes < size)) return fail(s, MVDR_E_CAPACITY, "buffer too small");
     if (size) HIP_TRY(s, hipMemcpy(dst, src, size, hipMemcpyDeviceToHost));
+    if (size && (tap == MVDR_TAP_JTJ_DIAG || tap == MVDR_TAP_JTJ_DIAG_INV || tap == MVDR_TAP_JTR ||
+                 tap == MVDR_TAP_JTJ_OFFDIAG)) {
+        const mvdr_status rc = to_reference_frame(s, tap, static_cast<float*>(dst), c.pairs);
+        if (rc != MVDR_OK) return rc;
+    }
     if (c.error) return fail(s, MVDR_E_CAPACITY, error_text(c.error));
+    return MVDR_OK;
+}
+
+mvdr_status mvdr_solver_bench_stage(mvdr_solver* s, int stage, uint32_t reps, void* stream, float* avg_ms) {
+    if (!s || !avg_ms || reps == 0) return MVDR_E_INVALID;
+    if (!s->indexed || s->iter == 0) return fail(s, MVDR_E_STATE, "bench_stage needs a completed solve");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const mvdr_solver_config& c = s->cfg;
+    const bool fp64 = c.precision == MVDR_PRECISION_MIXED;
+    const uint32_t k = s->iter - 1;
+    const Penalty pen = kernel_penalty(c, k);
+    hipEvent_t e0, e1;
+    HIP_TRY(s, hipEventCreate(&e0));
+    HIP_TRY(s, hipEventCreate(&e1));
+    HIP_TRY(s, hipEventRecord(e0, st));
+    for (uint32_t r = 0; r < reps; ++r) {
+        switch (stage) {
+            case MVDR_STAGE_EVAL_TERMS: launch_eval_terms(s->b, s->b.se3[k & 1], st); break;
+            case MVDR_STAGE_NODE_BLOCKS: launch_node_blocks(s->b, pen, fp64, c.max_nodes, st); break;
+            case MVDR_STAGE_PAIR_BLOCKS: launch_pair_blocks(s->b, pen, st); break;
+            case MVDR_STAGE_SPMV: launch_spmv_once(s->b, fp64, c.max_nodes, st); break;
+            case MVDR_STAGE_INDEX: enqueue_solve(s, st, true, 0, 0); break;
+            default:
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                return fail(s, MVDR_E_INVALID, "unknown stage");
+        }
+    }
+    HIP_TRY(s, hipEventRecord(e1, st));
+    HIP_TRY(s, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(s, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_TRY(s, hipGetLastError());
+    HIP_TRY(s, hipEventRecord(s->done, st));
+    *avg_ms = ms / (float)reps;
     return MVDR_OK;
 }
 

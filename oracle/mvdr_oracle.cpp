@@ -1334,6 +1334,46 @@ long long mvdr_oracle_get(mvdr_oracle* h, const char* name, void* dst, size_t by
     });
 }
 
+// ---- math hooks, FP64, for pinning the primitives against closed forms ----
+static void put_m34(const M34<double>& m, double out[12]) {
+    for (int c = 0; c < 3; ++c) { out[3 * c] = m.c[c].x; out[3 * c + 1] = m.c[c].y; out[3 * c + 2] = m.c[c].z; }
+    out[9] = m.t.x; out[10] = m.t.y; out[11] = m.t.z;
+}
+static M34<double> get_m34(const double in[12]) {
+    M34<double> m;
+    for (int c = 0; c < 3; ++c) m.c[c] = v3<double>(in[3 * c], in[3 * c + 1], in[3 * c + 2]);
+    m.t = v3<double>(in[9], in[10], in[11]);
+    return m;
+}
+static DQ<double> get_dq(const double d[8]) { return {{d[0], d[1], d[2], d[3]}, {d[4], d[5], d[6], d[7]}}; }
+static void put_dq(const DQ<double>& d, double out[8]) {
+    out[0] = d.q0.x; out[1] = d.q0.y; out[2] = d.q0.z; out[3] = d.q0.w;
+    out[4] = d.q1.x; out[5] = d.q1.y; out[6] = d.q1.z; out[7] = d.q1.w;
+}
+
+// mode 0: DualQuaternion::se3_matrix; mode 1: operator mat34.  Output mat34 as
+// 9 column-major rotation entries + 3 translation entries.
+void mvdr_oracle_dq_to_mat34(const double dq[8], int mode, double out[12]) {
+    const DQ<double> d = get_dq(dq);
+    put_m34(mode ? d.to_mat34() : d.se3_matrix(), out);
+}
+void mvdr_oracle_mat34_to_dq(const double m[12], double dq[8]) { put_dq(DQ<double>::from_mat34(get_m34(m)), dq); }
+void mvdr_oracle_twist_to_mat34(const double twist[6], double out[12]) {
+    put_m34(twist_to_se3(v3<double>(twist[0], twist[1], twist[2]), v3<double>(twist[3], twist[4], twist[5])), out);
+}
+// weighted blend of four dual quaternions (4 x 8 doubles) with knn 0..3
+void mvdr_oracle_blend(const double dqs[32], const double w[4], double out[8]) {
+    std::vector<DQ<double>> f(4);
+    for (int k = 0; k < 4; ++k) f[k] = get_dq(dqs + 8 * k);
+    const uint16_t knn[4] = {0, 1, 2, 3};
+    put_dq(blend(f, knn, w), out);
+}
+void mvdr_oracle_inverse6(const double A[36], double out[36]) { Oracle<double>::gauss_jordan6(A, out); }
+void mvdr_oracle_penalty(const mvdr_solver_config* cfg, uint32_t iter, double out[5]) {
+    const Penalty p = penalty_for(*cfg, iter);
+    out[0] = p.smooth; out[1] = p.density; out[2] = p.foreground; out[3] = p.feature; out[4] = p.cross;
+}
+
 int mvdr_oracle_set(mvdr_oracle* h, const char* name, const double* src, size_t count) {
     const std::string n(name ? name : "");
     return with<float>(h, [&](auto& o) -> int {
