@@ -55,7 +55,8 @@ typedef enum {
 
 typedef enum {
     MVDR_PRECISION_FP32 = 0,   /* everything FP32 (reference precision)          */
-    MVDR_PRECISION_MIXED = 1   /* FP64 dots / alpha / beta / 6x6 inverses        */
+    MVDR_PRECISION_MIXED = 1   /* FP64 term geometry (J, r stored FP32), FP64
+                                  dots / alpha / beta / 6x6 inverses / SpMV sums */
 } mvdr_precision;
 
 typedef enum {

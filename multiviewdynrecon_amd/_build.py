@@ -68,7 +68,8 @@ def build_oracle(verbose=False):
     odir = os.path.join(ROOT, "oracle")
     out = os.path.join(odir, "libmvdr_oracle.so")
     tmp = out + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-I", INCLUDE, "-o", tmp,
+    # no FMA contraction: selection thresholds round like the device's
+    _run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-shared", "-fPIC", "-I", INCLUDE, "-o", tmp,
           os.path.join(odir, "mvdr_oracle.cpp")])
     os.replace(tmp, out)
     if verbose:

@@ -20,29 +20,44 @@ constexpr uint32_t kPartialSlots = 160;
 constexpr uint32_t kSlotCost = 150;        // partials of the total residual
 
 // ---------------------------------------------------------------- vectors
-struct f3 {
-    float x, y, z;
+// V3<float> / M34<float> carry the reference's FP32 arithmetic; the mixed
+// precision mode instantiates the same code with double.
+template <typename R>
+struct V3 {
+    R x, y, z;
 };
+using f3 = V3<float>;
 __host__ __device__ inline f3 mk(float a, float b, float c) { return {a, b, c}; }
-__host__ __device__ inline f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-__host__ __device__ inline f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <typename R> __host__ __device__ inline V3<R> operator+(V3<R> a, V3<R> b) {
+    return {a.x + b.x, a.y + b.y, a.z + b.z};
+}
+template <typename R> __host__ __device__ inline V3<R> operator-(V3<R> a, V3<R> b) {
+    return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
 __host__ __device__ inline f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
-__host__ __device__ inline float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__host__ __device__ inline f3 cross3(f3 a, f3 b) {
+template <typename R> __host__ __device__ inline R dot3(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename R> __host__ __device__ inline V3<R> cross3(V3<R> a, V3<R> b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 __host__ __device__ inline f3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+template <typename R> __host__ __device__ inline V3<R> widen(f3 v) { return {(R)v.x, (R)v.y, (R)v.z}; }
+template <typename R> __host__ __device__ inline V3<R> xyz_as(float4 v) { return {(R)v.x, (R)v.y, (R)v.z}; }
 
 // Rigid transform, rotation as columns c0, c1, c2.
-struct Mat34 {
-    f3 c0, c1, c2, t;
-    __host__ __device__ f3 rot(f3 p) const {
+template <typename R>
+struct M34 {
+    V3<R> c0, c1, c2, t;
+    __host__ __device__ V3<R> rot(V3<R> p) const {
         return {c0.x * p.x + c1.x * p.y + c2.x * p.z, c0.y * p.x + c1.y * p.y + c2.y * p.z,
                 c0.z * p.x + c1.z * p.y + c2.z * p.z};
     }
-    __host__ __device__ f3 rot_t(f3 p) const { return {dot3(c0, p), dot3(c1, p), dot3(c2, p)}; }
-    __host__ __device__ f3 apply(f3 p) const { return rot(p) + t; }
+    __host__ __device__ V3<R> rot_t(V3<R> p) const { return {dot3(c0, p), dot3(c1, p), dot3(c2, p)}; }
+    __host__ __device__ V3<R> apply(V3<R> p) const { return rot(p) + t; }
 };
+using Mat34 = M34<float>;
+template <typename R> __host__ __device__ inline M34<R> widen(const Mat34& m) {
+    return {widen<R>(m.c0), widen<R>(m.c1), widen<R>(m.c2), widen<R>(m.t)};
+}
 
 inline Mat34 to_mat34(const mvdr_mat34& m) {
     Mat34 o;
@@ -54,13 +69,13 @@ inline Mat34 to_mat34(const mvdr_mat34& m) {
 }
 
 // mat34::inverse (MatUtils.h:585-590): R^T, -R^T t
-inline Mat34 inverse(const Mat34& m) {
-    Mat34 o;
-    o.c0 = mk(m.c0.x, m.c1.x, m.c2.x);
-    o.c1 = mk(m.c0.y, m.c1.y, m.c2.y);
-    o.c2 = mk(m.c0.z, m.c1.z, m.c2.z);
-    f3 rt = o.rot(m.t);
-    o.t = mk(-rt.x, -rt.y, -rt.z);
+template <typename R> __host__ __device__ inline M34<R> inverse(const M34<R>& m) {
+    M34<R> o;
+    o.c0 = {m.c0.x, m.c1.x, m.c2.x};
+    o.c1 = {m.c0.y, m.c1.y, m.c2.y};
+    o.c2 = {m.c0.z, m.c1.z, m.c2.z};
+    const V3<R> rt = o.rot(m.t);
+    o.t = {-rt.x, -rt.y, -rt.z};
     return o;
 }
 
@@ -111,9 +126,72 @@ __device__ inline Mat34 se3_of(DQ d) {
     return m;
 }
 
+// The same blend and normalisation in R: the mixed precision mode evaluates
+// term geometry in double from the stored FP32 dual quaternions.
+template <typename R>
+struct QR {
+    R x, y, z, w;
+};
+template <typename R>
+struct DQR {
+    QR<R> q0, q1;
+};
+template <typename R> __device__ inline QR<R> widen(float4 q) { return {(R)q.x, (R)q.y, (R)q.z, (R)q.w}; }
+template <typename R> __device__ inline QR<R> qmul(QR<R> a, QR<R> b) {
+    QR<R> q;
+    q.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
+    q.x = a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y;
+    q.y = a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x;
+    q.z = a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w;
+    return q;
+}
+template <typename R>
+__device__ inline DQR<R> blend4_as(const DQ* __restrict__ field, ushort4 knn, float4 w) {
+    const uint32_t id[4] = {knn.x, knn.y, knn.z, knn.w};
+    const R wk[4] = {(R)w.x, (R)w.y, (R)w.z, (R)w.w};
+    DQR<R> o;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const QR<R> a = widen<R>(field[id[k]].q0), b = widen<R>(field[id[k]].q1);
+        if (k == 0) {
+            o.q0 = {wk[0] * a.x, wk[0] * a.y, wk[0] * a.z, wk[0] * a.w};
+            o.q1 = {wk[0] * b.x, wk[0] * b.y, wk[0] * b.z, wk[0] * b.w};
+        } else {
+            o.q0 = {o.q0.x + wk[k] * a.x, o.q0.y + wk[k] * a.y, o.q0.z + wk[k] * a.z, o.q0.w + wk[k] * a.w};
+            o.q1 = {o.q1.x + wk[k] * b.x, o.q1.y + wk[k] * b.y, o.q1.z + wk[k] * b.z, o.q1.w + wk[k] * b.w};
+        }
+    }
+    return o;
+}
+template <typename R> __device__ inline DQR<R> widen(const DQ& d) { return {widen<R>(d.q0), widen<R>(d.q1)}; }
+template <typename R> __device__ inline M34<R> se3_as(DQR<R> d) {
+    const R inv = R(1) / sqrt(d.q0.w * d.q0.w + d.q0.x * d.q0.x + d.q0.y * d.q0.y + d.q0.z * d.q0.z);
+    const QR<R> a = {inv * d.q0.x, inv * d.q0.y, inv * d.q0.z, inv * d.q0.w};
+    QR<R> b = {inv * d.q1.x, inv * d.q1.y, inv * d.q1.z, inv * d.q1.w};
+    const R dd = a.w * b.w + a.x * b.x + a.y * b.y + a.z * b.z;
+    b = {b.x - dd * a.x, b.y - dd * a.y, b.z - dd * a.z, b.w - dd * a.w};
+    const R one = 1, two = 2;
+    M34<R> m;
+    m.c0 = {one - two * a.y * a.y - two * a.z * a.z, two * a.x * a.y + two * a.z * a.w, two * a.x * a.z - two * a.y * a.w};
+    m.c1 = {two * a.x * a.y - two * a.z * a.w, one - two * a.x * a.x - two * a.z * a.z, two * a.y * a.z + two * a.x * a.w};
+    m.c2 = {two * a.x * a.z + two * a.y * a.w, two * a.y * a.z - two * a.x * a.w, one - two * a.x * a.x - two * a.y * a.y};
+    const QR<R> tq = qmul<R>({two * b.x, two * b.y, two * b.z, two * b.w}, {-a.x, -a.y, -a.z, a.w});
+    m.t = {tq.x, tq.y, tq.z};
+    return m;
+}
+// Warp of a knn-blended point: FP32 exactly as the reference, or double.
+template <typename R> __device__ inline M34<R> warp_of(const DQ* __restrict__ field, ushort4 knn, float4 w) {
+    if constexpr (sizeof(R) == sizeof(float)) return se3_of(blend4(field, knn, w));
+    else return se3_as<R>(blend4_as<R>(field, knn, w));
+}
+template <typename R> __device__ inline M34<R> node_warp(const DQ& d) {
+    if constexpr (sizeof(R) == sizeof(float)) return se3_of(d);
+    else return se3_as<R>(widen<R>(d));
+}
+
 // __float2int_rn applied to the reference's projection, which evaluates
 // x / (z + 1e-10) * f + c in double and narrows to float first.
-__device__ inline int project_round(float x, float z, float f, float c) {
+template <typename R> __device__ inline int project_round(R x, R z, float f, float c) {
     const float u = (float)((double)x / ((double)z + 1e-10) * (double)f + (double)c);
     return (int)rintf(u);
 }
@@ -188,11 +266,15 @@ struct ScalarRec {      // dense depth, density, foreground
     float J[6];         // rotation xyz, translation xyz
     float pad1[2];
 };
+// The smooth residual is stored, not the second warped point: the two warped
+// points are ~1 m from the origin while their difference is ~1e-4 m, so their
+// FP32 difference would carry a ~1e-3 relative error.  T_j x_j = Ti_xj - r is
+// exact when both came from FP32 values (Sterbenz).
 struct SmoothRec {
     uint32_t ij;        // i | j << 16
     uint32_t valid;
     float Ti_xj[3];
-    float Tj_xj[3];
+    float r[3];         // T_i x_j - T_j x_j
 };
 struct P2PRec {         // feature, cross view
     ushort4 knn;

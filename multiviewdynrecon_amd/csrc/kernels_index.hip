@@ -5,6 +5,11 @@
 #include "common.h"
 #include "device_util.h"
 
+// Selection decisions (window searches, distance thresholds) are evaluated
+// without FMA contraction so that they round exactly like the same
+// expressions on the host.
+#pragma clang fp contract(off)
+
 namespace mvdr {
 
 constexpr int kTileThreads = 256;

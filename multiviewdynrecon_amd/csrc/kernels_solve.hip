@@ -7,35 +7,46 @@
 namespace mvdr {
 
 // ---------------------------------------------------------- term evaluation
+// R = float is the reference's arithmetic; R = double (mixed precision mode)
+// evaluates the warp, camera chain and projection in FP64 and stores J and r in
+// FP32.  In FP32, p_live carries ~1e-7 m of rounding: enough to move ~0.1% of
+// the projections to a neighbouring depth pixel and to cost ~1e-3 in the twist.
+template <typename R> __device__ inline M34<R> init_inverse(const FrameDev& f, uint32_t v) {
+    if constexpr (sizeof(R) == sizeof(float)) return f.init_inv[v];
+    else return inverse(widen<R>(f.init[v]));
+}
+
 // computeDenseDepthJacobianKernel (DenseDepthJacobianResidual.cu:24-104).  The
 // camera of a term is the view stored with its compacted pixel.
+template <typename R>
 __device__ inline void eval_dense(const FrameDev& f, const DQ* se3, ushort4 px, ScalarRec& rec) {
     const uint32_t v = px.z;
     const uint32_t o = px.y * f.W + px.x;
-    const f3 cv = xyz(f.ref_v[v][o]), cn = xyz(f.ref_n[v][o]);
-    const Mat34& I = f.init[v];
-    const f3 v0 = I.apply(cv), n0 = I.rot(cn);
-    const Mat34 T = se3_of(blend4(se3, rec.knn, rec.w));
-    const f3 pl = T.apply(v0), nl = T.rot(n0);
-    f3 pc = f.w2c[v].apply(pl), nc = f.w2c[v].rot(nl);
-    pc = f.init_inv[v].apply(pc);
-    nc = f.init_inv[v].rot(nc);
+    const V3<R> cv = xyz_as<R>(f.ref_v[v][o]), cn = xyz_as<R>(f.ref_n[v][o]);
+    const M34<R> I = widen<R>(f.init[v]), W2C = widen<R>(f.w2c[v]), Iinv = init_inverse<R>(f, v);
+    const V3<R> v0 = I.apply(cv), n0 = I.rot(cn);
+    const M34<R> T = warp_of<R>(se3, rec.knn, rec.w);
+    const V3<R> pl = T.apply(v0), nl = T.rot(n0);
+    V3<R> pc = W2C.apply(pl), nc = W2C.rot(nl);
+    pc = Iinv.apply(pc);
+    nc = Iinv.rot(nc);
     float r = 0.f, J[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    const int u = project_round(pc.x, pc.z, f.fx[v], f.cx[v]);
-    const int w = project_round(pc.y, pc.z, f.fy[v], f.cy[v]);
+    const int u = project_round<R>(pc.x, pc.z, f.fx[v], f.cx[v]);
+    const int w = project_round<R>(pc.y, pc.z, f.fy[v], f.cy[v]);
     if (u >= 0 && u < (int)f.W && w >= 0 && w < (int)f.H) {
         const float4 dv = f.vertex[v][w * f.W + u];
         const float4 dn = f.normal[v][w * f.W + u];
-        const f3 d = xyz(dv), n = xyz(dn);
+        const V3<R> d = xyz_as<R>(dv), n = xyz_as<R>(dn);
         bool valid = !is_zero_vertex(dv);
-        if (dot3(n, nc) < 0.7f) valid = false;
-        const f3 diff = d - pc;
-        if (dot3(diff, diff) > 0.075f * 0.075f) valid = false;
+        if (dot3(n, nc) < R(0.7f)) valid = false;
+        const V3<R> diff = d - pc;
+        if (dot3(diff, diff) > R(0.075f) * R(0.075f)) valid = false;
         if (valid) {
-            r = dot3(n, pc - d);
-            const f3 jt = f.w2c[v].rot_t(I.rot(n));
-            const f3 jr = cross3(pl, jt);
-            J[0] = jr.x; J[1] = jr.y; J[2] = jr.z; J[3] = jt.x; J[4] = jt.y; J[5] = jt.z;
+            r = (float)dot3(n, pc - d);
+            const V3<R> jt = W2C.rot_t(I.rot(n));
+            const V3<R> jr = cross3(pl, jt);
+            J[0] = (float)jr.x; J[1] = (float)jr.y; J[2] = (float)jr.z;
+            J[3] = (float)jt.x; J[4] = (float)jt.y; J[5] = (float)jt.z;
         }
     }
     rec.r = r;
@@ -46,37 +57,38 @@ __device__ inline void eval_dense(const FrameDev& f, const DQ* se3, ushort4 px, 
 // computeImageDensityJacobainAndResidual (density_map_jacobian.cuh:12-79) as
 // called for density and foreground terms (DensityForegroundJacobianResidual.cu:41-142).
 // The camera-v reference vertex is warped without the move to camera 0, as there.
+template <typename R>
 __device__ inline void eval_image(const FrameDev& f, const DQ* se3, ushort4 px, bool density, ScalarRec& rec) {
     const uint32_t v = px.z;
     const uint32_t o = px.y * f.W + px.x;
-    const f3 cv = xyz(f.ref_v[v][o]);
-    float geometry = 0.f;
+    const V3<R> cv = xyz_as<R>(f.ref_v[v][o]);
+    R geometry = 0;
     if (density) {
         const float4 rgb = f.rgb[v][o];
-        geometry = 0.299f * rgb.x + 0.587f * rgb.y + 0.114f * rgb.z;
+        geometry = R(0.299f) * rgb.x + R(0.587f) * rgb.y + R(0.114f) * rgb.z;
     }
-    const Mat34 T = se3_of(blend4(se3, rec.knn, rec.w));
-    const f3 pl = T.apply(cv);
-    const f3 pc = f.init_inv[v].apply(f.w2c[v].apply(pl));
+    const M34<R> T = warp_of<R>(se3, rec.knn, rec.w);
+    const V3<R> pl = T.apply(cv);
+    const V3<R> pc = init_inverse<R>(f, v).apply(widen<R>(f.w2c[v]).apply(pl));
     float r = 0.f, J[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    const float fx = f.fx[v], fy = f.fy[v];
-    const int u = project_round(pc.x, pc.z, fx, f.cx[v]);
-    const int w = project_round(pc.y, pc.z, fy, f.cy[v]);
+    const R fx = f.fx[v], fy = f.fy[v];
+    const int u = project_round<R>(pc.x, pc.z, f.fx[v], f.cx[v]);
+    const int w = project_round<R>(pc.y, pc.z, f.fy[v], f.cy[v]);
     if (u >= 0 && u < (int)f.W && w >= 0 && w < (int)f.H) {
         const uint32_t q = w * f.W + u;
-        const float img = density ? f.gray[v][q] : f.ffg[v][q];
+        const R img = density ? f.gray[v][q] : f.ffg[v][q];
         const float2 g = density ? f.gray_grad[v][q] : f.fg_grad[v][q];
-        f3 d;
-        d.x = g.x * fx / pc.z;
-        d.y = g.y * fy / pc.z;
-        d.z = -1.0f / (pc.z * pc.z) * (fx * pc.x * g.x + fy * pc.y * g.y);
-        d = f.init[v].rot(d);
-        d = f.w2c[v].rot_t(d);
-        J[0] = -d.y * pl.z + d.z * pl.y;
-        J[1] = d.x * pl.z - d.z * pl.x;
-        J[2] = -d.x * pl.y + d.y * pl.x;
-        J[3] = d.x; J[4] = d.y; J[5] = d.z;
-        r = img - geometry;
+        V3<R> d;
+        d.x = R(g.x) * fx / pc.z;
+        d.y = R(g.y) * fy / pc.z;
+        d.z = -R(1) / (pc.z * pc.z) * (fx * pc.x * R(g.x) + fy * pc.y * R(g.y));
+        d = widen<R>(f.init[v]).rot(d);
+        d = widen<R>(f.w2c[v]).rot_t(d);
+        J[0] = (float)(-d.y * pl.z + d.z * pl.y);
+        J[1] = (float)(d.x * pl.z - d.z * pl.x);
+        J[2] = (float)(-d.x * pl.y + d.y * pl.x);
+        J[3] = (float)d.x; J[4] = (float)d.y; J[5] = (float)d.z;
+        r = (float)(img - geometry);
     }
     rec.r = r;
 #pragma unroll
@@ -86,6 +98,7 @@ __device__ inline void eval_image(const FrameDev& f, const DQ* se3, ushort4 px, 
 // One thread per global term; smooth terms forward-warp the graph edge
 // (NodeGraphSmoothHandler.cu:10-43), feature / cross terms their canonical
 // vertex (SparseCorrespondenceHandler.cu:193-220).
+template <typename R>
 __global__ void __launch_bounds__(256) eval_terms_kernel(Buffers b, const DQ* __restrict__ se3) {
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
@@ -95,13 +108,13 @@ __global__ void __launch_bounds__(256) eval_terms_kernel(Buffers b, const DQ* __
         if (t < c.off[0]) {
             const ushort4 px = b.dense_px[t];
             ScalarRec rec = b.dense[t];
-            eval_dense(f, se3, px, rec);
+            eval_dense<R>(f, se3, px, rec);
             b.dense[t].r = rec.r;
 #pragma unroll
             for (int k = 0; k < 6; ++k) b.dense[t].J[k] = rec.J[k];
             if (b.density) {
                 ScalarRec d = b.density[t];
-                eval_image(f, se3, px, true, d);
+                eval_image<R>(f, se3, px, true, d);
                 b.density[t].r = d.r;
 #pragma unroll
                 for (int k = 0; k < 6; ++k) b.density[t].J[k] = d.J[k];
@@ -110,19 +123,20 @@ __global__ void __launch_bounds__(256) eval_terms_kernel(Buffers b, const DQ* __
             const uint32_t e = t - c.off[0];
             const ushort2 ij = f.graph[e];
             const float4 xi = f.nodes[ij.x], xj = f.nodes[ij.y];
-            const Mat34 Ti = se3_of(se3[ij.x]), Tj = se3_of(se3[ij.y]);
-            const f3 a = Ti.apply(xyz(xj)), s = Tj.apply(xyz(xj));
+            const M34<R> Ti = node_warp<R>(se3[ij.x]), Tj = node_warp<R>(se3[ij.y]);
+            const V3<R> a = Ti.apply(xyz_as<R>(xj)), s = Tj.apply(xyz_as<R>(xj));
+            const V3<R> r = a - s;
             const f3 d = xyz(xi) - xyz(xj);
             SmoothRec rec;
             rec.ij = (uint32_t)ij.x | ((uint32_t)ij.y << 16);
             rec.valid = dot3(d, d) > 64 * 0.025f * 0.025f ? 0u : 1u;
-            rec.Ti_xj[0] = a.x; rec.Ti_xj[1] = a.y; rec.Ti_xj[2] = a.z;
-            rec.Tj_xj[0] = s.x; rec.Tj_xj[1] = s.y; rec.Tj_xj[2] = s.z;
+            rec.Ti_xj[0] = (float)a.x; rec.Ti_xj[1] = (float)a.y; rec.Ti_xj[2] = (float)a.z;
+            rec.r[0] = (float)r.x; rec.r[1] = (float)r.y; rec.r[2] = (float)r.z;
             b.smooth[e] = rec;
         } else if (t < c.off[2]) {
             const uint32_t k = t - c.off[1];
             ScalarRec rec = b.fg[k];
-            eval_image(f, se3, b.fg_px[k], false, rec);
+            eval_image<R>(f, se3, b.fg_px[k], false, rec);
             b.fg[k].r = rec.r;
 #pragma unroll
             for (int q = 0; q < 6; ++q) b.fg[k].J[q] = rec.J[q];
@@ -131,9 +145,9 @@ __global__ void __launch_bounds__(256) eval_terms_kernel(Buffers b, const DQ* __
             const uint32_t k = is_cross ? t - c.off[3] : t - c.off[2];
             P2PRec* recs = is_cross ? b.cross : b.feat;
             const float4 can = is_cross ? b.cross_canon[k] : b.feat_canon[k];
-            const Mat34 T = se3_of(blend4(se3, recs[k].knn, recs[k].w));
-            const f3 w = T.apply(xyz(can));
-            recs[k].warped = make_float4(w.x, w.y, w.z, 1.0f);
+            const M34<R> T = warp_of<R>(se3, recs[k].knn, recs[k].w);
+            const V3<R> w = T.apply(xyz_as<R>(can));
+            recs[k].warped = make_float4((float)w.x, (float)w.y, (float)w.z, 1.0f);
         }
     }
 }
@@ -172,7 +186,8 @@ __device__ inline void p2p_rows(const P2PRec& t, float J[3][6], float res[3]) {
 // computeSmoothTermJacobian (geometry_icp_jacobian.cuh:323-343) for node i or j.
 __device__ inline void smooth_rows(const SmoothRec& s, bool node_i, float J[3][6], float res[3]) {
     const float sg = node_i ? 1.f : -1.f;
-    const float* a = node_i ? s.Ti_xj : s.Tj_xj;
+    const float a[3] = {node_i ? s.Ti_xj[0] : s.Ti_xj[0] - s.r[0], node_i ? s.Ti_xj[1] : s.Ti_xj[1] - s.r[1],
+                        node_i ? s.Ti_xj[2] : s.Ti_xj[2] - s.r[2]};
     const float on = s.valid ? 1.f : 0.f;
     J[0][0] = 0.f;             J[0][1] = on * sg * a[2];  J[0][2] = -on * sg * a[1];
     J[1][0] = -on * sg * a[2]; J[1][1] = 0.f;             J[1][2] = on * sg * a[0];
@@ -180,9 +195,9 @@ __device__ inline void smooth_rows(const SmoothRec& s, bool node_i, float J[3][6
     J[0][3] = on * sg; J[0][4] = 0.f;     J[0][5] = 0.f;
     J[1][3] = 0.f;     J[1][4] = on * sg; J[1][5] = 0.f;
     J[2][3] = 0.f;     J[2][4] = 0.f;     J[2][5] = on * sg;
-    res[0] = on * (s.Ti_xj[0] - s.Tj_xj[0]);
-    res[1] = on * (s.Ti_xj[1] - s.Tj_xj[1]);
-    res[2] = on * (s.Ti_xj[2] - s.Tj_xj[2]);
+    res[0] = on * s.r[0];
+    res[1] = on * s.r[1];
+    res[2] = on * s.r[2];
 }
 
 // Jacobian rows of term t with respect to node n (knn weight applied),
@@ -458,8 +473,11 @@ __global__ void __launch_bounds__(256) cost_kernel(Buffers b, Penalty pen, int d
 }
 
 // ------------------------------------------------------------- host launch
-void launch_eval_terms(const Buffers& b, const DQ* se3, hipStream_t stream) {
-    hipLaunchKernelGGL(eval_terms_kernel, dim3(1024), dim3(256), 0, stream, b, se3);
+void launch_eval_terms(const Buffers& b, const DQ* se3, bool fp64_geometry, hipStream_t stream) {
+    if (fp64_geometry)
+        hipLaunchKernelGGL(eval_terms_kernel<double>, dim3(1024), dim3(256), 0, stream, b, se3);
+    else
+        hipLaunchKernelGGL(eval_terms_kernel<float>, dim3(1024), dim3(256), 0, stream, b, se3);
 }
 
 void launch_node_blocks(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream) {

@@ -14,7 +14,7 @@
 namespace mvdr {
 void launch_index_build(const Buffers& b, uint32_t cap_tiles, uint32_t cap_nodes, bool use_foreground, bool use_cross,
                         hipStream_t stream);
-void launch_eval_terms(const Buffers& b, const DQ* se3, hipStream_t stream);
+void launch_eval_terms(const Buffers& b, const DQ* se3, bool fp64_geometry, hipStream_t stream);
 void launch_assemble(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream);
 void launch_node_blocks(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream);
 void launch_pair_blocks(const Buffers& b, const Penalty& pen, hipStream_t stream);
@@ -118,7 +118,7 @@ void enqueue_solve(mvdr_solver* s, hipStream_t stream, bool index, uint32_t firs
         const Penalty pen = kernel_penalty(c, k);
         const DQ* cur = s->b.se3[k & 1];
         DQ* next = s->b.se3[(k + 1) & 1];
-        launch_eval_terms(s->b, cur, stream);
+        launch_eval_terms(s->b, cur, fp64, stream);
         launch_assemble(s->b, pen, fp64, c.max_nodes, stream);
         launch_pcg(s->b, c.pcg_iterations, fp64 ? c.refine_steps : 0, fp64, c.max_nodes, stream);
         launch_apply_update(s->b, cur, next, stream);
@@ -488,7 +488,7 @@ mvdr_status mvdr_solver_total_residual(mvdr_solver* s, void* stream, double* cos
     if (!s->indexed) return fail(s, MVDR_E_STATE, "total_residual before build_index");
     hipStream_t st = static_cast<hipStream_t>(stream);
     const Penalty pen = penalty_for(s->cfg, s->iter);
-    launch_eval_terms(s->b, s->b.se3[s->iter & 1], st);
+    launch_eval_terms(s->b, s->b.se3[s->iter & 1], s->cfg.precision == MVDR_PRECISION_MIXED, st);
     Penalty kp = pen;
     if (!s->cfg.density_in_jtj) kp.density = 0.f;
     const uint32_t parts = launch_cost(s->b, kp, s->cfg.density_in_jtj != 0, st);
@@ -601,7 +601,7 @@ mvdr_status mvdr_solver_bench_stage(mvdr_solver* s, int stage, uint32_t reps, vo
     HIP_TRY(s, hipEventRecord(e0, st));
     for (uint32_t r = 0; r < reps; ++r) {
         switch (stage) {
-            case MVDR_STAGE_EVAL_TERMS: launch_eval_terms(s->b, s->b.se3[k & 1], st); break;
+            case MVDR_STAGE_EVAL_TERMS: launch_eval_terms(s->b, s->b.se3[k & 1], s->cfg.precision == MVDR_PRECISION_MIXED, st); break;
             case MVDR_STAGE_NODE_BLOCKS: launch_node_blocks(s->b, pen, fp64, c.max_nodes, st); break;
             case MVDR_STAGE_PAIR_BLOCKS: launch_pair_blocks(s->b, pen, st); break;
             case MVDR_STAGE_SPMV: launch_spmv_once(s->b, fp64, c.max_nodes, st); break;

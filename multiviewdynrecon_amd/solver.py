@@ -161,10 +161,14 @@ def decode_scalar_terms(raw):
 
 
 def decode_smooth_terms(raw):
+    """MVDR_TAP_SMOOTH_TERMS records (8 x u32): i | j << 16, valid, T_i x_j, and the
+    residual r = T_i x_j - T_j x_j; T_j x_j is returned in float64 as T_i x_j - r."""
     raw = np.asarray(raw).reshape(-1, 8)
     f = raw.view(np.float32)
+    ti = f[:, 2:5].astype(np.float64)
+    r = f[:, 5:8].astype(np.float64)
     return {"i": (raw[:, 0] & 0xFFFF).astype(np.uint16), "j": (raw[:, 0] >> 16).astype(np.uint16),
-            "valid": raw[:, 1].astype(np.uint8), "Ti_xj": f[:, 2:5].copy(), "Tj_xj": f[:, 5:8].copy()}
+            "valid": raw[:, 1].astype(np.uint8), "Ti_xj": ti, "r": r, "Tj_xj": ti - r}
 
 
 def decode_p2p_terms(raw):
