@@ -15,6 +15,8 @@ The JSON line carries the contract fields plus:
                 timed on a bounded sample of frames of the same workload
   twist_rel_err max over GN iterations of |dx - dx_fp64| / |dx_fp64| against the
                 FP64 host oracle from the same linearisation point
+  plain_fp32    the plain FP32 GPU path (reference precision, no hipGraph) on
+                the same frame: ms/frame and its twist error, for comparison
 """
 import argparse
 import json
@@ -77,18 +79,21 @@ def stage_time_ms(solver, stage, reps):
     return float(out.value)
 
 
-def cpu_baseline_leg(scene, cfg_kwargs, frames, gpu_solver_factory, frame_dev):
-    """CPU restatement timed on a bounded sample, and the FP64 oracle as the checker
-    of the GPU twist updates.  The oracle is test infrastructure: it is only the
-    timed baseline and the checker here, never part of the measured GPU path."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
+    return Oracle
 
+
+def cpu_baseline_leg(scene, workload, frames):
+    """CPU restatement (oracle/, FP32, one thread) timed on a bounded sample.  The
+    oracle is test infrastructure: here it is only the timed baseline."""
+    Oracle = _oracle()
     from multiviewdynrecon_amd.config import workload_config
     from multiviewdynrecon_amd.frame import HostFrame
 
     hf = HostFrame(scene)
-    cfg32 = workload_config(cfg_kwargs["workload"], precision=0, refine_steps=0)
+    cfg32 = workload_config(workload, precision=0, refine_steps=0)
     times = []
     t_all = time.perf_counter()
     for _ in range(frames):
@@ -100,13 +105,20 @@ def cpu_baseline_leg(scene, cfg_kwargs, frames, gpu_solver_factory, frame_dev):
             break
     cpu = {"value": float(np.median(times) * 1e3), "unit": "ms/frame", "cores": 1, "kind": "port",
            "sample": f"{len(times)} full solves (index + GN x PCG) of the benchmark frame, FP32, single thread, median"}
+    return cpu
 
-    # checker: per-GN-iteration twist error of the GPU path against the FP64 oracle
-    cfg = workload_config(cfg_kwargs["workload"], **cfg_kwargs["overrides"])
-    solver = gpu_solver_factory(cfg)
+
+def twist_errors(scene, cfg, frame_dev):
+    """Per-GN-iteration twist error of the GPU path against the FP64 oracle from
+    the same linearisation point (the oracle is the checker only)."""
+    Oracle = _oracle()
+    from multiviewdynrecon_amd.frame import HostFrame
+    from multiviewdynrecon_amd.solver import NonRigidSolver
+
+    solver = NonRigidSolver(cfg)
     solver.SetSolverInput(frame_dev)
     solver.BuildSolverIndex()
-    o64 = Oracle(cfg, hf, "fp64")
+    o64 = Oracle(cfg, HostFrame(scene), "fp64")
     o64.build_index()
     errs = []
     for k in range(cfg.gn_iterations):
@@ -120,14 +132,39 @@ def cpu_baseline_leg(scene, cfg_kwargs, frames, gpu_solver_factory, frame_dev):
         o64.pcg(cfg.pcg_iterations)
         ref = o64.get("twist").ravel()
         errs.append(float(np.linalg.norm(tw - ref) / max(np.linalg.norm(ref), 1e-30)))
-    return cpu, errs
+    return errs
+
+
+def time_solves(solver, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        solver.SolveNonRigidAlignment()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        solver.SolveNonRigidAlignment()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{WORKLOAD_NAMES[workload]}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
 
 
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build solves on a single GPU; --gpus {args.gpus} is not supported "
-              "(independent frames shard one per GPU, see DESIGN.md)", file=sys.stderr)
+              "(frames are independent and would be sharded one solver per GPU)", file=sys.stderr)
         sys.exit(2)
     import torch
 
@@ -193,12 +230,25 @@ def main():
     spmv_gbs = ab["spmv"] / (stages["spmv"] * 1e-3) / 1e9
     jtj_gbs = (ab["pair_blocks"] + ab["node_blocks"]) / ((stages["pair_blocks"] + stages["node_blocks"]) * 1e-3) / 1e9
 
+    traffic = measured_traffic(dominant, args.workload)
+    roofline["traffic"] = traffic
+
+    # the plain FP32 GPU path on the same frame (reference precision, no graph)
+    plain_cfg = workload_config(args.workload, precision=0, refine_steps=0, use_graph=0)
+    plain = NonRigidSolver(plain_cfg)
+    plain.SetSolverInput(frame)
+    plain_ms = time_solves(plain, max(args.steps // 2, 1), 2)
+    del plain
+    print(f"[bench] plain FP32 path {plain_ms:.3f} ms/frame", file=sys.stderr, flush=True)
+
     cpu = None
-    errs = None
+    errs = plain_errs = None
     if not args.no_cpu_baseline:
-        cpu, errs = cpu_baseline_leg(scene, {"workload": args.workload, "overrides": overrides},
-                                     args.cpu_frames, lambda c: NonRigidSolver(c), frame)
-        print(f"[bench] cpu baseline {cpu['value']:.1f} ms/frame; twist rel err per GN iteration {errs}",
+        cpu = cpu_baseline_leg(scene, args.workload, args.cpu_frames)
+        print(f"[bench] cpu baseline {cpu['value']:.1f} ms/frame", file=sys.stderr, flush=True)
+        errs = twist_errors(scene, cfg, frame)
+        plain_errs = twist_errors(scene, plain_cfg, frame)
+        print(f"[bench] twist rel err per GN iteration: {args.precision} {errs} plain {plain_errs}",
               file=sys.stderr, flush=True)
 
     out = {
@@ -226,6 +276,8 @@ def main():
         "stage_ms": {k: round(v, 5) for k, v in stages.items()},
         "cpu_baseline": cpu,
         "twist_rel_err": max(errs) if errs else None,
+        "plain_fp32": {"ms_per_frame": round(plain_ms, 4), "speedup": round(plain_ms / ms_per_step, 3),
+                       "twist_rel_err": max(plain_errs) if plain_errs else None},
     }
     print(json.dumps(out), flush=True)
 

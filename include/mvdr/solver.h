@@ -153,11 +153,14 @@ typedef enum {
     MVDR_TAP_NODEPAIR_KEY = 2,       /* u32[P]  (i << 16) | j, i < j, ascending          */
     MVDR_TAP_NODEPAIR_RANGE = 3,     /* u32[2P] [start, end) into NODEPAIR_TERM_INDEX    */
     MVDR_TAP_NODEPAIR_TERM_INDEX = 4,/* u32[valid KV2]                                   */
-    MVDR_TAP_DENSE_TERMS = 5,        /* 16 x u32/f32 per dense term (layout in DESIGN)   */
-    MVDR_TAP_SMOOTH_TERMS = 6,       /* 8 x u32/f32 per edge                             */
-    MVDR_TAP_FEATURE_TERMS = 7,      /* 16 x u32/f32 per feature term                    */
-    MVDR_TAP_CROSS_TERMS = 8,        /* 16 x u32/f32 per cross term                      */
-    MVDR_TAP_FOREGROUND_TERMS = 9,   /* 16 x u32/f32 per foreground term                 */
+    MVDR_TAP_DENSE_TERMS = 5,        /* 16 x u32/f32 per dense term: [0:2] knn u16x4,
+                                        [2] r, [4:8] knn weights, [8:14] J (w, v)     */
+    MVDR_TAP_SMOOTH_TERMS = 6,       /* 8 x u32/f32 per edge: [0] i | j << 16, [1] valid,
+                                        [2:5] T_i x_j, [5:8] r = T_i x_j - T_j x_j    */
+    MVDR_TAP_FEATURE_TERMS = 7,      /* 16 x u32/f32 per feature term: [0:2] knn u16x4,
+                                        [4:8] weights, [8:11] target, [12:15] warped  */
+    MVDR_TAP_CROSS_TERMS = 8,        /* 16 x u32/f32 per cross term (feature layout)     */
+    MVDR_TAP_FOREGROUND_TERMS = 9,   /* 16 x u32/f32 per foreground term (dense layout)  */
     MVDR_TAP_DENSE_PIXELS = 10,      /* u16[4] (x, y, view, 0) per dense term            */
     MVDR_TAP_JTJ_DIAG = 11,          /* f32[36N] row-major, damping included             */
     MVDR_TAP_JTJ_DIAG_INV = 12,      /* f32[36N] row-major                               */

@@ -43,6 +43,12 @@ def lib():
         path = os.path.join(_HERE, "libmvdr_solver.so")
         if not os.path.exists(path):
             raise RuntimeError(f"{path} is missing: the HIP solver library must be built (__graft_entry__.build())")
+        # One HIP runtime per process: PyTorch ships its own libamdhip64.so.7 /
+        # libhsa-runtime64.so.1 with the same sonames as /opt/rocm's.  Whichever
+        # loads first serves the whole process, and torch finds no GPU when
+        # /opt/rocm's copy came first, so torch's runtime is loaded before ours.
+        import torch  # noqa: F401
+
         L = C.CDLL(path)
         L.mvdr_solver_create.argtypes = [C.POINTER(_abi.SolverConfig), C.POINTER(C.c_void_p)]
         L.mvdr_solver_destroy.argtypes = [C.c_void_p]

@@ -9,7 +9,7 @@ Bars:
 * J^T J blocks and J^T r from identical term records: relative Frobenius error
   per block <= 1e-5 (diagonal, off-diagonal) and <= 1e-5 on |J^T r|;
 * twist update of every GN iteration (mixed precision) against the FP64 oracle
-  from the same linearisation point: <= 1e-4 relative (BASELINE metric bound);
+  from the same linearisation point: <= 1e-4 relative (the headline error bound);
 * warp update from identical twists: <= 1e-6 absolute on the dual quaternions.
 """
 import numpy as np
@@ -102,7 +102,7 @@ def test_gn_iterations_match_fp64_oracle(run1):
         both = vg & vo
         dJ = np.abs(g["J"][both] - r[both, 8:14])
         assert (dJ <= 1e-5 + 1e-4 * np.abs(r[both, 8:14])).mean() > 0.999
-        # the full-FP64 twist from this warp field: the BASELINE error metric
+        # the full-FP64 twist from this warp field: the headline error metric
         o.assemble()
         o.pcg(cfg.pcg_iterations)
         ref_full = o.get("twist").ravel()
