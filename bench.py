@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=8, help="frames in the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage-reps", type=int, default=50)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="run one device-to-device copy of this size first: the known-size kernel that "
+                         "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
 
 
@@ -179,6 +182,15 @@ def main():
     else:
         overrides.update(precision=0, refine_steps=0)
     cfg = workload_config(args.workload, **overrides)
+
+    if args.copy_mib:
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy: {args.copy_mib} MiB read + {args.copy_mib} MiB written", file=sys.stderr)
+        del src, dst
 
     scene = make_scene(args.workload)
     frame = DeviceFrame(scene)
