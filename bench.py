@@ -55,21 +55,26 @@ def parse():
     return ap.parse_args()
 
 
-def algorithmic_bytes(sz):
+def algorithmic_bytes(sz, mixed):
     """Minimum compulsory bytes per launch of each stage (format independent)."""
     N, P = sz["num_nodes"], sz["num_node_pairs"]
     Td, Tfg, E = sz["num_dense_terms"], sz["num_foreground_terms"], sz["num_smooth_terms"]
     Tf, Tc = sz["num_feature_terms"], sz["num_cross_terms"]
-    kv1, kv2 = sz["num_node2term"], sz["num_nodepair_terms"]
+    kv1 = sz["num_node2term"]
+    # term records: knn ids + weights + J + r (dense / foreground), edge ids + valid
+    # + warped point + residual (smooth), knn + weights + target + warped (p2p)
     r_terms = 52 * (Td + Tfg) + 29 * E + 56 * (Tf + Tc)
+    inv = 144 * N + (288 * N if mixed else 0)
     return {
-        "pair_blocks": r_terms + 4 * kv2 + 12 * P + 144 * P,
-        "node_blocks": r_terms + 4 * kv1 + 4 * (N + 1) + 144 * N + 24 * N + 144 * N,
-        "spmv": 144 * (N + P) + 4 * (N + P) + 4 * (N + 1) + 24 * N + 24 * N,
+        # Node2Term lists + offsets, pair keys + offsets, node positions in;
+        # diagonal blocks, inverses, J^T r and off-diagonal blocks out
+        "assemble": r_terms + 4 * kv1 + 8 * (N + 1) + 4 * P + 16 * N + 144 * N + inv + 24 * N + 144 * P,
+        # blocks + block-row structure + the direction vector in, A d out
+        "spmv": 144 * (N + P) + 4 * (N + P) + 8 * (N + 1) + 8 * P + 24 * N + 24 * N,
     }
 
 
-STAGE_IDS = {"eval_terms": 0, "node_blocks": 1, "pair_blocks": 2, "spmv": 3, "index": 4}
+STAGE_IDS = {"eval_terms": 0, "assemble": 1, "pcg_update": 2, "spmv": 3, "index": 4}
 
 
 def stage_time_ms(solver, stage, reps):
@@ -222,25 +227,26 @@ def main():
 
     # per-stage kernel timing on the solver's stream (hipEvents around repeated launches)
     stages = {}
-    for st in ("eval_terms", "node_blocks", "pair_blocks", "spmv", "index"):
+    for st in STAGE_IDS:
         stages[st] = stage_time_ms(solver, st, args.stage_reps)
     solver.SolveNonRigidAlignment()  # restore a consistent state after the stage loops
     torch.cuda.synchronize()
-    ab = algorithmic_bytes(sizes)
+    ab = algorithmic_bytes(sizes, args.precision == "mixed")
     per_solve = {  # launches of each stage in one solve
-        "eval_terms": cfg.gn_iterations, "node_blocks": cfg.gn_iterations, "pair_blocks": cfg.gn_iterations,
+        "eval_terms": cfg.gn_iterations, "assemble": cfg.gn_iterations,
+        "pcg_update": cfg.gn_iterations * cfg.pcg_iterations,
         "spmv": cfg.gn_iterations * (cfg.pcg_iterations + 2 * cfg.refine_steps), "index": 1,
     }
     share = {k: stages[k] * per_solve[k] for k in stages}
     print("[bench] stage ms/launch " + json.dumps({k: round(v, 4) for k, v in stages.items()}), file=sys.stderr)
     print("[bench] stage ms/solve  " + json.dumps({k: round(v, 4) for k, v in share.items()}), file=sys.stderr)
-    dominant = max(("pair_blocks", "node_blocks", "spmv"), key=lambda k: share[k])
+    dominant = max(("assemble", "spmv"), key=lambda k: share[k])
     achieved = ab[dominant] / (stages[dominant] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "bytes_per_launch": int(ab[dominant]), "launch_ms": round(stages[dominant], 5)}
     spmv_gbs = ab["spmv"] / (stages["spmv"] * 1e-3) / 1e9
-    jtj_gbs = (ab["pair_blocks"] + ab["node_blocks"]) / ((stages["pair_blocks"] + stages["node_blocks"]) * 1e-3) / 1e9
+    jtj_gbs = ab["assemble"] / (stages["assemble"] * 1e-3) / 1e9
 
     traffic = measured_traffic(dominant, args.workload)
     roofline["traffic"] = traffic

@@ -209,8 +209,9 @@ mvdr_status mvdr_solver_debug_get(mvdr_solver* solver, int tap, void* dst, size_
 
 typedef enum {
     MVDR_STAGE_EVAL_TERMS = 0,     /* term residuals / Jacobians                      */
-    MVDR_STAGE_NODE_BLOCKS = 1,    /* J^T J diagonal + J^T r + block inverse          */
-    MVDR_STAGE_PAIR_BLOCKS = 2,    /* J^T J off-diagonal blocks                       */
+    MVDR_STAGE_ASSEMBLE = 1,       /* J^T J diagonal + off-diagonal blocks, J^T r,
+                                      block inverse (one fused pass per node row)   */
+    MVDR_STAGE_PCG_UPDATE = 2,     /* one PCG vector update (x, r, M^-1 r, dot)       */
     MVDR_STAGE_SPMV = 3,           /* one block-sparse ApplyJtJ (q = A d) + dot(d, q) */
     MVDR_STAGE_INDEX = 4           /* the whole index build                           */
 } mvdr_stage;

@@ -248,7 +248,7 @@ struct Counts {
     uint32_t dense, smooth, fg, feat, cross;   // term counts
     uint32_t off[5];                           // global term offsets (TermTypeOffset)
     uint32_t kv1, kv2, pairs;                  // Node2Term entries, pair-term entries, unique pairs
-    uint32_t max_seg, max_row;                 // longest node / pair-row list seen
+    uint32_t max_seg, max_row;                 // longest Node2Term list, most pairs of a row
     uint32_t error;                            // bit mask of capacity overflows
 };
 
@@ -309,25 +309,19 @@ struct Buffers {
     uint32_t* tile_offset;
     uint32_t* feat_flag;
     // node -> term
+    uint32_t* n2t_hist;      // chunks x N: per-chunk node counts, then chunk offsets
     uint32_t* n2t_count;     // N + 1
     uint32_t* n2t_offset;    // N + 1
-    uint32_t* n2t_cursor;    // N
     uint32_t* n2t_index;     // kv1
-    // node pairs
-    uint32_t* row_count;     // N + 1 : pair-term entries per row
-    uint32_t* row_offset;    // N + 1
-    uint32_t* row_pairs;     // N + 1 : unique pairs per row
-    uint32_t* pair_offset;   // N + 1
-    uint32_t* uniq_j;        // kv2 scratch: unique j per row position
-    uint2* uniq_range;       // kv2 scratch
-    uint32_t* pair_terms;    // kv2
-    uint32_t* pair_key;      // cap_pairs
-    uint2* pair_range;       // cap_pairs
+    // node pairs (i, j), i < j, in (i, j) order
+    uint32_t* row_up;        // N + 1 : pairs (i, j > i) per row i
+    uint32_t* row_low;       // N + 1 : pairs (k < i, i) per row i
+    uint32_t* pair_offset;   // N + 1 : first pair of row i
+    uint32_t* pair_key;      // cap_pairs: i << 16 | j
+    uint2* pair_range;       // cap_pairs: term-list range (built on request)
     // block-sparse rows
-    uint32_t* low_count;     // N + 1
     uint32_t* low_offset;    // N + 1
-    uint32_t* low_cursor;    // N
-    uint2* low_ent;          // cap_pairs: (k, pair)
+    uint2* low_ent;          // cap_pairs: (k, pair) of row i's lower entries, k ascending
     // linear system
     float* diag;             // 36 N
     float* diag_inv;         // 36 N

@@ -15,8 +15,6 @@ namespace mvdr {
 constexpr int kTileThreads = 256;
 constexpr int kTileRounds = 16;
 constexpr int kTilePixels = kTileThreads * kTileRounds;  // 4096 pixels per tile
-constexpr int kSegCap = 8192;                            // longest node term list sorted in LDS
-constexpr int kRowCap = 4096;                            // longest pair row sorted in LDS
 
 // ------------------------------------------------------------- term selection
 struct PixelCoord {
@@ -84,9 +82,8 @@ __global__ void __launch_bounds__(kTileThreads) compact_count_kernel(Buffers b, 
         const uint32_t stride = gridDim.x * blockDim.x;
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= b.cap_nodes; i += stride) {
             b.n2t_count[i] = 0;
-            b.row_count[i] = 0;
-            b.row_pairs[i] = 0;
-            b.low_count[i] = 0;
+            b.row_up[i] = 0;
+            b.row_low[i] = 0;
         }
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             Counts* c = b.counts;
@@ -357,22 +354,40 @@ __device__ inline void term_offsets(const Counts& c, uint32_t E, uint32_t off[5]
 
 // Node keys of term t (buildTermKeyValueKernel, Node2TermsIndex.cu:8-110):
 // four knn nodes, or the two ends of a graph edge.  Returns the key count.
-__device__ inline int term_nodes(const Buffers& b, const FrameDev& f, const uint32_t off[5], uint32_t t, uint16_t n[4]) {
-    ushort4 k;
+__device__ inline int term_nodes(const Buffers& b, const FrameDev& f, const uint32_t off[5], uint32_t t, ushort4& k) {
     if (t < off[0]) k = b.dense[t].knn;
     else if (t < off[1]) {
         const ushort2 e = f.graph[t - off[0]];
-        n[0] = e.x;
-        n[1] = e.y;
+        k = make_ushort4(e.x, e.y, 0, 0);
         return 2;
     } else if (t < off[2]) k = b.fg[t - off[1]].knn;
     else if (t < off[3]) k = b.feat[t - off[2]].knn;
     else k = b.cross[t - off[3]].knn;
-    n[0] = k.x; n[1] = k.y; n[2] = k.z; n[3] = k.w;
     return 4;
 }
 
-__global__ void n2t_count_kernel(Buffers b) {
+__device__ inline uint32_t slot(ushort4 k, int q) { return q == 0 ? k.x : (q == 1 ? k.y : (q == 2 ? k.z : k.w)); }
+
+// The reference builds Node2Term by a stable key sort of (node, term) pairs
+// generated in term order (Node2TermsIndex.cu:112-166): each node's list holds
+// its terms ascending, a term once per knn slot naming the node.  Here the same
+// lists come from a stable counting sort over chunks of kChunkTerms terms, one
+// wave per chunk:
+//   n2t_hist    per-chunk node histograms H[c][n] (LDS counters)
+//   n2t_colscan H[c][n] -> exclusive prefix over chunks, per-node totals
+//   node_scan   per-node totals -> list offsets
+//   n2t_place   each chunk writes its entries at offset[n] + H[c][n] + rank,
+//               rank = position among the chunk's earlier entries of node n
+// No global atomics and no sorts: the result is deterministic by construction.
+constexpr uint32_t kChunkTerms = 512;
+constexpr uint32_t kHistNodes = 16384;  // largest LDS node tile (64 KB of u32 counters)
+
+// LDS node tile of the counting-sort kernels (dynamic shared memory).
+uint32_t hist_tile(uint32_t cap_nodes) { return std::min<uint32_t>(cap_nodes, kHistNodes); }
+
+__device__ inline uint32_t num_chunks(const Counts& c) { return (c.off[4] + kChunkTerms - 1) / kChunkTerms; }
+
+__global__ void __launch_bounds__(64) n2t_hist_kernel(Buffers b, uint32_t tile) {
     const FrameDev& f = *b.frame;
     Counts& c = *b.counts;
     uint32_t off[5];
@@ -381,247 +396,325 @@ __global__ void n2t_count_kernel(Buffers b) {
         c.smooth = f.E;
         for (int k = 0; k < 5; ++k) c.off[k] = off[k];
     }
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < off[4]; t += stride) {
-        uint16_t n[4];
-        const int m = term_nodes(b, f, off, t, n);
-        for (int q = 0; q < m; ++q) {
-            if (n[q] < f.N) atomicAdd(&b.n2t_count[n[q]], 1u);
-            else atomicOr(&c.error, kErrNodes);
+    const uint32_t t0 = blockIdx.x * kChunkTerms;
+    if (t0 >= off[4]) return;
+    const uint32_t t1 = min(off[4], t0 + kChunkTerms);
+    const uint32_t N = f.N;
+    extern __shared__ uint32_t hist[];
+    uint32_t* H = b.n2t_hist + (size_t)blockIdx.x * N;
+    for (uint32_t n0 = 0; n0 < N; n0 += tile) {
+        const uint32_t span = min(tile, N - n0);
+        for (uint32_t k = threadIdx.x; k < span; k += 64) hist[k] = 0;
+        __syncthreads();
+        for (uint32_t t = t0 + threadIdx.x; t < t1; t += 64) {
+            ushort4 k;
+            const int m = term_nodes(b, f, off, t, k);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t n = slot(k, q);
+                if (q >= m) continue;
+                if (n >= N) {
+                    if (n0 == 0) atomicOr(&c.error, kErrNodes);
+                } else if (n >= n0 && n - n0 < span) {
+                    atomicAdd(&hist[n - n0], 1u);
+                }
+            }
         }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < span; k += 64) H[n0 + k] = hist[k];
+        __syncthreads();
     }
 }
 
-// Exclusive scan over the N node counters.  which: 0 = Node2Term segments,
-// 1 = pair-term rows, 2 = unique pairs per row, 3 = lower block-row entries.
-__global__ void __launch_bounds__(1024) node_scan_kernel(Buffers b, int which) {
-    const FrameDev& f = *b.frame;
-    const uint32_t N = f.N;
-    Counts& c = *b.counts;
-    const uint32_t* in;
-    uint32_t* out;
-    uint32_t* cursor = nullptr;
-    switch (which) {
-        case 0: in = b.n2t_count; out = b.n2t_offset; cursor = b.n2t_cursor; break;
-        case 1: in = b.row_count; out = b.row_offset; break;
-        case 2: in = b.row_pairs; out = b.pair_offset; break;
-        default: in = b.low_count; out = b.low_offset; cursor = b.low_cursor; break;
+// One lane per node, four waves splitting the chunk range: H[c][n] becomes the
+// number of node-n entries in chunks before c; n2t_count[n] the node's total.
+__global__ void __launch_bounds__(256) n2t_colscan_kernel(Buffers b) {
+    const uint32_t N = b.frame->N;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t n = blockIdx.x * 64 + lane;
+    const uint32_t C = num_chunks(*b.counts);
+    const uint32_t per = (C + 3) / 4;
+    const uint32_t c0 = min(C, w * per), c1 = min(C, c0 + per);
+    __shared__ uint32_t part[4][64];
+    uint32_t sum = 0;
+    if (n < N)
+        for (uint32_t ch = c0; ch < c1; ++ch) sum += b.n2t_hist[(size_t)ch * N + n];
+    part[w][lane] = sum;
+    __syncthreads();
+    uint32_t run = 0, total = 0;
+    for (uint32_t k = 0; k < 4; ++k) {
+        run += k < w ? part[k][lane] : 0u;
+        total += part[k][lane];
     }
+    if (n >= N) return;
+    for (uint32_t ch = c0; ch < c1; ++ch) {
+        uint32_t* h = b.n2t_hist + (size_t)ch * N + n;
+        const uint32_t v = *h;
+        *h = run;
+        run += v;
+    }
+    if (w == 3) b.n2t_count[n] = total;
+}
+
+__global__ void __launch_bounds__(64) n2t_place_kernel(Buffers b, uint32_t tile) {
+    const FrameDev& f = *b.frame;
+    const Counts& c = *b.counts;
+    if (c.error) return;
+    const uint32_t t0 = blockIdx.x * kChunkTerms;
+    if (t0 >= c.off[4]) return;
+    const uint32_t t1 = min(c.off[4], t0 + kChunkTerms);
+    const uint32_t N = f.N;
+    const uint32_t lane = threadIdx.x;
+    const uint64_t below = (1ull << lane) - 1ull;
+    extern __shared__ uint32_t cur[];
+    const uint32_t* H = b.n2t_hist + (size_t)blockIdx.x * N;
+    for (uint32_t n0 = 0; n0 < N; n0 += tile) {
+        const uint32_t span = min(tile, N - n0);
+        for (uint32_t k = lane; k < span; k += 64) cur[k] = b.n2t_offset[n0 + k] + H[n0 + k];
+        __syncthreads();
+        for (uint32_t tb = t0; tb < t1; tb += 64) {
+            const uint32_t t = tb + lane;
+            ushort4 k = make_ushort4(0, 0, 0, 0);
+            const int m = t < t1 ? term_nodes(b, f, c.off, t, k) : 0;
+            const uint32_t n[4] = {k.x, k.y, k.z, k.w};
+            bool live[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) live[q] = q < m && n[q] >= n0 && n[q] - n0 < span;
+            uint64_t pend[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) pend[q] = __ballot(live[q]);
+            // one distinct node per iteration; entries ordered by (term, slot)
+            while (pend[0] | pend[1] | pend[2] | pend[3]) {
+                const int q0 = pend[0] ? 0 : (pend[1] ? 1 : (pend[2] ? 2 : 3));
+                const int leader = __builtin_ctzll(pend[q0]);
+                const uint32_t v = (uint32_t)__shfl((int)slot(k, q0), leader, 64);
+                uint64_t hit[4];
+                uint32_t total = 0, before = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    hit[q] = __ballot(live[q] && n[q] == v);
+                    total += (uint32_t)__popcll(hit[q]);
+                    before += (uint32_t)__popcll(hit[q] & below);
+                }
+                const uint32_t base = cur[v - n0];
+                uint32_t own = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (live[q] && n[q] == v) {
+                        b.n2t_index[base + before + own] = t;
+                        ++own;
+                        live[q] = false;
+                    }
+                __syncthreads();
+                if (lane == 0) cur[v - n0] = base + total;
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < 4; ++q) pend[q] &= ~hit[q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Exclusive scans over the N node counters.  which: 0 = Node2Term lists,
+// 1 = upper pairs per row (block 0) and lower entries per row (block 1).
+__global__ void __launch_bounds__(1024) node_scan_kernel(Buffers b, int which) {
+    const uint32_t N = b.frame->N;
+    Counts& c = *b.counts;
+    const int job = which == 0 ? 0 : 1 + (int)blockIdx.x;
+    const uint32_t* in = job == 0 ? b.n2t_count : (job == 1 ? b.row_up : b.row_low);
+    uint32_t* out = job == 0 ? b.n2t_offset : (job == 1 ? b.pair_offset : b.low_offset);
     uint32_t mx = 0;
-    const uint32_t total = block_exclusive_scan_1024(in, out, N, cursor, &mx);
+    const uint32_t total = block_exclusive_scan_1024(in, out, N, nullptr, &mx);
     if (threadIdx.x == 0) {
         out[N] = total;
-        switch (which) {
-            case 0:
-                if (total > b.cap_kv1) atomicOr(&c.error, kErrKv1);
-                if (mx > kSegCap) atomicOr(&c.error, kErrSeg);
-                c.kv1 = total;
-                c.max_seg = mx;
-                break;
-            case 1:
-                if (total > b.cap_kv2) atomicOr(&c.error, kErrKv2);
-                if (mx > kRowCap) atomicOr(&c.error, kErrRow);
-                c.kv2 = total;
-                c.max_row = mx;
-                break;
-            case 2:
-                if (total > b.cap_pairs) atomicOr(&c.error, kErrPairs);
-                c.pairs = total;
-                break;
-            default:
-                break;
+        if (job == 0) {
+            if (total > b.cap_kv1) atomicOr(&c.error, kErrKv1);
+            c.kv1 = total;
+            c.max_seg = mx;
+        } else if (job == 1) {
+            if (total > b.cap_pairs) atomicOr(&c.error, kErrPairs);
+            c.pairs = total;
         }
     }
 }
 
-__global__ void n2t_scatter_kernel(Buffers b) {
+// ------------------------------------------------------------- node pairs
+// The unique pairs (i, j), i < j, of nodes sharing a term -- the key set of the
+// reference's NodePair2Term index (NodePair2TermsIndex.cu:11-185) -- and, per
+// block row, the lower-triangle entries (NodePairRowColumnIndex.cu:19-179).
+// One wave per row i collects the partner nodes of its Node2Term list in an LDS
+// hash set, then emits them ascending (wave minimum per step).
+constexpr uint32_t kRowHash = 512;  // partner set capacity per row
+
+// Partners of node i in term t with the reference's pair multiplicity: every
+// knn slot q with knn[q] != i, or the other end of a non-loop graph edge.
+__device__ inline ushort4 term_partners(const Buffers& b, const FrameDev& f, const uint32_t off[5], uint32_t t, uint32_t i) {
+    constexpr uint16_t kNone = 0xFFFF;
+    ushort4 k;
+    const int m = term_nodes(b, f, off, t, k);
+    if (m == 2) return make_ushort4(k.x != k.y ? (k.x == i ? k.y : k.x) : kNone, kNone, kNone, kNone);
+    return make_ushort4(k.x != i ? k.x : kNone, k.y != i ? k.y : kNone, k.z != i ? k.z : kNone, k.w != i ? k.w : kNone);
+}
+
+template <bool kWrite>
+__global__ void __launch_bounds__(64) pair_rows_kernel(Buffers b) {
     const FrameDev& f = *b.frame;
+    Counts& c = *b.counts;
+    if (c.error) return;
+    const uint32_t i = blockIdx.x;
+    if (i >= f.N) return;
+    const uint32_t lane = threadIdx.x;
+    __shared__ uint32_t set[kRowHash];
+    for (uint32_t k = lane; k < kRowHash; k += 64) set[k] = 0xFFFFFFFFu;
+    __syncthreads();
+    const uint32_t beg = b.n2t_offset[i], end = b.n2t_offset[i + 1];
+    uint32_t upper_entries = 0;
+    bool overflow = false;
+    for (uint32_t p = beg + lane; p < end; p += 64) {
+        const ushort4 o = term_partners(b, f, c.off, b.n2t_index[p], i);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t v = slot(o, q);
+            if (v == 0xFFFF) continue;
+            upper_entries += v > i ? 1u : 0u;
+            uint32_t h = (v * 2654435761u) >> 23;  // 9 bits
+            for (uint32_t probe = 0;; ++probe) {
+                if (probe == kRowHash) {
+                    overflow = true;
+                    break;
+                }
+                const uint32_t old = atomicCAS(&set[h], 0xFFFFFFFFu, v);
+                if (old == 0xFFFFFFFFu || old == v) break;
+                h = (h + 1) & (kRowHash - 1);
+            }
+        }
+    }
+    if (__ballot(overflow)) {
+        if (lane == 0) atomicOr(&c.error, kErrRow);
+        return;
+    }
+    __syncthreads();
+    uint32_t mine[kRowHash / 64];
+#pragma unroll
+    for (int k = 0; k < (int)(kRowHash / 64); ++k) mine[k] = set[lane + 64 * k];
+    uint32_t up = 0, low = 0;
+    const uint32_t up_base = kWrite ? b.pair_offset[i] : 0u, low_base = kWrite ? b.low_offset[i] : 0u;
+    int64_t last = -1;
+    for (;;) {
+        uint32_t m = 0xFFFFFFFFu;
+#pragma unroll
+        for (int k = 0; k < (int)(kRowHash / 64); ++k)
+            if ((int64_t)mine[k] > last && mine[k] < m) m = mine[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint32_t u = (uint32_t)__shfl_xor((int)m, o, 64);
+            m = u < m ? u : m;
+        }
+        if (m == 0xFFFFFFFFu) break;
+        if (m > i) {
+            if (kWrite && lane == 0) b.pair_key[up_base + up] = (i << 16) | m;
+            ++up;
+        } else {
+            if (kWrite && lane == 0) b.low_ent[low_base + low] = make_uint2(m, 0u);
+            ++low;
+        }
+        last = m;
+    }
+    if (!kWrite && lane == 0) {
+        b.row_up[i] = up;
+        b.row_low[i] = low;
+        atomicMax(&c.max_row, up + low);
+    }
+    if (!kWrite) {
+        upper_entries = wave_sum(upper_entries);
+        if (lane == 0) atomicAdd(&c.kv2, upper_entries);
+    }
+}
+
+// Pair index of every lower entry (k, .) of row j: the position of (k, j) in
+// row k's ascending upper list.
+__global__ void __launch_bounds__(256) pair_link_kernel(Buffers b) {
     const Counts& c = *b.counts;
     if (c.error) return;
+    const uint32_t N = b.frame->N;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < c.off[4]; t += stride) {
-        uint16_t n[4];
-        const int m = term_nodes(b, f, c.off, t, n);
-        for (int q = 0; q < m; ++q)
-            if (n[q] < f.N) {
-                const uint32_t pos = atomicAdd(&b.n2t_cursor[n[q]], 1u);
-                b.n2t_index[pos] = t;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < N; j += stride) {
+        for (uint32_t l = b.low_offset[j]; l < b.low_offset[j + 1]; ++l) {
+            const uint32_t k = b.low_ent[l].x;
+            const uint32_t key = (k << 16) | j;
+            uint32_t lo = b.pair_offset[k], hi = b.pair_offset[k + 1];
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (b.pair_key[mid] < key) lo = mid + 1;
+                else hi = mid;
             }
-    }
-}
-
-// Sort each node's term list ascending: the order a stable key sort of
-// (node, term) pairs generated in term order produces (Node2TermsIndex.cu:136-166).
-__global__ void __launch_bounds__(256) n2t_sort_kernel(Buffers b) {
-    const FrameDev& f = *b.frame;
-    if (b.counts->error) return;
-    __shared__ uint32_t keys[kSegCap];
-    for (uint32_t n = blockIdx.x; n < f.N; n += gridDim.x) {
-        const uint32_t beg = b.n2t_offset[n], len = b.n2t_offset[n + 1] - beg;
-        if (len < 2) continue;
-        const uint32_t m = next_pow2(len);
-        for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) keys[i] = i < len ? b.n2t_index[beg + i] : 0xFFFFFFFFu;
-        __syncthreads();
-        bitonic_sort_lds(keys, m);
-        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) b.n2t_index[beg + i] = keys[i];
-        __syncthreads();
-    }
-}
-
-// ------------------------------------------------------------ NodePair2Term
-// Pairs (i, j), i < j, that term t contributes to row i, counted with the
-// multiplicity of the reference's key generation: for every knn slot p holding
-// i, one entry per slot q with knn[q] > i (computeKVNodePairKNN,
-// NodePair2TermsIndex.cu:11-27); the slot multiplicity is already carried by
-// the Node2Term list, which holds t once per slot equal to i.  A graph edge
-// gives (min, max) once (NodePair2TermsIndex.cu:63-73).
-__device__ inline int row_entries(const Buffers& b, const FrameDev& f, const uint32_t off[5], uint32_t t, uint32_t i,
-                                  uint16_t out[4]) {
-    uint16_t n[4];
-    const int m = term_nodes(b, f, off, t, n);
-    int k = 0;
-    if (m == 2) {
-        const uint16_t other = (n[0] == i) ? n[1] : n[0];
-        if (other > i && n[0] != n[1]) out[k++] = other;
-        return k;
-    }
-    for (int q = 0; q < 4; ++q)
-        if (n[q] > i) out[k++] = n[q];
-    return k;
-}
-
-__global__ void __launch_bounds__(256) pair_count_kernel(Buffers b) {
-    const FrameDev& f = *b.frame;
-    const Counts& c = *b.counts;
-    if (c.error) return;
-    for (uint32_t i = blockIdx.x; i < f.N; i += gridDim.x) {
-        const uint32_t beg = b.n2t_offset[i], end = b.n2t_offset[i + 1];
-        uint32_t cnt = 0;
-        for (uint32_t p = beg + threadIdx.x; p < end; p += blockDim.x) {
-            uint16_t o[4];
-            cnt += row_entries(b, f, c.off, b.n2t_index[p], i, o);
-        }
-        cnt = block_sum_u32<256>(cnt);
-        if (threadIdx.x == 0) b.row_count[i] = cnt;
-        __syncthreads();
-    }
-}
-
-// Row i: gather (j, term) entries, sort, write the term column of the sorted
-// pair-term array and the unique j's with their ranges.
-__global__ void __launch_bounds__(256) pair_fill_kernel(Buffers b) {
-    const FrameDev& f = *b.frame;
-    const Counts& c = *b.counts;
-    if (c.error) return;
-    __shared__ uint64_t keys[kRowCap];
-    __shared__ uint32_t fill;
-    __shared__ uint32_t wave_cnt[2][4];
-    for (uint32_t i = blockIdx.x; i < f.N; i += gridDim.x) {
-        const uint32_t len = b.row_count[i];
-        const uint32_t out_base = b.row_offset[i];
-        if (threadIdx.x == 0) fill = 0;
-        __syncthreads();
-        const uint32_t beg = b.n2t_offset[i], end = b.n2t_offset[i + 1];
-        for (uint32_t p = beg + threadIdx.x; p < end; p += blockDim.x) {
-            const uint32_t t = b.n2t_index[p];
-            uint16_t o[4];
-            const int k = row_entries(b, f, c.off, t, i, o);
-            if (k) {
-                const uint32_t at = atomicAdd(&fill, (uint32_t)k);
-                for (int q = 0; q < k; ++q) keys[at + q] = ((uint64_t)o[q] << 32) | t;
-            }
-        }
-        __syncthreads();
-        const uint32_t m = next_pow2(len);
-        for (uint32_t q = len + threadIdx.x; q < m; q += blockDim.x) keys[q] = ~0ull;
-        __syncthreads();
-        if (len > 1) bitonic_sort_lds(keys, m);
-        // segment heads -> unique pairs, in order
-        uint32_t running = 0;
-        int parity = 0;
-        for (uint32_t base = 0; base < len; base += blockDim.x, parity ^= 1) {
-            const uint32_t q = base + threadIdx.x;
-            bool head = false;
-            uint32_t j = 0;
-            if (q < len) {
-                j = (uint32_t)(keys[q] >> 32);
-                b.pair_terms[out_base + q] = (uint32_t)keys[q];
-                head = q == 0 || (uint32_t)(keys[q - 1] >> 32) != j;
-            }
-            const uint64_t mask = __ballot(head);
-            if ((threadIdx.x & 63) == 0) wave_cnt[parity][threadIdx.x >> 6] = (uint32_t)__popcll(mask);
-            __syncthreads();
-            uint32_t before = 0, chunk = 0;
-            for (int w = 0; w < 4; ++w) {
-                before += (w < (int)(threadIdx.x >> 6)) ? wave_cnt[parity][w] : 0u;
-                chunk += wave_cnt[parity][w];
-            }
-            if (head) {
-                uint32_t e = q + 1;
-                while (e < len && (uint32_t)(keys[e] >> 32) == j) ++e;
-                const uint32_t u = running + before + lane_prefix(mask);
-                b.uniq_j[out_base + u] = j;
-                b.uniq_range[out_base + u] = make_uint2(out_base + q, out_base + e);
-            }
-            running += chunk;
-        }
-        if (threadIdx.x == 0) b.row_pairs[i] = running;
-        __syncthreads();
-    }
-}
-
-// Unique pairs in (i, j) order with their term ranges, and the count of
-// lower-triangle entries each row j receives.
-__global__ void __launch_bounds__(256) pair_emit_kernel(Buffers b) {
-    const FrameDev& f = *b.frame;
-    const Counts& c = *b.counts;
-    if (c.error) return;
-    for (uint32_t i = blockIdx.x; i < f.N; i += gridDim.x) {
-        const uint32_t n = b.row_pairs[i], src = b.row_offset[i], dst = b.pair_offset[i];
-        for (uint32_t u = threadIdx.x; u < n; u += blockDim.x) {
-            const uint32_t j = b.uniq_j[src + u];
-            b.pair_key[dst + u] = (i << 16) | j;
-            b.pair_range[dst + u] = b.uniq_range[src + u];
-            atomicAdd(&b.low_count[j], 1u);
+            if (lo == b.pair_offset[k + 1] || b.pair_key[lo] != key) atomicOr(&b.counts->error, kErrPairs);
+            b.low_ent[l].y = lo;
         }
     }
 }
 
-__global__ void low_scatter_kernel(Buffers b) {
+// ------------------------------------------ NodePair2Term term lists (taps)
+// The per-pair term lists of the reference's NodePair2Term index
+// (NodePair2TermsIndex.cu:11-185): for pair (i, j), every term of node i's
+// list once per knn slot holding i (the Node2Term multiplicity) times the
+// number of slots holding j; graph edges once.  The solve does not need them
+// (assembly walks Node2Term rows); they are built on request for inspection.
+__device__ inline uint32_t partner_count(const Buffers& b, const FrameDev& f, const uint32_t off[5], uint32_t t,
+                                         uint32_t i, uint32_t j) {
+    const ushort4 o = term_partners(b, f, off, t, i);
+    return (o.x == j) + (o.y == j) + (o.z == j) + (o.w == j);
+}
+
+__global__ void __launch_bounds__(256) pair_terms_kernel(Buffers b, uint32_t* pair_terms, int fill) {
+    const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
-    if (c.error) return;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < c.pairs; p += stride) {
         const uint32_t key = b.pair_key[p];
         const uint32_t i = key >> 16, j = key & 0xFFFF;
-        const uint32_t pos = atomicAdd(&b.low_cursor[j], 1u);
-        b.low_ent[pos] = make_uint2(i, p);
+        uint32_t at = fill ? b.pair_range[p].x : 0u;
+        for (uint32_t q = b.n2t_offset[i]; q < b.n2t_offset[i + 1]; ++q) {
+            const uint32_t t = b.n2t_index[q];
+            const uint32_t m = partner_count(b, f, c.off, t, i, j);
+            if (fill)
+                for (uint32_t r = 0; r < m; ++r) pair_terms[at + r] = t;
+            at += m;
+        }
+        if (!fill) b.pair_range[p].y = at;  // count, turned into ranges by pair_ranges_kernel
     }
 }
 
-// Lower-triangle entries of each block row in ascending column order.
-__global__ void low_sort_kernel(Buffers b) {
-    const FrameDev& f = *b.frame;
-    if (b.counts->error) return;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < f.N; i += stride) {
-        const uint32_t beg = b.low_offset[i], end = b.low_offset[i + 1];
-        for (uint32_t a = beg + 1; a < end; ++a) {
-            const uint2 key = b.low_ent[a];
-            uint32_t z = a;
-            while (z > beg && b.low_ent[z - 1].x > key.x) {
-                b.low_ent[z] = b.low_ent[z - 1];
-                --z;
-            }
-            b.low_ent[z] = key;
-        }
+__global__ void __launch_bounds__(1024) pair_ranges_kernel(Buffers b) {
+    __shared__ uint32_t wave_tot[16];
+    const uint32_t P = b.counts->pairs;
+    // sequential chunks per thread, as block_exclusive_scan_1024, over pair_range[].y
+    const uint32_t per = (P + 1023) / 1024;
+    const uint32_t beg = min(P, threadIdx.x * per), end = min(P, beg + per);
+    uint32_t local = 0;
+    for (uint32_t p = beg; p < end; ++p) local += b.pair_range[p].y;
+    uint32_t incl = local;
+    const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(incl, o, 64);
+        if (lane >= (uint32_t)o) incl += u;
+    }
+    if (lane == 63) wave_tot[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    uint32_t run = incl - local;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) run += wave_tot[w];
+    for (uint32_t p = beg; p < end; ++p) {
+        const uint32_t m = b.pair_range[p].y;
+        b.pair_range[p] = make_uint2(run, run + m);
+        run += m;
     }
 }
 
 // ------------------------------------------------------------- host launch
-void launch_index_build(const Buffers& b, uint32_t cap_tiles, uint32_t cap_nodes, bool use_foreground, bool use_cross,
-                        hipStream_t stream) {
-    const uint32_t node_blocks = std::min<uint32_t>(cap_nodes, 8192u);
+void launch_index_build(const Buffers& b, uint32_t cap_tiles, uint32_t cap_nodes, uint32_t cap_chunks,
+                        bool use_foreground, bool use_cross, hipStream_t stream) {
     hipLaunchKernelGGL(compact_count_kernel<DensePred>, dim3(cap_tiles), dim3(kTileThreads), 0, stream, b, DensePred{}, 1);
     hipLaunchKernelGGL(compact_scan_kernel, dim3(1), dim3(1024), 0, stream, b, 0);
     hipLaunchKernelGGL(compact_scatter_kernel<DensePred>, dim3(cap_tiles), dim3(kTileThreads), 0, stream, b, DensePred{}, 0);
@@ -633,19 +726,24 @@ void launch_index_build(const Buffers& b, uint32_t cap_tiles, uint32_t cap_nodes
                            ForegroundPred{}, 1);
     }
     hipLaunchKernelGGL(select_pairs_kernel, dim3(2), dim3(1024), 0, stream, b, use_cross ? 1 : 0);
-    const uint32_t term_blocks = 1024;
-    hipLaunchKernelGGL(n2t_count_kernel, dim3(term_blocks), dim3(256), 0, stream, b);
+    const uint32_t tile = hist_tile(cap_nodes);
+    hipLaunchKernelGGL(n2t_hist_kernel, dim3(cap_chunks), dim3(64), tile * sizeof(uint32_t), stream, b, tile);
+    hipLaunchKernelGGL(n2t_colscan_kernel, dim3((cap_nodes + 63) / 64), dim3(256), 0, stream, b);
     hipLaunchKernelGGL(node_scan_kernel, dim3(1), dim3(1024), 0, stream, b, 0);
-    hipLaunchKernelGGL(n2t_scatter_kernel, dim3(term_blocks), dim3(256), 0, stream, b);
-    hipLaunchKernelGGL(n2t_sort_kernel, dim3(node_blocks), dim3(256), 0, stream, b);
-    hipLaunchKernelGGL(pair_count_kernel, dim3(node_blocks), dim3(256), 0, stream, b);
-    hipLaunchKernelGGL(node_scan_kernel, dim3(1), dim3(1024), 0, stream, b, 1);
-    hipLaunchKernelGGL(pair_fill_kernel, dim3(node_blocks), dim3(256), 0, stream, b);
-    hipLaunchKernelGGL(node_scan_kernel, dim3(1), dim3(1024), 0, stream, b, 2);
-    hipLaunchKernelGGL(pair_emit_kernel, dim3(node_blocks), dim3(256), 0, stream, b);
-    hipLaunchKernelGGL(node_scan_kernel, dim3(1), dim3(1024), 0, stream, b, 3);
-    hipLaunchKernelGGL(low_scatter_kernel, dim3(term_blocks), dim3(256), 0, stream, b);
-    hipLaunchKernelGGL(low_sort_kernel, dim3((cap_nodes + 255) / 256), dim3(256), 0, stream, b);
+    hipLaunchKernelGGL(n2t_place_kernel, dim3(cap_chunks), dim3(64), tile * sizeof(uint32_t), stream, b, tile);
+    hipLaunchKernelGGL(pair_rows_kernel<false>, dim3(cap_nodes), dim3(64), 0, stream, b);
+    hipLaunchKernelGGL(node_scan_kernel, dim3(2), dim3(1024), 0, stream, b, 1);
+    hipLaunchKernelGGL(pair_rows_kernel<true>, dim3(cap_nodes), dim3(64), 0, stream, b);
+    hipLaunchKernelGGL(pair_link_kernel, dim3((cap_nodes + 255) / 256), dim3(256), 0, stream, b);
+}
+
+uint32_t index_chunks(uint64_t max_terms) { return (uint32_t)((max_terms + kChunkTerms - 1) / kChunkTerms); }
+
+// NodePair2Term term lists into pair_terms (capacity checked by the caller).
+void launch_pair_terms(const Buffers& b, uint32_t* pair_terms, hipStream_t stream) {
+    hipLaunchKernelGGL(pair_terms_kernel, dim3(256), dim3(256), 0, stream, b, pair_terms, 0);
+    hipLaunchKernelGGL(pair_ranges_kernel, dim3(1), dim3(1024), 0, stream, b);
+    hipLaunchKernelGGL(pair_terms_kernel, dim3(256), dim3(256), 0, stream, b, pair_terms, 1);
 }
 
 }  // namespace mvdr

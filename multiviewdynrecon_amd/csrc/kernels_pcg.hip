@@ -442,6 +442,12 @@ void launch_spmv_once(const Buffers& b, bool fp64, uint32_t cap_nodes, hipStream
     else hipLaunchKernelGGL(pcg_spmv_kernel<float>, dim3(sg), dim3(256), 0, stream, b, 0u, b.d0, b.d1);
 }
 
+void launch_pcg_update_once(const Buffers& b, bool fp64, hipStream_t stream) {
+    const uint32_t sg = spmv_grid(b.cap_nodes);
+    if (fp64) hipLaunchKernelGGL(pcg_update_kernel<double>, dim3(kVecBlocks), dim3(256), 0, stream, b, 0u, sg, b.d1);
+    else hipLaunchKernelGGL(pcg_update_kernel<float>, dim3(kVecBlocks), dim3(256), 0, stream, b, 0u, sg, b.d1);
+}
+
 void launch_apply_update(const Buffers& b, const DQ* in, DQ* out, hipStream_t stream) {
     hipLaunchKernelGGL(apply_update_kernel, dim3(64), dim3(256), 0, stream, b, in, out);
 }

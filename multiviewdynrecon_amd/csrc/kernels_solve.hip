@@ -200,48 +200,6 @@ __device__ inline void smooth_rows(const SmoothRec& s, bool node_i, float J[3][6
     res[2] = on * s.r[2];
 }
 
-// Jacobian rows of term t with respect to node n (knn weight applied),
-// residuals and per-row lambda^2.  Returns the number of rows (1..4).
-__device__ inline int term_rows(const Buffers& b, const Counts& c, const Penalty& pen, uint32_t t, uint32_t n,
-                                float J[4][6], float res[4], float lam2[4]) {
-    uint32_t k;
-    const int type = type_of(c, t, k);
-    if (type == kDense || type == kForeground) {
-        const ScalarRec& rec = type == kDense ? b.dense[k] : b.fg[k];
-        const float w = slot_weight(rec.knn, rec.w, n);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) J[0][q] = w * rec.J[q];
-        res[0] = rec.r;
-        lam2[0] = type == kDense ? 1.0f : pen.foreground * pen.foreground;
-        if (type == kDense && b.density && pen.density != 0.f) {
-            const ScalarRec& d = b.density[k];
-#pragma unroll
-            for (int q = 0; q < 6; ++q) J[1][q] = w * d.J[q];
-            res[1] = d.r;
-            lam2[1] = pen.density * pen.density;
-            return 2;
-        }
-        return 1;
-    }
-    if (type == kSmooth) {
-        const SmoothRec s = b.smooth[k];
-        smooth_rows(s, n == (s.ij & 0xFFFF), J, res);
-        lam2[0] = lam2[1] = lam2[2] = pen.smooth * pen.smooth;
-        return 3;
-    }
-    const P2PRec& rec = type == kFeature ? b.feat[k] : b.cross[k];
-    p2p_rows(rec, J, res);
-    const float w = slot_weight(rec.knn, rec.w, n);
-    const float l = type == kFeature ? pen.feature : pen.cross;
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-#pragma unroll
-        for (int q = 0; q < 6; ++q) J[ch][q] *= w;
-        lam2[ch] = l * l;
-    }
-    return 3;
-}
-
 // ------------------------------------------------ node-centred coordinates
 // The reference twist (w, v) moves a point p by w x p + v: its rotation is
 // about the world origin, so every node block has a near-null direction
@@ -313,123 +271,206 @@ __device__ inline void spd_inverse6(const float* A, float* out, double* out64) {
         }
 }
 
-// ------------------------------------------- diagonal blocks, J^T r, inverse
-// One wave per node over its Node2Term list: D_n = sum lambda^2 J^T J + 1e-2 I
-// (computeBlockDiagonalPreconditionerKernel, PreconditionerBlocked.cu:172-284),
-// b_n = -sum lambda^2 J^T r (computeJtResidualWithIndexKernel,
-// JtDotResidualIndexed.cu:116-224), then the block-Jacobi inverse
-// (blockDiagonalInverseKernel, BlockDiagonalPreconditionerInverse.cu:6-41).
+// ------------------------------------------------------ fused row assembly
+// One block per node row i walks the row's Node2Term list once and produces
+//   D_i  = sum lambda^2 J_i^T J_i + 1e-2 I
+//          (computeBlockDiagonalPreconditionerKernel, PreconditionerBlocked.cu:172-284),
+//   b_i  = -sum lambda^2 J_i^T r
+//          (computeJtResidualWithIndexKernel, JtDotResidualIndexed.cu:116-224),
+//   B_ij = sum lambda^2 J_i^T J_j for every pair (i, j), j > i, of the row
+//          (computeJtJNonDiagonalBlockKernel, JtJMaterializer.cu:270-365),
+// then the block-Jacobi inverse (blockDiagonalInverseKernel,
+// BlockDiagonalPreconditionerInverse.cu:6-41).  A term enters B_ij once per
+// Node2Term entry of row i times the number of its knn slots holding j -- the
+// multiplicity of the reference's NodePair2Term lists -- so no per-pair term
+// list is needed.  Batches of kAsmBatch entries are staged in LDS (node-frame
+// i-side rows, raw partner rows, residuals, weights, one coefficient per pair);
+// lane (u, a) then accumulates row a of block u (u = 0: the diagonal) over the
+// batch in term order.  Deterministic: a fixed summation order per output.
+constexpr int kAsmThreads = 128;
+constexpr int kAsmBatch = 64;
+constexpr int kAsmPairs = 20;   // pairs per pass: (1 + 20) * 6 = 126 lanes
+constexpr int kAsmStride = 63;  // floats per staged entry (odd: conflict-free LDS)
+constexpr int kOffJi = 0, kOffJp = 18, kOffRes = 36, kOffLam = 39, kOffRows = 42, kOffCoef = 43;
+static_assert(kOffCoef + kAsmPairs == kAsmStride, "staged entry layout");
+
+// Rows of term t seen from node i: i-side rows (slot weight of i applied), the
+// raw rows a partner node multiplies by its own weight (a graph edge: the rows
+// of the other end), residuals and lambda^2.  coef(j) for a partner j follows.
+struct RowTerm {
+    int rows;
+    float Ji[3][6], Jp[3][6], res[3], lam2[3];
+    bool smooth;
+    uint32_t partner;  // smooth: the other end (0xFFFFFFFF for a loop)
+    ushort4 knn;
+    float4 w;
+};
+
+__device__ inline void row_term(const Buffers& b, const Counts& c, const Penalty& pen, uint32_t t, uint32_t i,
+                                RowTerm& o) {
+    uint32_t k;
+    const int type = type_of(c, t, k);
+    o.smooth = false;
+    o.partner = 0xFFFFFFFFu;
+    if (type == kDense || type == kForeground) {
+        const ScalarRec& rec = type == kDense ? b.dense[k] : b.fg[k];
+        o.knn = rec.knn;
+        o.w = rec.w;
+        const float wi = slot_weight(rec.knn, rec.w, i);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            o.Jp[0][q] = rec.J[q];
+            o.Ji[0][q] = wi * rec.J[q];
+        }
+        o.res[0] = rec.r;
+        o.lam2[0] = type == kDense ? 1.0f : pen.foreground * pen.foreground;
+        o.rows = 1;
+        if (type == kDense && b.density && pen.density != 0.f) {
+            const ScalarRec& d = b.density[k];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                o.Jp[1][q] = d.J[q];
+                o.Ji[1][q] = wi * d.J[q];
+            }
+            o.res[1] = d.r;
+            o.lam2[1] = pen.density * pen.density;
+            o.rows = 2;
+        }
+        return;
+    }
+    if (type == kSmooth) {
+        const SmoothRec s = b.smooth[k];
+        const uint32_t first = s.ij & 0xFFFF, second = s.ij >> 16;
+        const bool is_first = i == first;
+        float r2[3];
+        smooth_rows(s, is_first, o.Ji, o.res);
+        smooth_rows(s, !is_first, o.Jp, r2);
+        o.smooth = true;
+        o.partner = first == second ? 0xFFFFFFFFu : (is_first ? second : first);
+        o.lam2[0] = o.lam2[1] = o.lam2[2] = pen.smooth * pen.smooth;
+        o.rows = 3;
+        return;
+    }
+    const P2PRec& rec = type == kFeature ? b.feat[k] : b.cross[k];
+    o.knn = rec.knn;
+    o.w = rec.w;
+    p2p_rows(rec, o.Jp, o.res);
+    const float wi = slot_weight(rec.knn, rec.w, i);
+    const float l = type == kFeature ? pen.feature : pen.cross;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) o.Ji[ch][q] = wi * o.Jp[ch][q];
+        o.lam2[ch] = l * l;
+    }
+    o.rows = 3;
+}
+
+// Weight of partner j in term o: (knn slots holding j) x slot weight of j.
+__device__ inline float partner_coef(const RowTerm& o, uint32_t j) {
+    if (o.smooth) return o.partner == j ? 1.0f : 0.0f;
+    const int cnt = (o.knn.x == j) + (o.knn.y == j) + (o.knn.z == j) + (o.knn.w == j);
+    return cnt ? (float)cnt * slot_weight(o.knn, o.w, j) : 0.0f;
+}
+
 template <typename InvT>
-__global__ void __launch_bounds__(256) node_blocks_kernel(Buffers b, Penalty pen) {
+__global__ void __launch_bounds__(kAsmThreads) assemble_kernel(Buffers b, Penalty pen) {
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-    for (uint32_t n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); n < f.N; n += waves) {
-        float acc[27];
+    const uint32_t i = blockIdx.x;
+    if (i >= f.N) return;
+    __shared__ float stage[kAsmBatch * kAsmStride];
+    __shared__ float sD[36];
+    __shared__ float4 sG[kAsmPairs + 1];
+    __shared__ uint32_t sJ[kAsmPairs];
+    const uint32_t tid = threadIdx.x;
+    const float4 gi = f.nodes[i];
+    const uint32_t beg = b.n2t_offset[i], end = b.n2t_offset[i + 1];
+    const uint32_t pbeg = b.pair_offset[i], U = b.pair_offset[i + 1] - pbeg;
+    const uint32_t passes = U == 0 ? 1u : (U + kAsmPairs - 1) / kAsmPairs;
+    const uint32_t u = tid / 6, a = tid % 6;
+    for (uint32_t pass = 0; pass < passes; ++pass) {
+        const uint32_t u0 = pass * kAsmPairs;
+        const uint32_t np = min((uint32_t)kAsmPairs, U - min(U, u0));
+        if (tid < np) {
+            const uint32_t j = b.pair_key[pbeg + u0 + tid] & 0xFFFF;
+            sJ[tid] = j;
+            sG[tid + 1] = f.nodes[j];
+        }
+        if (tid == 0) sG[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+        // lanes: u = 0 the diagonal (first pass only), u = 1..np the row's pairs
+        const bool active = tid < 6 * (1 + np) && (u > 0 || pass == 0);
+        float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        float accr = 0.f;
+        __syncthreads();
+        const float4 g = sG[u < np + 1 ? u : 0];
+        for (uint32_t base = beg; base < end; base += kAsmBatch) {
+            const uint32_t nb = min((uint32_t)kAsmBatch, end - base);
+            if (tid < nb) {
+                RowTerm o;
+                row_term(b, c, pen, b.n2t_index[base + tid], i, o);
+                float* E = stage + tid * kAsmStride;
 #pragma unroll
-        for (int q = 0; q < 27; ++q) acc[q] = 0.f;
-        const float4 g = f.nodes[n];
-        const uint32_t beg = b.n2t_offset[n], end = b.n2t_offset[n + 1];
-        for (uint32_t p = beg + lane; p < end; p += 64) {
-            float J[4][6], res[4], lam2[4];
-            const int rows = term_rows(b, c, pen, b.n2t_index[p], n, J, res, lam2);
-            for (int ch = 0; ch < rows; ++ch) {
-                to_node_frame(J[ch], g);
-                int o = 0;
+                for (int r = 0; r < 3; ++r) {
+                    if (r < o.rows) {
+                        to_node_frame(o.Ji[r], gi);
 #pragma unroll
-                for (int a = 0; a < 6; ++a) {
-#pragma unroll
-                    for (int k = a; k < 6; ++k) acc[o++] += lam2[ch] * J[ch][a] * J[ch][k];
+                        for (int q = 0; q < 6; ++q) {
+                            E[kOffJi + 6 * r + q] = o.Ji[r][q];
+                            E[kOffJp + 6 * r + q] = o.Jp[r][q];
+                        }
+                        E[kOffRes + r] = o.res[r];
+                        E[kOffLam + r] = o.lam2[r];
+                    }
                 }
-#pragma unroll
-                for (int a = 0; a < 6; ++a) acc[21 + a] -= lam2[ch] * J[ch][a] * res[ch];
+                E[kOffRows] = (float)o.rows;
+                for (uint32_t q = 0; q < np; ++q) E[kOffCoef + q] = partner_coef(o, sJ[q]);
             }
-        }
+            __syncthreads();
+            if (active) {
+                for (uint32_t e = 0; e < nb; ++e) {
+                    const float* E = stage + e * kAsmStride;
+                    const float coef = u == 0 ? 1.0f : E[kOffCoef + u - 1];
+                    if (coef == 0.f) continue;
+                    const int rows = (int)E[kOffRows];
+                    for (int r = 0; r < rows; ++r) {
+                        const float A = E[kOffLam + r] * E[kOffJi + 6 * r + a] * coef;
+                        const float* P = E + (u == 0 ? kOffJi : kOffJp) + 6 * r;
+                        float p[6] = {P[0], P[1], P[2], P[3], P[4], P[5]};
+                        // partner rows into the partner's node frame (identity for u = 0: g = 0)
+                        p[0] += p[4] * g.z - p[5] * g.y;
+                        p[1] += p[5] * g.x - p[3] * g.z;
+                        p[2] += p[3] * g.y - p[4] * g.x;
 #pragma unroll
-        for (int q = 0; q < 27; ++q) acc[q] = wave_sum(acc[q]);
-        float D[36];
-        {
-            int o = 0;
-#pragma unroll
-            for (int a = 0; a < 6; ++a)
-#pragma unroll
-                for (int k = a; k < 6; ++k) {
-                    D[6 * a + k] = acc[o];
-                    D[6 * k + a] = acc[o];
-                    ++o;
+                        for (int q = 0; q < 6; ++q) acc[q] += A * p[q];
+                        accr += A * E[kOffRes + r];
+                    }
                 }
-            add_node_frame_damping(D, g);
+            }
+            __syncthreads();
         }
-        float* Dout = b.diag + 36 * (size_t)n;
-        if (lane < 36) {
-            float val = 0.f;
+        if (active && u > 0) {
+            float* out = b.offdiag + 36 * (size_t)(pbeg + u0 + u - 1) + 6 * a;
 #pragma unroll
-            for (int q = 0; q < 36; ++q) val = (q == (int)lane) ? D[q] : val;
-            Dout[lane] = val;
+            for (int q = 0; q < 6; ++q) out[q] = acc[q];
         }
-        if (lane < 6) {
-            float val = 0.f;
+        if (active && u == 0) {
 #pragma unroll
-            for (int q = 0; q < 6; ++q) val = (q == (int)lane) ? acc[21 + q] : val;
-            b.jtr[6 * (size_t)n + lane] = val;
+            for (int q = 0; q < 6; ++q) sD[6 * a + q] = acc[q];
+            b.jtr[6 * (size_t)i + a] = -accr;
         }
-        if (lane == 0)
-            spd_inverse6<InvT>(D, b.diag_inv + 36 * (size_t)n,
-                               sizeof(InvT) == 8 ? b.diag_inv64 + 36 * (size_t)n : nullptr);
+        __syncthreads();
     }
-}
-
-// ----------------------------------------------------- off-diagonal blocks
-// kPairLanes lanes per unique pair (i < j): B_ij[a][b] = sum lambda^2 J_i[a] J_j[b]
-// over the pair's term list (computeJtJNonDiagonalBlockKernel,
-// JtJMaterializer.cu:270-365).  Stored once, row-major, row node i.
-constexpr int kPairLanes = 16;
-
-__global__ void __launch_bounds__(256) pair_blocks_kernel(Buffers b, Penalty pen) {
-    const Counts& c = *b.counts;
-    if (c.error) return;
-    const uint32_t sub = threadIdx.x % kPairLanes;
-    const uint32_t groups = gridDim.x * (blockDim.x / kPairLanes);
-    const uint32_t P = c.pairs;
-    for (uint32_t base = blockIdx.x * (blockDim.x / kPairLanes); base < P + 0; base += groups) {
-        const uint32_t p = base + threadIdx.x / kPairLanes;
-        float acc[36];
+    if (tid == 0) {
+        float D[36];
 #pragma unroll
-        for (int q = 0; q < 36; ++q) acc[q] = 0.f;
-        if (p < P) {
-            const uint32_t key = b.pair_key[p];
-            const uint32_t i = key >> 16, j = key & 0xFFFF;
-            const uint2 range = b.pair_range[p];
-            const float4 gi = b.frame->nodes[i], gj = b.frame->nodes[j];
-            for (uint32_t q = range.x + sub; q < range.y; q += kPairLanes) {
-                const uint32_t t = b.pair_terms[q];
-                float Ji[4][6], Jj[4][6], ri[4], rj[4], li[4], lj[4];
-                const int rows = term_rows(b, c, pen, t, i, Ji, ri, li);
-                term_rows(b, c, pen, t, j, Jj, rj, lj);
-                for (int ch = 0; ch < rows; ++ch) {
-                    to_node_frame(Ji[ch], gi);
-                    to_node_frame(Jj[ch], gj);
-                }
-                for (int ch = 0; ch < rows; ++ch)
+        for (int q = 0; q < 36; ++q) D[q] = sD[q];
+        add_node_frame_damping(D, gi);
+        float* Dout = b.diag + 36 * (size_t)i;
 #pragma unroll
-                    for (int a = 0; a < 6; ++a)
-#pragma unroll
-                        for (int k = 0; k < 6; ++k) acc[6 * a + k] += li[ch] * Ji[ch][a] * Jj[ch][k];
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 36; ++q) {
-#pragma unroll
-            for (int o = kPairLanes / 2; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
-        }
-        if (p < P) {
-            float* out = b.offdiag + 36 * (size_t)p;
-#pragma unroll
-            for (int q = 0; q < 36; ++q)
-                if ((uint32_t)(q % kPairLanes) == sub) out[q] = acc[q];
-        }
+        for (int q = 0; q < 36; ++q) Dout[q] = D[q];
+        spd_inverse6<InvT>(D, b.diag_inv + 36 * (size_t)i, sizeof(InvT) == 8 ? b.diag_inv64 + 36 * (size_t)i : nullptr);
     }
 }
 
@@ -480,22 +521,11 @@ void launch_eval_terms(const Buffers& b, const DQ* se3, bool fp64_geometry, hipS
         hipLaunchKernelGGL(eval_terms_kernel<float>, dim3(1024), dim3(256), 0, stream, b, se3);
 }
 
-void launch_node_blocks(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream) {
-    const uint32_t node_blocks = (cap_nodes + 3) / 4;
-    if (fp64_inverse)
-        hipLaunchKernelGGL(node_blocks_kernel<double>, dim3(node_blocks), dim3(256), 0, stream, b, pen);
-    else
-        hipLaunchKernelGGL(node_blocks_kernel<float>, dim3(node_blocks), dim3(256), 0, stream, b, pen);
-}
-
-void launch_pair_blocks(const Buffers& b, const Penalty& pen, hipStream_t stream) {
-    const uint32_t pair_blocks = std::min<uint32_t>((b.cap_pairs + 15) / 16, 4096u);
-    hipLaunchKernelGGL(pair_blocks_kernel, dim3(pair_blocks), dim3(256), 0, stream, b, pen);
-}
-
 void launch_assemble(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream) {
-    launch_node_blocks(b, pen, fp64_inverse, cap_nodes, stream);
-    launch_pair_blocks(b, pen, stream);
+    if (fp64_inverse)
+        hipLaunchKernelGGL(assemble_kernel<double>, dim3(cap_nodes), dim3(kAsmThreads), 0, stream, b, pen);
+    else
+        hipLaunchKernelGGL(assemble_kernel<float>, dim3(cap_nodes), dim3(kAsmThreads), 0, stream, b, pen);
 }
 
 // Returns the number of partials written to slot kSlotCost.

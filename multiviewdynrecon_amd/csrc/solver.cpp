@@ -12,13 +12,14 @@
 #include "common.h"
 
 namespace mvdr {
-void launch_index_build(const Buffers& b, uint32_t cap_tiles, uint32_t cap_nodes, bool use_foreground, bool use_cross,
-                        hipStream_t stream);
+void launch_index_build(const Buffers& b, uint32_t cap_tiles, uint32_t cap_nodes, uint32_t cap_chunks,
+                        bool use_foreground, bool use_cross, hipStream_t stream);
+uint32_t index_chunks(uint64_t max_terms);
+void launch_pair_terms(const Buffers& b, uint32_t* pair_terms, hipStream_t stream);
 void launch_eval_terms(const Buffers& b, const DQ* se3, bool fp64_geometry, hipStream_t stream);
 void launch_assemble(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream);
-void launch_node_blocks(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream);
-void launch_pair_blocks(const Buffers& b, const Penalty& pen, hipStream_t stream);
 void launch_spmv_once(const Buffers& b, bool fp64, uint32_t cap_nodes, hipStream_t stream);
+void launch_pcg_update_once(const Buffers& b, bool fp64, hipStream_t stream);
 uint32_t launch_cost(const Buffers& b, const Penalty& pen, bool density_in_jtj, hipStream_t stream);
 void launch_pcg(const Buffers& b, uint32_t iters, uint32_t refine, bool fp64, uint32_t cap_nodes, hipStream_t stream);
 void launch_apply_update(const Buffers& b, const DQ* in, DQ* out, hipStream_t stream);
@@ -34,6 +35,7 @@ struct mvdr_solver {
     FrameDev frame_host{};
     std::vector<void*> allocations;
     uint32_t cap_tiles = 0;
+    uint32_t cap_chunks = 0;            // Node2Term counting-sort chunks
     bool has_input = false;
     bool indexed = false;
     uint32_t iter = 0;                  // GN iterations done on the current frame
@@ -41,12 +43,15 @@ struct mvdr_solver {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t graph_stream = nullptr;
+    uint32_t* pair_terms = nullptr;     // NodePair2Term lists, built on request (taps only)
+    size_t pair_terms_cap = 0;
     std::string error;
 
     ~mvdr_solver() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (done) (void)hipEventDestroy(done);
+        if (pair_terms) (void)hipFree(pair_terms);
         for (void* p : allocations) (void)hipFree(p);
     }
 };
@@ -101,8 +106,7 @@ const char* error_text(uint32_t bits) {
     if (bits & kErrKv1) return "node-to-term entries exceed capacity";
     if (bits & kErrKv2) return "node-pair term entries exceed capacity";
     if (bits & kErrPairs) return "unique node pairs exceed 64 * max_nodes";
-    if (bits & kErrSeg) return "a node's term list exceeds the 8192-entry sort capacity";
-    if (bits & kErrRow) return "a node-pair row exceeds the 4096-entry sort capacity";
+    if (bits & kErrRow) return "a node shares terms with more than 512 other nodes";
     if (bits & kErrNodes) return "an input node / surfel index is out of range";
     return "capacity exceeded";
 }
@@ -112,7 +116,8 @@ void enqueue_solve(mvdr_solver* s, hipStream_t stream, bool index, uint32_t firs
     const bool fp64 = c.precision == MVDR_PRECISION_MIXED;
     if (index) {
         launch_copy_se3(s->b, s->b.se3[0], stream);
-        launch_index_build(s->b, s->cap_tiles, c.max_nodes, c.use_foreground != 0, c.use_cross_view != 0, stream);
+        launch_index_build(s->b, s->cap_tiles, c.max_nodes, s->cap_chunks, c.use_foreground != 0,
+                           c.use_cross_view != 0, stream);
     }
     for (uint32_t k = first_iter; k < first_iter + iters; ++k) {
         const Penalty pen = kernel_penalty(c, k);
@@ -273,6 +278,7 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     b.cap_kv2 = (uint32_t)kv2;
     b.cap_pairs = (uint32_t)std::min<uint64_t>(kv2, 64ull * c.max_nodes);
     s->cap_tiles = (uint32_t)((pixels + 4095) / 4096);
+    s->cap_chunks = index_chunks(knn_terms + b.cap_edges);
     const uint32_t N1 = c.max_nodes + 1;
 
     hipError_t e = hipSuccess;
@@ -294,22 +300,16 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     A(b.tile_count, s->cap_tiles);
     A(b.tile_offset, s->cap_tiles);
     A(b.feat_flag, 1);
+    A(b.n2t_hist, (size_t)s->cap_chunks * c.max_nodes);
     A(b.n2t_count, N1);
     A(b.n2t_offset, N1);
-    A(b.n2t_cursor, N1);
     A(b.n2t_index, b.cap_kv1);
-    A(b.row_count, N1);
-    A(b.row_offset, N1);
-    A(b.row_pairs, N1);
+    A(b.row_up, N1);
+    A(b.row_low, N1);
     A(b.pair_offset, N1);
-    A(b.uniq_j, b.cap_kv2);
-    A(b.uniq_range, b.cap_kv2);
-    A(b.pair_terms, b.cap_kv2);
     A(b.pair_key, b.cap_pairs);
     A(b.pair_range, b.cap_pairs);
-    A(b.low_count, N1);
     A(b.low_offset, N1);
-    A(b.low_cursor, N1);
     A(b.low_ent, b.cap_pairs);
     A(b.diag, 36 * (size_t)c.max_nodes);
     A(b.diag_inv, 36 * (size_t)c.max_nodes);
@@ -542,8 +542,29 @@ mvdr_status mvdr_solver_debug_get(mvdr_solver* s, int tap, void* dst, size_t byt
         case MVDR_TAP_NODE2TERM_OFFSET: src = b.n2t_offset; size = 4ull * (N + 1); break;
         case MVDR_TAP_NODE2TERM_INDEX: src = b.n2t_index; size = 4ull * c.kv1; break;
         case MVDR_TAP_NODEPAIR_KEY: src = b.pair_key; size = 4ull * c.pairs; break;
-        case MVDR_TAP_NODEPAIR_RANGE: src = b.pair_range; size = 8ull * c.pairs; break;
-        case MVDR_TAP_NODEPAIR_TERM_INDEX: src = b.pair_terms; size = 4ull * c.kv2; break;
+        case MVDR_TAP_NODEPAIR_RANGE:
+        case MVDR_TAP_NODEPAIR_TERM_INDEX: {
+            // the solve does not use per-pair term lists: build them now
+            if (c.error) return fail(s, MVDR_E_CAPACITY, error_text(c.error));
+            if (s->pair_terms_cap < c.kv2) {
+                if (s->pair_terms) HIP_TRY(s, hipFree(s->pair_terms));
+                s->pair_terms = nullptr;
+                s->pair_terms_cap = 0;
+                HIP_TRY(s, hipMalloc(&s->pair_terms, 4ull * std::max<uint32_t>(c.kv2, 1)));
+                s->pair_terms_cap = std::max<uint32_t>(c.kv2, 1);
+            }
+            launch_pair_terms(b, s->pair_terms, nullptr);
+            HIP_TRY(s, hipStreamSynchronize(nullptr));
+            HIP_TRY(s, hipGetLastError());
+            if (tap == MVDR_TAP_NODEPAIR_RANGE) {
+                src = b.pair_range;
+                size = 8ull * c.pairs;
+            } else {
+                src = s->pair_terms;
+                size = 4ull * c.kv2;
+            }
+            break;
+        }
         case MVDR_TAP_DENSE_TERMS: src = b.dense; size = sizeof(ScalarRec) * (size_t)c.dense; break;
         case MVDR_TAP_SMOOTH_TERMS: src = b.smooth; size = sizeof(SmoothRec) * (size_t)c.smooth; break;
         case MVDR_TAP_FEATURE_TERMS: src = b.feat; size = sizeof(P2PRec) * (size_t)c.feat; break;
@@ -602,8 +623,8 @@ mvdr_status mvdr_solver_bench_stage(mvdr_solver* s, int stage, uint32_t reps, vo
     for (uint32_t r = 0; r < reps; ++r) {
         switch (stage) {
             case MVDR_STAGE_EVAL_TERMS: launch_eval_terms(s->b, s->b.se3[k & 1], s->cfg.precision == MVDR_PRECISION_MIXED, st); break;
-            case MVDR_STAGE_NODE_BLOCKS: launch_node_blocks(s->b, pen, fp64, c.max_nodes, st); break;
-            case MVDR_STAGE_PAIR_BLOCKS: launch_pair_blocks(s->b, pen, st); break;
+            case MVDR_STAGE_ASSEMBLE: launch_assemble(s->b, pen, fp64, c.max_nodes, st); break;
+            case MVDR_STAGE_PCG_UPDATE: launch_pcg_update_once(s->b, fp64, st); break;
             case MVDR_STAGE_SPMV: launch_spmv_once(s->b, fp64, c.max_nodes, st); break;
             case MVDR_STAGE_INDEX: enqueue_solve(s, st, true, 0, 0); break;
             default:
