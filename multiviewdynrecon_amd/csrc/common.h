@@ -313,12 +313,20 @@ struct Buffers {
     uint32_t* n2t_count;     // N + 1
     uint32_t* n2t_offset;    // N + 1
     uint32_t* n2t_index;     // kv1
+    uint16_t* n2t_node;      // kv1: the node of each Node2Term entry
     // node pairs (i, j), i < j, in (i, j) order
     uint32_t* row_up;        // N + 1 : pairs (i, j > i) per row i
     uint32_t* row_low;       // N + 1 : pairs (k < i, i) per row i
+    uint32_t* row_kv2;       // N + 1 : pair-term entries of row i's pairs
     uint32_t* pair_offset;   // N + 1 : first pair of row i
+    uint32_t* kv2_offset;    // N + 1 : first pair-term entry of row i
     uint32_t* pair_key;      // cap_pairs: i << 16 | j
-    uint2* pair_range;       // cap_pairs: term-list range (built on request)
+    uint2* pair_range;       // cap_pairs: [begin, end) into pair_terms
+    uint32_t* pair_terms;    // kv2: NodePair2Term term lists
+    uint32_t* kv2_pair;      // kv2: the pair of each entry
+    // assembly partials of runs crossing 64-entry chunk boundaries
+    float* part_diag;        // (kv1 / 64 + 1) x 2 x 27
+    float* part_pair;        // (kv2 / 64 + 1) x 2 x 36
     // block-sparse rows
     uint32_t* low_offset;    // N + 1
     uint2* low_ent;          // cap_pairs: (k, pair) of row i's lower entries, k ascending

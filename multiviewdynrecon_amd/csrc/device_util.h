@@ -11,6 +11,14 @@ __device__ inline uint32_t lane_prefix(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// Orders this wave's LDS accesses for its other lanes (single-wave blocks):
+// waits for the wave's outstanding LDS operations only.  A workgroup barrier
+// would also wait for every outstanding global store.
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0), vmcnt / expcnt untouched
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <typename T>
 __device__ inline T wave_sum(T v) {
 #pragma unroll

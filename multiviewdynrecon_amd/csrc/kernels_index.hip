@@ -84,6 +84,7 @@ __global__ void __launch_bounds__(kTileThreads) compact_count_kernel(Buffers b, 
             b.n2t_count[i] = 0;
             b.row_up[i] = 0;
             b.row_low[i] = 0;
+            b.row_kv2[i] = 0;
         }
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             Counts* c = b.counts;
@@ -405,7 +406,7 @@ __global__ void __launch_bounds__(64) n2t_hist_kernel(Buffers b, uint32_t tile) 
     for (uint32_t n0 = 0; n0 < N; n0 += tile) {
         const uint32_t span = min(tile, N - n0);
         for (uint32_t k = threadIdx.x; k < span; k += 64) hist[k] = 0;
-        __syncthreads();
+        wave_lds_sync();
         for (uint32_t t = t0 + threadIdx.x; t < t1; t += 64) {
             ushort4 k;
             const int m = term_nodes(b, f, off, t, k);
@@ -420,40 +421,45 @@ __global__ void __launch_bounds__(64) n2t_hist_kernel(Buffers b, uint32_t tile) 
                 }
             }
         }
-        __syncthreads();
+        wave_lds_sync();
         for (uint32_t k = threadIdx.x; k < span; k += 64) H[n0 + k] = hist[k];
-        __syncthreads();
+        wave_lds_sync();
     }
 }
 
-// One lane per node, four waves splitting the chunk range: H[c][n] becomes the
+// One lane per node, 16 waves splitting the chunk range: H[c][n] becomes the
 // number of node-n entries in chunks before c; n2t_count[n] the node's total.
-__global__ void __launch_bounds__(256) n2t_colscan_kernel(Buffers b) {
+constexpr uint32_t kScanWaves = 16;
+
+__global__ void __launch_bounds__(64 * kScanWaves) n2t_colscan_kernel(Buffers b) {
     const uint32_t N = b.frame->N;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t n = blockIdx.x * 64 + lane;
     const uint32_t C = num_chunks(*b.counts);
-    const uint32_t per = (C + 3) / 4;
+    const uint32_t per = (C + kScanWaves - 1) / kScanWaves;
     const uint32_t c0 = min(C, w * per), c1 = min(C, c0 + per);
-    __shared__ uint32_t part[4][64];
+    __shared__ uint32_t part[kScanWaves][64];
     uint32_t sum = 0;
-    if (n < N)
+    if (n < N) {
+#pragma unroll 8
         for (uint32_t ch = c0; ch < c1; ++ch) sum += b.n2t_hist[(size_t)ch * N + n];
+    }
     part[w][lane] = sum;
     __syncthreads();
     uint32_t run = 0, total = 0;
-    for (uint32_t k = 0; k < 4; ++k) {
+    for (uint32_t k = 0; k < kScanWaves; ++k) {
         run += k < w ? part[k][lane] : 0u;
         total += part[k][lane];
     }
     if (n >= N) return;
+#pragma unroll 8
     for (uint32_t ch = c0; ch < c1; ++ch) {
         uint32_t* h = b.n2t_hist + (size_t)ch * N + n;
         const uint32_t v = *h;
         *h = run;
         run += v;
     }
-    if (w == 3) b.n2t_count[n] = total;
+    if (w == kScanWaves - 1) b.n2t_count[n] = total;
 }
 
 __global__ void __launch_bounds__(64) n2t_place_kernel(Buffers b, uint32_t tile) {
@@ -465,65 +471,70 @@ __global__ void __launch_bounds__(64) n2t_place_kernel(Buffers b, uint32_t tile)
     const uint32_t t1 = min(c.off[4], t0 + kChunkTerms);
     const uint32_t N = f.N;
     const uint32_t lane = threadIdx.x;
-    const uint64_t below = (1ull << lane) - 1ull;
     extern __shared__ uint32_t cur[];
+    __shared__ uint4 round_nodes[64];  // entry (lane, slot) of the current round
     const uint32_t* H = b.n2t_hist + (size_t)blockIdx.x * N;
     for (uint32_t n0 = 0; n0 < N; n0 += tile) {
         const uint32_t span = min(tile, N - n0);
+#pragma unroll 8
         for (uint32_t k = lane; k < span; k += 64) cur[k] = b.n2t_offset[n0 + k] + H[n0 + k];
-        __syncthreads();
         for (uint32_t tb = t0; tb < t1; tb += 64) {
             const uint32_t t = tb + lane;
             ushort4 k = make_ushort4(0, 0, 0, 0);
             const int m = t < t1 ? term_nodes(b, f, c.off, t, k) : 0;
-            const uint32_t n[4] = {k.x, k.y, k.z, k.w};
-            bool live[4];
+            uint32_t n[4] = {k.x, k.y, k.z, k.w};
 #pragma unroll
-            for (int q = 0; q < 4; ++q) live[q] = q < m && n[q] >= n0 && n[q] - n0 < span;
-            uint64_t pend[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) pend[q] = __ballot(live[q]);
-            // one distinct node per iteration; entries ordered by (term, slot)
-            while (pend[0] | pend[1] | pend[2] | pend[3]) {
-                const int q0 = pend[0] ? 0 : (pend[1] ? 1 : (pend[2] ? 2 : 3));
-                const int leader = __builtin_ctzll(pend[q0]);
-                const uint32_t v = (uint32_t)__shfl((int)slot(k, q0), leader, 64);
-                uint64_t hit[4];
-                uint32_t total = 0, before = 0;
+            for (int q = 0; q < 4; ++q)
+                if (!(q < m && n[q] >= n0 && n[q] - n0 < span)) n[q] = 0xFFFFFFFFu;  // not placed here
+            round_nodes[lane] = make_uint4(n[0], n[1], n[2], n[3]);
+            wave_lds_sync();
+            // stable rank among the round's entries of the same node, ordered by
+            // (term, slot); `after` marks the node's last entry of the round
+            uint32_t before[4] = {0, 0, 0, 0}, after[4] = {0, 0, 0, 0};
+            for (uint32_t l = 0; l < 64; ++l) {
+                const uint4 o = round_nodes[l];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    hit[q] = __ballot(live[q] && n[q] == v);
-                    total += (uint32_t)__popcll(hit[q]);
-                    before += (uint32_t)__popcll(hit[q] & below);
+                    const uint32_t eq = (o.x == n[q]) + (o.y == n[q]) + (o.z == n[q]) + (o.w == n[q]);
+                    before[q] += l < lane ? eq : 0u;
+                    after[q] += l > lane ? eq : 0u;
                 }
-                const uint32_t base = cur[v - n0];
-                uint32_t own = 0;
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (live[q] && n[q] == v) {
-                        b.n2t_index[base + before + own] = t;
-                        ++own;
-                        live[q] = false;
-                    }
-                __syncthreads();
-                if (lane == 0) cur[v - n0] = base + total;
-                __syncthreads();
-#pragma unroll
-                for (int q = 0; q < 4; ++q) pend[q] &= ~hit[q];
             }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq) {
+                    before[q] += (qq < q && n[qq] == n[q]) ? 1u : 0u;
+                    after[q] += (qq > q && n[qq] == n[q]) ? 1u : 0u;
+                }
+            uint32_t base[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                base[q] = 0;
+                if (n[q] != 0xFFFFFFFFu) {
+                    base[q] = cur[n[q] - n0];
+                    b.n2t_index[base[q] + before[q]] = t;
+                    b.n2t_node[base[q] + before[q]] = (uint16_t)n[q];
+                }
+            }
+            wave_lds_sync();  // every lane has read cur before the updates
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (n[q] != 0xFFFFFFFFu && after[q] == 0) cur[n[q] - n0] = base[q] + before[q] + 1;
+            wave_lds_sync();
         }
-        __syncthreads();
     }
 }
 
-// Exclusive scans over the N node counters.  which: 0 = Node2Term lists,
-// 1 = upper pairs per row (block 0) and lower entries per row (block 1).
+// Exclusive scans over the N node counters.  which = 0: Node2Term lists;
+// which = 1: pairs (i, j > i) per row (block 0), lower entries per row
+// (block 1), pair-term entries per row (block 2).
 __global__ void __launch_bounds__(1024) node_scan_kernel(Buffers b, int which) {
     const uint32_t N = b.frame->N;
     Counts& c = *b.counts;
     const int job = which == 0 ? 0 : 1 + (int)blockIdx.x;
-    const uint32_t* in = job == 0 ? b.n2t_count : (job == 1 ? b.row_up : b.row_low);
-    uint32_t* out = job == 0 ? b.n2t_offset : (job == 1 ? b.pair_offset : b.low_offset);
+    const uint32_t* in = job == 0 ? b.n2t_count : (job == 1 ? b.row_up : (job == 2 ? b.row_low : b.row_kv2));
+    uint32_t* out = job == 0 ? b.n2t_offset : (job == 1 ? b.pair_offset : (job == 2 ? b.low_offset : b.kv2_offset));
     uint32_t mx = 0;
     const uint32_t total = block_exclusive_scan_1024(in, out, N, nullptr, &mx);
     if (threadIdx.x == 0) {
@@ -535,6 +546,10 @@ __global__ void __launch_bounds__(1024) node_scan_kernel(Buffers b, int which) {
         } else if (job == 1) {
             if (total > b.cap_pairs) atomicOr(&c.error, kErrPairs);
             c.pairs = total;
+            c.max_row = mx;
+        } else if (job == 3) {
+            if (total > b.cap_kv2) atomicOr(&c.error, kErrKv2);
+            c.kv2 = total;
         }
     }
 }
@@ -544,7 +559,8 @@ __global__ void __launch_bounds__(1024) node_scan_kernel(Buffers b, int which) {
 // reference's NodePair2Term index (NodePair2TermsIndex.cu:11-185) -- and, per
 // block row, the lower-triangle entries (NodePairRowColumnIndex.cu:19-179).
 // One wave per row i collects the partner nodes of its Node2Term list in an LDS
-// hash set, then emits them ascending (wave minimum per step).
+// hash set, then emits them ascending (wave minimum per step).  The blocks are
+// single waves, so LDS traffic between lanes needs no workgroup barrier.
 constexpr uint32_t kRowHash = 512;  // partner set capacity per row
 
 // Partners of node i in term t with the reference's pair multiplicity: every
@@ -557,21 +573,47 @@ __device__ inline ushort4 term_partners(const Buffers& b, const FrameDev& f, con
     return make_ushort4(k.x != i ? k.x : kNone, k.y != i ? k.y : kNone, k.z != i ? k.z : kNone, k.w != i ? k.w : kNone);
 }
 
+__device__ inline uint32_t lower_bound_lds(const uint32_t* a, uint32_t n, uint32_t v) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// kWrite = false: pairs above / below the diagonal and pair-term entries of
+// row i.  kWrite = true: the row's pair keys, lower entries, and its pair-term
+// lists (NodePair2Term) with their ranges: for pair (i, j) every Node2Term
+// entry of row i once per knn slot holding j, in Node2Term order -- the
+// order of the reference's stable key sort.  kRowWaves waves per row; wave w
+// places a contiguous w-th share of the row's entries.
+constexpr uint32_t kRowWaves = 4;
+
 template <bool kWrite>
-__global__ void __launch_bounds__(64) pair_rows_kernel(Buffers b) {
+__global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(Buffers b) {
     const FrameDev& f = *b.frame;
     Counts& c = *b.counts;
     if (c.error) return;
     const uint32_t i = blockIdx.x;
     if (i >= f.N) return;
-    const uint32_t lane = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
     __shared__ uint32_t set[kRowHash];
-    for (uint32_t k = lane; k < kRowHash; k += 64) set[k] = 0xFFFFFFFFu;
+    __shared__ uint32_t ul[kRowHash];             // ascending partners j > i
+    __shared__ uint32_t ll[kRowHash];             // ascending partners k < i
+    __shared__ uint32_t cnt[kRowWaves][kRowHash]; // entries per pair and wave, then next positions
+    __shared__ uint32_t start[kRowHash + 1];      // pair-term range starts
+    __shared__ uint32_t part[kRowWaves];
+    __shared__ uint32_t nup, nlow, ovf;
+    for (uint32_t k = tid; k < kRowHash; k += 64 * kRowWaves) set[k] = 0xFFFFFFFFu;
+    if (tid == 0) ovf = 0;
     __syncthreads();
     const uint32_t beg = b.n2t_offset[i], end = b.n2t_offset[i + 1];
     uint32_t upper_entries = 0;
     bool overflow = false;
-    for (uint32_t p = beg + lane; p < end; p += 64) {
+    for (uint32_t p = beg + tid; p < end; p += 64 * kRowWaves) {
         const ushort4 o = term_partners(b, f, c.off, b.n2t_index[p], i);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -590,46 +632,134 @@ __global__ void __launch_bounds__(64) pair_rows_kernel(Buffers b) {
             }
         }
     }
-    if (__ballot(overflow)) {
-        if (lane == 0) atomicOr(&c.error, kErrRow);
+    if (overflow) ovf = 1;
+    upper_entries = wave_sum(upper_entries);
+    if (lane == 0) part[w] = upper_entries;
+    __syncthreads();
+    if (ovf) {
+        if (tid == 0) atomicOr(&c.error, kErrRow);
         return;
     }
+    if (w == 0) {
+        // ascending extraction, one partner per step (wave minimum)
+        uint32_t mine[kRowHash / 64];
+#pragma unroll
+        for (int k = 0; k < (int)(kRowHash / 64); ++k) mine[k] = set[lane + 64 * k];
+        uint32_t up = 0, low = 0;
+        int64_t last = -1;
+        for (;;) {
+            uint32_t m = 0xFFFFFFFFu;
+#pragma unroll
+            for (int k = 0; k < (int)(kRowHash / 64); ++k)
+                if ((int64_t)mine[k] > last && mine[k] < m) m = mine[k];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const uint32_t u = (uint32_t)__shfl_xor((int)m, o, 64);
+                m = u < m ? u : m;
+            }
+            if (m == 0xFFFFFFFFu) break;
+            if (m > i) ul[up++] = m;  // same value from every lane
+            else ll[low++] = m;
+            last = m;
+        }
+        if (lane == 0) {
+            nup = up;
+            nlow = low;
+        }
+    }
     __syncthreads();
-    uint32_t mine[kRowHash / 64];
-#pragma unroll
-    for (int k = 0; k < (int)(kRowHash / 64); ++k) mine[k] = set[lane + 64 * k];
-    uint32_t up = 0, low = 0;
-    const uint32_t up_base = kWrite ? b.pair_offset[i] : 0u, low_base = kWrite ? b.low_offset[i] : 0u;
-    int64_t last = -1;
-    for (;;) {
-        uint32_t m = 0xFFFFFFFFu;
-#pragma unroll
-        for (int k = 0; k < (int)(kRowHash / 64); ++k)
-            if ((int64_t)mine[k] > last && mine[k] < m) m = mine[k];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const uint32_t u = (uint32_t)__shfl_xor((int)m, o, 64);
-            m = u < m ? u : m;
-        }
-        if (m == 0xFFFFFFFFu) break;
-        if (m > i) {
-            if (kWrite && lane == 0) b.pair_key[up_base + up] = (i << 16) | m;
-            ++up;
-        } else {
-            if (kWrite && lane == 0) b.low_ent[low_base + low] = make_uint2(m, 0u);
-            ++low;
-        }
-        last = m;
-    }
-    if (!kWrite && lane == 0) {
-        b.row_up[i] = up;
-        b.row_low[i] = low;
-        atomicMax(&c.max_row, up + low);
-    }
+    const uint32_t up = nup, low = nlow;
     if (!kWrite) {
-        upper_entries = wave_sum(upper_entries);
-        if (lane == 0) atomicAdd(&c.kv2, upper_entries);
+        if (tid == 0) {
+            b.row_up[i] = up;
+            b.row_low[i] = low;
+            uint32_t s2 = 0;
+            for (uint32_t k = 0; k < kRowWaves; ++k) s2 += part[k];
+            b.row_kv2[i] = s2;
+        }
+        return;
     }
+    const uint32_t up_base = b.pair_offset[i], low_base = b.low_offset[i];
+    const uint32_t len = end - beg, per = (len + kRowWaves - 1) / kRowWaves;
+    const uint32_t wb = beg + min(len, w * per), we = beg + min(len, (w + 1) * per);
+    uint32_t* run = cnt[w];
+    if (up > 0) {
+        // pass 1: this wave's entries per pair
+        for (uint32_t u = lane; u < up; u += 64) run[u] = 0;
+        wave_lds_sync();
+        for (uint32_t p = wb + lane; p < we; p += 64) {
+            const ushort4 o = term_partners(b, f, c.off, b.n2t_index[p], i);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t v = slot(o, q);
+                if (v != 0xFFFF && v > i) atomicAdd(&run[lower_bound_lds(ul, up, v)], 1u);
+            }
+        }
+        __syncthreads();
+        // per pair: range start, then every wave's first position
+        if (tid == 0) {
+            uint32_t acc = b.kv2_offset[i];
+            for (uint32_t u = 0; u < up; ++u) {
+                start[u] = acc;
+                for (uint32_t k = 0; k < kRowWaves; ++k) {
+                    const uint32_t n = cnt[k][u];
+                    cnt[k][u] = acc;
+                    acc += n;
+                }
+            }
+            start[up] = acc;
+        }
+        __syncthreads();
+        for (uint32_t u = tid; u < up; u += 64 * kRowWaves)
+            b.pair_range[up_base + u] = make_uint2(start[u], start[u + 1]);
+        // pass 2: entries in Node2Term order, slot order within an entry
+        for (uint32_t base = wb; base < we; base += 64) {
+            const uint32_t p = base + lane;
+            uint32_t t = 0;
+            uint32_t su[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+            if (p < we) {
+                t = b.n2t_index[p];
+                const ushort4 o = term_partners(b, f, c.off, t, i);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t v = slot(o, q);
+                    if (v != 0xFFFF && v > i) su[q] = lower_bound_lds(ul, up, v);
+                }
+            }
+            uint64_t pend = __ballot(su[0] != 0xFFFFFFFFu || su[1] != 0xFFFFFFFFu || su[2] != 0xFFFFFFFFu ||
+                                     su[3] != 0xFFFFFFFFu);
+            while (pend) {
+                const int leader = __builtin_ctzll(pend);
+                const uint32_t lsu = su[0] != 0xFFFFFFFFu ? su[0]
+                                   : (su[1] != 0xFFFFFFFFu ? su[1] : (su[2] != 0xFFFFFFFFu ? su[2] : su[3]));
+                const uint32_t u = (uint32_t)__shfl((int)lsu, leader, 64);
+                uint64_t hit[4];
+                uint32_t total = 0, before = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    hit[q] = __ballot(su[q] == u);
+                    total += (uint32_t)__popcll(hit[q]);
+                    before += (uint32_t)__popcll(hit[q] & below);
+                }
+                uint32_t at = 0;
+                if ((int)lane == leader) at = atomicAdd(&run[u], total);
+                at = (uint32_t)__shfl((int)at, leader, 64);
+                uint32_t own = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (su[q] == u) {
+                        b.pair_terms[at + before + own] = t;
+                        b.kv2_pair[at + before + own] = up_base + u;
+                        ++own;
+                        su[q] = 0xFFFFFFFFu;
+                    }
+                pend = __ballot(su[0] != 0xFFFFFFFFu || su[1] != 0xFFFFFFFFu || su[2] != 0xFFFFFFFFu ||
+                                su[3] != 0xFFFFFFFFu);
+            }
+        }
+    }
+    for (uint32_t u = tid; u < up; u += 64 * kRowWaves) b.pair_key[up_base + u] = (i << 16) | ul[u];
+    for (uint32_t k = tid; k < low; k += 64 * kRowWaves) b.low_ent[low_base + k] = make_uint2(ll[k], 0u);
 }
 
 // Pair index of every lower entry (k, .) of row j: the position of (k, j) in
@@ -655,63 +785,6 @@ __global__ void __launch_bounds__(256) pair_link_kernel(Buffers b) {
     }
 }
 
-// ------------------------------------------ NodePair2Term term lists (taps)
-// The per-pair term lists of the reference's NodePair2Term index
-// (NodePair2TermsIndex.cu:11-185): for pair (i, j), every term of node i's
-// list once per knn slot holding i (the Node2Term multiplicity) times the
-// number of slots holding j; graph edges once.  The solve does not need them
-// (assembly walks Node2Term rows); they are built on request for inspection.
-__device__ inline uint32_t partner_count(const Buffers& b, const FrameDev& f, const uint32_t off[5], uint32_t t,
-                                         uint32_t i, uint32_t j) {
-    const ushort4 o = term_partners(b, f, off, t, i);
-    return (o.x == j) + (o.y == j) + (o.z == j) + (o.w == j);
-}
-
-__global__ void __launch_bounds__(256) pair_terms_kernel(Buffers b, uint32_t* pair_terms, int fill) {
-    const FrameDev& f = *b.frame;
-    const Counts& c = *b.counts;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < c.pairs; p += stride) {
-        const uint32_t key = b.pair_key[p];
-        const uint32_t i = key >> 16, j = key & 0xFFFF;
-        uint32_t at = fill ? b.pair_range[p].x : 0u;
-        for (uint32_t q = b.n2t_offset[i]; q < b.n2t_offset[i + 1]; ++q) {
-            const uint32_t t = b.n2t_index[q];
-            const uint32_t m = partner_count(b, f, c.off, t, i, j);
-            if (fill)
-                for (uint32_t r = 0; r < m; ++r) pair_terms[at + r] = t;
-            at += m;
-        }
-        if (!fill) b.pair_range[p].y = at;  // count, turned into ranges by pair_ranges_kernel
-    }
-}
-
-__global__ void __launch_bounds__(1024) pair_ranges_kernel(Buffers b) {
-    __shared__ uint32_t wave_tot[16];
-    const uint32_t P = b.counts->pairs;
-    // sequential chunks per thread, as block_exclusive_scan_1024, over pair_range[].y
-    const uint32_t per = (P + 1023) / 1024;
-    const uint32_t beg = min(P, threadIdx.x * per), end = min(P, beg + per);
-    uint32_t local = 0;
-    for (uint32_t p = beg; p < end; ++p) local += b.pair_range[p].y;
-    uint32_t incl = local;
-    const uint32_t lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = __shfl_up(incl, o, 64);
-        if (lane >= (uint32_t)o) incl += u;
-    }
-    if (lane == 63) wave_tot[threadIdx.x >> 6] = incl;
-    __syncthreads();
-    uint32_t run = incl - local;
-    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) run += wave_tot[w];
-    for (uint32_t p = beg; p < end; ++p) {
-        const uint32_t m = b.pair_range[p].y;
-        b.pair_range[p] = make_uint2(run, run + m);
-        run += m;
-    }
-}
-
 // ------------------------------------------------------------- host launch
 void launch_index_build(const Buffers& b, uint32_t cap_tiles, uint32_t cap_nodes, uint32_t cap_chunks,
                         bool use_foreground, bool use_cross, hipStream_t stream) {
@@ -728,22 +801,15 @@ void launch_index_build(const Buffers& b, uint32_t cap_tiles, uint32_t cap_nodes
     hipLaunchKernelGGL(select_pairs_kernel, dim3(2), dim3(1024), 0, stream, b, use_cross ? 1 : 0);
     const uint32_t tile = hist_tile(cap_nodes);
     hipLaunchKernelGGL(n2t_hist_kernel, dim3(cap_chunks), dim3(64), tile * sizeof(uint32_t), stream, b, tile);
-    hipLaunchKernelGGL(n2t_colscan_kernel, dim3((cap_nodes + 63) / 64), dim3(256), 0, stream, b);
+    hipLaunchKernelGGL(n2t_colscan_kernel, dim3((cap_nodes + 63) / 64), dim3(64 * kScanWaves), 0, stream, b);
     hipLaunchKernelGGL(node_scan_kernel, dim3(1), dim3(1024), 0, stream, b, 0);
     hipLaunchKernelGGL(n2t_place_kernel, dim3(cap_chunks), dim3(64), tile * sizeof(uint32_t), stream, b, tile);
-    hipLaunchKernelGGL(pair_rows_kernel<false>, dim3(cap_nodes), dim3(64), 0, stream, b);
-    hipLaunchKernelGGL(node_scan_kernel, dim3(2), dim3(1024), 0, stream, b, 1);
-    hipLaunchKernelGGL(pair_rows_kernel<true>, dim3(cap_nodes), dim3(64), 0, stream, b);
+    hipLaunchKernelGGL(pair_rows_kernel<false>, dim3(cap_nodes), dim3(64 * kRowWaves), 0, stream, b);
+    hipLaunchKernelGGL(node_scan_kernel, dim3(3), dim3(1024), 0, stream, b, 1);
+    hipLaunchKernelGGL(pair_rows_kernel<true>, dim3(cap_nodes), dim3(64 * kRowWaves), 0, stream, b);
     hipLaunchKernelGGL(pair_link_kernel, dim3((cap_nodes + 255) / 256), dim3(256), 0, stream, b);
 }
 
 uint32_t index_chunks(uint64_t max_terms) { return (uint32_t)((max_terms + kChunkTerms - 1) / kChunkTerms); }
-
-// NodePair2Term term lists into pair_terms (capacity checked by the caller).
-void launch_pair_terms(const Buffers& b, uint32_t* pair_terms, hipStream_t stream) {
-    hipLaunchKernelGGL(pair_terms_kernel, dim3(256), dim3(256), 0, stream, b, pair_terms, 0);
-    hipLaunchKernelGGL(pair_ranges_kernel, dim3(1), dim3(1024), 0, stream, b);
-    hipLaunchKernelGGL(pair_terms_kernel, dim3(256), dim3(256), 0, stream, b, pair_terms, 1);
-}
 
 }  // namespace mvdr

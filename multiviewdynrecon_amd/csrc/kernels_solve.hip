@@ -271,206 +271,280 @@ __device__ inline void spd_inverse6(const float* A, float* out, double* out64) {
         }
 }
 
-// ------------------------------------------------------ fused row assembly
-// One block per node row i walks the row's Node2Term list once and produces
-//   D_i  = sum lambda^2 J_i^T J_i + 1e-2 I
-//          (computeBlockDiagonalPreconditionerKernel, PreconditionerBlocked.cu:172-284),
-//   b_i  = -sum lambda^2 J_i^T r
-//          (computeJtResidualWithIndexKernel, JtDotResidualIndexed.cu:116-224),
-//   B_ij = sum lambda^2 J_i^T J_j for every pair (i, j), j > i, of the row
-//          (computeJtJNonDiagonalBlockKernel, JtJMaterializer.cu:270-365),
-// then the block-Jacobi inverse (blockDiagonalInverseKernel,
-// BlockDiagonalPreconditionerInverse.cu:6-41).  A term enters B_ij once per
-// Node2Term entry of row i times the number of its knn slots holding j -- the
-// multiplicity of the reference's NodePair2Term lists -- so no per-pair term
-// list is needed.  Batches of kAsmBatch entries are staged in LDS (node-frame
-// i-side rows, raw partner rows, residuals, weights, one coefficient per pair);
-// lane (u, a) then accumulates row a of block u (u = 0: the diagonal) over the
-// batch in term order.  Deterministic: a fixed summation order per output.
-constexpr int kAsmThreads = 128;
-constexpr int kAsmBatch = 64;
-constexpr int kAsmPairs = 20;   // pairs per pass: (1 + 20) * 6 = 126 lanes
-constexpr int kAsmStride = 63;  // floats per staged entry (odd: conflict-free LDS)
-constexpr int kOffJi = 0, kOffJp = 18, kOffRes = 36, kOffLam = 39, kOffRows = 42, kOffCoef = 43;
-static_assert(kOffCoef + kAsmPairs == kAsmStride, "staged entry layout");
-
-// Rows of term t seen from node i: i-side rows (slot weight of i applied), the
-// raw rows a partner node multiplies by its own weight (a graph edge: the rows
-// of the other end), residuals and lambda^2.  coef(j) for a partner j follows.
-struct RowTerm {
-    int rows;
-    float Ji[3][6], Jp[3][6], res[3], lam2[3];
-    bool smooth;
-    uint32_t partner;  // smooth: the other end (0xFFFFFFFF for a loop)
-    ushort4 knn;
-    float4 w;
-};
-
-__device__ inline void row_term(const Buffers& b, const Counts& c, const Penalty& pen, uint32_t t, uint32_t i,
-                                RowTerm& o) {
+// --------------------------------------------------------- per-term rows
+// Jacobian rows of term t with respect to node n (knn slot weight of n
+// applied), residuals and lambda^2.  Returns the number of rows (1..3).
+__device__ inline int term_rows(const Buffers& b, const Counts& c, const Penalty& pen, uint32_t t, uint32_t n,
+                                float J[3][6], float res[3], float lam2[3]) {
     uint32_t k;
     const int type = type_of(c, t, k);
-    o.smooth = false;
-    o.partner = 0xFFFFFFFFu;
     if (type == kDense || type == kForeground) {
         const ScalarRec& rec = type == kDense ? b.dense[k] : b.fg[k];
-        o.knn = rec.knn;
-        o.w = rec.w;
-        const float wi = slot_weight(rec.knn, rec.w, i);
+        const float w = slot_weight(rec.knn, rec.w, n);
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            o.Jp[0][q] = rec.J[q];
-            o.Ji[0][q] = wi * rec.J[q];
-        }
-        o.res[0] = rec.r;
-        o.lam2[0] = type == kDense ? 1.0f : pen.foreground * pen.foreground;
-        o.rows = 1;
+        for (int q = 0; q < 6; ++q) J[0][q] = w * rec.J[q];
+        res[0] = rec.r;
+        lam2[0] = type == kDense ? 1.0f : pen.foreground * pen.foreground;
         if (type == kDense && b.density && pen.density != 0.f) {
             const ScalarRec& d = b.density[k];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                o.Jp[1][q] = d.J[q];
-                o.Ji[1][q] = wi * d.J[q];
-            }
-            o.res[1] = d.r;
-            o.lam2[1] = pen.density * pen.density;
-            o.rows = 2;
+            for (int q = 0; q < 6; ++q) J[1][q] = w * d.J[q];
+            res[1] = d.r;
+            lam2[1] = pen.density * pen.density;
+            return 2;
         }
-        return;
+        return 1;
     }
     if (type == kSmooth) {
         const SmoothRec s = b.smooth[k];
-        const uint32_t first = s.ij & 0xFFFF, second = s.ij >> 16;
-        const bool is_first = i == first;
-        float r2[3];
-        smooth_rows(s, is_first, o.Ji, o.res);
-        smooth_rows(s, !is_first, o.Jp, r2);
-        o.smooth = true;
-        o.partner = first == second ? 0xFFFFFFFFu : (is_first ? second : first);
-        o.lam2[0] = o.lam2[1] = o.lam2[2] = pen.smooth * pen.smooth;
-        o.rows = 3;
-        return;
+        smooth_rows(s, n == (s.ij & 0xFFFF), J, res);
+        lam2[0] = lam2[1] = lam2[2] = pen.smooth * pen.smooth;
+        return 3;
     }
     const P2PRec& rec = type == kFeature ? b.feat[k] : b.cross[k];
-    o.knn = rec.knn;
-    o.w = rec.w;
-    p2p_rows(rec, o.Jp, o.res);
-    const float wi = slot_weight(rec.knn, rec.w, i);
+    p2p_rows(rec, J, res);
+    const float w = slot_weight(rec.knn, rec.w, n);
     const float l = type == kFeature ? pen.feature : pen.cross;
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
 #pragma unroll
-        for (int q = 0; q < 6; ++q) o.Ji[ch][q] = wi * o.Jp[ch][q];
-        o.lam2[ch] = l * l;
+        for (int q = 0; q < 6; ++q) J[ch][q] *= w;
+        lam2[ch] = l * l;
     }
-    o.rows = 3;
+    return 3;
 }
 
-// Weight of partner j in term o: (knn slots holding j) x slot weight of j.
-__device__ inline float partner_coef(const RowTerm& o, uint32_t j) {
-    if (o.smooth) return o.partner == j ? 1.0f : 0.0f;
-    const int cnt = (o.knn.x == j) + (o.knn.y == j) + (o.knn.z == j) + (o.knn.w == j);
-    return cnt ? (float)cnt * slot_weight(o.knn, o.w, j) : 0.0f;
+// ------------------------------------------------- J^T J, J^T r assembly
+// Segmented reductions over the flat index arrays, one wave per 64 entries:
+//   Node2Term entries (node-major)     -> D_n = sum lambda^2 J^T J and
+//     b_n = -sum lambda^2 J^T r (computeBlockDiagonalPreconditionerKernel,
+//     PreconditionerBlocked.cu:172-284; computeJtResidualWithIndexKernel,
+//     JtDotResidualIndexed.cu:116-224),
+//   NodePair2Term entries (pair-major) -> B_ij = sum lambda^2 J_i^T J_j
+//     (computeJtJNonDiagonalBlockKernel, JtJMaterializer.cu:270-365).
+// Every lane evaluates its entry's contribution in node-centred coordinates;
+// an inclusive segmented scan across the wave (runs of equal node / pair)
+// leaves each run's sum in its last lane.  A run that lies inside the chunk is
+// final and written in place; a run cut by a chunk boundary leaves a partial
+// (slot A: the run holding lane 0, slot B: the run holding lane 63) that
+// assemble_fixup_kernel adds up in chunk order.  Work per wave is fixed
+// whatever the row and pair lengths; summation order is fixed.
+constexpr uint32_t kDiagVals = 27;  // 21 upper-triangle entries + 6 of J^T r
+constexpr uint32_t kPairVals = 36;
+
+// Inclusive segmented scan across the wave by DPP moves (row shifts 1, 2, 4, 8
+// inside each 16-lane row, then the row_bcast15 / row_bcast31 steps across
+// rows); a lane adds what it receives only when it comes from a lane with the
+// same key.  Runs of a key are contiguous, so equal keys mean the same run.
+template <int CTRL, int ROW_MASK>
+__device__ inline uint32_t dpp_u32(uint32_t v, uint32_t old) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROW_MASK, 0xF, false);
+}
+template <int CTRL, int ROW_MASK, int V>
+__device__ inline void seg_step(float (&v)[V], uint32_t key) {
+    const bool same = dpp_u32<CTRL, ROW_MASK>(key, 0xFFFFFFFEu) == key;
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+        const float o = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v[q]), CTRL, ROW_MASK, 0xF, false));
+        v[q] += same ? o : 0.f;
+    }
+}
+template <int V>
+__device__ inline void segmented_scan(float (&v)[V], uint32_t key) {
+    seg_step<0x111, 0xF>(v, key);  // row_shr:1
+    seg_step<0x112, 0xF>(v, key);  // row_shr:2
+    seg_step<0x114, 0xF>(v, key);  // row_shr:4
+    seg_step<0x118, 0xF>(v, key);  // row_shr:8
+    seg_step<0x142, 0xA>(v, key);  // row_bcast:15 into rows 1, 3
+    seg_step<0x143, 0xC>(v, key);  // row_bcast:31 into rows 2, 3
 }
 
+__device__ inline void diag_chunk(const Buffers& b, const Counts& c, const Penalty& pen, uint32_t ch, uint32_t lane) {
+    const FrameDev& f = *b.frame;
+    const uint32_t total = c.kv1;
+    const uint32_t e = ch * 64 + lane;
+    const bool valid = e < total;
+    const uint32_t key = valid ? b.n2t_node[e] : 0xFFFFFFFFu;
+    float v[kDiagVals];
+#pragma unroll
+    for (int q = 0; q < (int)kDiagVals; ++q) v[q] = 0.f;
+    if (valid) {
+        float J[3][6], res[3], lam2[3];
+        const int rows = term_rows(b, c, pen, b.n2t_index[e], key, J, res, lam2);
+        const float4 g = f.nodes[key];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            if (r < rows) {
+                to_node_frame(J[r], g);
+                int o = 0;
+#pragma unroll
+                for (int a = 0; a < 6; ++a)
+#pragma unroll
+                    for (int k2 = a; k2 < 6; ++k2) v[o++] += lam2[r] * J[r][a] * J[r][k2];
+#pragma unroll
+                for (int a = 0; a < 6; ++a) v[21 + a] -= lam2[r] * J[r][a] * res[r];
+            }
+        }
+    }
+    segmented_scan(v, key);
+    const uint32_t key_next = (uint32_t)__shfl_down((int)key, 1, 64);
+    const uint32_t first_key = (uint32_t)__shfl((int)key, 0, 64);
+    const bool last = valid && (lane == 63 || key_next != key);
+    if (!last) return;
+    const bool has_lane0 = key == first_key;
+    const bool open_head = has_lane0 && ch > 0 && b.n2t_node[ch * 64 - 1] == key;
+    const bool open_tail = lane == 63 && e + 1 < total && b.n2t_node[e + 1] == key;
+    if (open_head || open_tail) {
+        float* out = b.part_diag + ((size_t)ch * 2 + (has_lane0 ? 0 : 1)) * kDiagVals;
+#pragma unroll
+        for (int q = 0; q < (int)kDiagVals; ++q) out[q] = v[q];
+        return;
+    }
+    float* D = b.diag + 36 * (size_t)key;
+    int o = 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+        for (int k2 = a; k2 < 6; ++k2) {
+            D[6 * a + k2] = v[o];
+            D[6 * k2 + a] = v[o];
+            ++o;
+        }
+#pragma unroll
+    for (int a = 0; a < 6; ++a) b.jtr[6 * (size_t)key + a] = v[21 + a];
+}
+
+__device__ inline void pair_chunk(const Buffers& b, const Counts& c, const Penalty& pen, uint32_t ch, uint32_t lane) {
+    const FrameDev& f = *b.frame;
+    const uint32_t total = c.kv2;
+    const uint32_t e = ch * 64 + lane;
+    const bool valid = e < total;
+    const uint32_t key = valid ? b.kv2_pair[e] : 0xFFFFFFFFu;
+    float v[kPairVals];
+#pragma unroll
+    for (int q = 0; q < (int)kPairVals; ++q) v[q] = 0.f;
+    if (valid) {
+        const uint32_t pk = b.pair_key[key];
+        const uint32_t i = pk >> 16, j = pk & 0xFFFF;
+        const uint32_t t = b.pair_terms[e];
+        float Ji[3][6], Jj[3][6], ri[3], rj[3], li[3], lj[3];
+        const int rows = term_rows(b, c, pen, t, i, Ji, ri, li);
+        term_rows(b, c, pen, t, j, Jj, rj, lj);
+        const float4 gi = f.nodes[i], gj = f.nodes[j];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            if (r < rows) {
+                to_node_frame(Ji[r], gi);
+                to_node_frame(Jj[r], gj);
+#pragma unroll
+                for (int a = 0; a < 6; ++a)
+#pragma unroll
+                    for (int k2 = 0; k2 < 6; ++k2) v[6 * a + k2] += li[r] * Ji[r][a] * Jj[r][k2];
+            }
+        }
+    }
+    segmented_scan(v, key);
+    const uint32_t key_next = (uint32_t)__shfl_down((int)key, 1, 64);
+    const uint32_t first_key = (uint32_t)__shfl((int)key, 0, 64);
+    const bool last = valid && (lane == 63 || key_next != key);
+    if (!last) return;
+    const bool has_lane0 = key == first_key;
+    const bool open_head = has_lane0 && ch > 0 && b.kv2_pair[ch * 64 - 1] == key;
+    const bool open_tail = lane == 63 && e + 1 < total && b.kv2_pair[e + 1] == key;
+    float* out = (open_head || open_tail) ? b.part_pair + ((size_t)ch * 2 + (has_lane0 ? 0 : 1)) * kPairVals
+                                          : b.offdiag + 36 * (size_t)key;
+#pragma unroll
+    for (int q = 0; q < (int)kPairVals; ++q) out[q] = v[q];
+}
+
+__global__ void __launch_bounds__(256) assemble_chunks_kernel(Buffers b, Penalty pen) {
+    const Counts& c = *b.counts;
+    if (c.error) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t c1 = (c.kv1 + 63) / 64, c2 = (c.kv2 + 63) / 64;
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < c1 + c2; w += waves) {
+        if (w < c1) diag_chunk(b, c, pen, w, lane);
+        else pair_chunk(b, c, pen, w - c1, lane);
+    }
+}
+
+// Runs cut by chunk boundaries: the run's first chunk holds it in slot B, or
+// in slot A when the run starts at that chunk's lane 0; every later chunk in
+// slot A.  Then the node rows get the damping and their block-Jacobi inverse
+// (blockDiagonalInverseKernel, BlockDiagonalPreconditionerInverse.cu:6-41).
 template <typename InvT>
-__global__ void __launch_bounds__(kAsmThreads) assemble_kernel(Buffers b, Penalty pen) {
+__global__ void __launch_bounds__(256) assemble_fixup_kernel(Buffers b) {
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
-    const uint32_t i = blockIdx.x;
-    if (i >= f.N) return;
-    __shared__ float stage[kAsmBatch * kAsmStride];
-    __shared__ float sD[36];
-    __shared__ float4 sG[kAsmPairs + 1];
-    __shared__ uint32_t sJ[kAsmPairs];
-    const uint32_t tid = threadIdx.x;
-    const float4 gi = f.nodes[i];
-    const uint32_t beg = b.n2t_offset[i], end = b.n2t_offset[i + 1];
-    const uint32_t pbeg = b.pair_offset[i], U = b.pair_offset[i + 1] - pbeg;
-    const uint32_t passes = U == 0 ? 1u : (U + kAsmPairs - 1) / kAsmPairs;
-    const uint32_t u = tid / 6, a = tid % 6;
-    for (uint32_t pass = 0; pass < passes; ++pass) {
-        const uint32_t u0 = pass * kAsmPairs;
-        const uint32_t np = min((uint32_t)kAsmPairs, U - min(U, u0));
-        if (tid < np) {
-            const uint32_t j = b.pair_key[pbeg + u0 + tid] & 0xFFFF;
-            sJ[tid] = j;
-            sG[tid + 1] = f.nodes[j];
-        }
-        if (tid == 0) sG[0] = make_float4(0.f, 0.f, 0.f, 0.f);
-        // lanes: u = 0 the diagonal (first pass only), u = 1..np the row's pairs
-        const bool active = tid < 6 * (1 + np) && (u > 0 || pass == 0);
-        float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        float accr = 0.f;
-        __syncthreads();
-        const float4 g = sG[u < np + 1 ? u : 0];
-        for (uint32_t base = beg; base < end; base += kAsmBatch) {
-            const uint32_t nb = min((uint32_t)kAsmBatch, end - base);
-            if (tid < nb) {
-                RowTerm o;
-                row_term(b, c, pen, b.n2t_index[base + tid], i, o);
-                float* E = stage + tid * kAsmStride;
+    const uint32_t N = f.N, P = c.pairs;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N + P; x += stride) {
+        if (x < N) {
+            const uint32_t n = x;
+            const uint32_t beg = b.n2t_offset[n], end = b.n2t_offset[n + 1];
+            float D[36], jtr[6];
+            if (beg == end) {
 #pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    if (r < o.rows) {
-                        to_node_frame(o.Ji[r], gi);
+                for (int q = 0; q < 36; ++q) D[q] = 0.f;
 #pragma unroll
-                        for (int q = 0; q < 6; ++q) {
-                            E[kOffJi + 6 * r + q] = o.Ji[r][q];
-                            E[kOffJp + 6 * r + q] = o.Jp[r][q];
-                        }
-                        E[kOffRes + r] = o.res[r];
-                        E[kOffLam + r] = o.lam2[r];
-                    }
+                for (int q = 0; q < 6; ++q) jtr[q] = 0.f;
+            } else if (beg / 64 == (end - 1) / 64) {
+#pragma unroll
+                for (int q = 0; q < 36; ++q) D[q] = b.diag[36 * (size_t)n + q];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) jtr[q] = b.jtr[6 * (size_t)n + q];
+            } else {
+                float v[kDiagVals];
+#pragma unroll
+                for (int q = 0; q < (int)kDiagVals; ++q) v[q] = 0.f;
+                for (uint32_t ch = beg / 64; ch <= (end - 1) / 64; ++ch) {
+                    const uint32_t slot = (ch == beg / 64 && beg % 64 != 0) ? 1u : 0u;
+                    const float* p = b.part_diag + ((size_t)ch * 2 + slot) * kDiagVals;
+#pragma unroll
+                    for (int q = 0; q < (int)kDiagVals; ++q) v[q] += p[q];
                 }
-                E[kOffRows] = (float)o.rows;
-                for (uint32_t q = 0; q < np; ++q) E[kOffCoef + q] = partner_coef(o, sJ[q]);
-            }
-            __syncthreads();
-            if (active) {
-                for (uint32_t e = 0; e < nb; ++e) {
-                    const float* E = stage + e * kAsmStride;
-                    const float coef = u == 0 ? 1.0f : E[kOffCoef + u - 1];
-                    if (coef == 0.f) continue;
-                    const int rows = (int)E[kOffRows];
-                    for (int r = 0; r < rows; ++r) {
-                        const float A = E[kOffLam + r] * E[kOffJi + 6 * r + a] * coef;
-                        const float* P = E + (u == 0 ? kOffJi : kOffJp) + 6 * r;
-                        float p[6] = {P[0], P[1], P[2], P[3], P[4], P[5]};
-                        // partner rows into the partner's node frame (identity for u = 0: g = 0)
-                        p[0] += p[4] * g.z - p[5] * g.y;
-                        p[1] += p[5] * g.x - p[3] * g.z;
-                        p[2] += p[3] * g.y - p[4] * g.x;
+                int o = 0;
 #pragma unroll
-                        for (int q = 0; q < 6; ++q) acc[q] += A * p[q];
-                        accr += A * E[kOffRes + r];
+                for (int a = 0; a < 6; ++a)
+#pragma unroll
+                    for (int k2 = a; k2 < 6; ++k2) {
+                        D[6 * a + k2] = v[o];
+                        D[6 * k2 + a] = v[o];
+                        ++o;
                     }
-                }
+#pragma unroll
+                for (int q = 0; q < 6; ++q) jtr[q] = v[21 + q];
             }
-            __syncthreads();
+            add_node_frame_damping(D, f.nodes[n]);
+#pragma unroll
+            for (int q = 0; q < 36; ++q) b.diag[36 * (size_t)n + q] = D[q];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) b.jtr[6 * (size_t)n + q] = jtr[q];
+            spd_inverse6<InvT>(D, b.diag_inv + 36 * (size_t)n,
+                               sizeof(InvT) == 8 ? b.diag_inv64 + 36 * (size_t)n : nullptr);
+        } else {
+            const uint32_t p = x - N;
+            const uint2 r = b.pair_range[p];
+            if (r.x == r.y) {
+#pragma unroll
+                for (int q = 0; q < 36; ++q) b.offdiag[36 * (size_t)p + q] = 0.f;
+                continue;
+            }
+            if (r.x / 64 == (r.y - 1) / 64) continue;  // written by its chunk
+            float v[kPairVals];
+#pragma unroll
+            for (int q = 0; q < (int)kPairVals; ++q) v[q] = 0.f;
+            for (uint32_t ch = r.x / 64; ch <= (r.y - 1) / 64; ++ch) {
+                const uint32_t slot = (ch == r.x / 64 && r.x % 64 != 0) ? 1u : 0u;
+                const float* pp = b.part_pair + ((size_t)ch * 2 + slot) * kPairVals;
+#pragma unroll
+                for (int q = 0; q < (int)kPairVals; ++q) v[q] += pp[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 36; ++q) b.offdiag[36 * (size_t)p + q] = v[q];
         }
-        if (active && u > 0) {
-            float* out = b.offdiag + 36 * (size_t)(pbeg + u0 + u - 1) + 6 * a;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) out[q] = acc[q];
-        }
-        if (active && u == 0) {
-#pragma unroll
-            for (int q = 0; q < 6; ++q) sD[6 * a + q] = acc[q];
-            b.jtr[6 * (size_t)i + a] = -accr;
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        float D[36];
-#pragma unroll
-        for (int q = 0; q < 36; ++q) D[q] = sD[q];
-        add_node_frame_damping(D, gi);
-        float* Dout = b.diag + 36 * (size_t)i;
-#pragma unroll
-        for (int q = 0; q < 36; ++q) Dout[q] = D[q];
-        spd_inverse6<InvT>(D, b.diag_inv + 36 * (size_t)i, sizeof(InvT) == 8 ? b.diag_inv64 + 36 * (size_t)i : nullptr);
     }
 }
 
@@ -522,10 +596,12 @@ void launch_eval_terms(const Buffers& b, const DQ* se3, bool fp64_geometry, hipS
 }
 
 void launch_assemble(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream) {
+    hipLaunchKernelGGL(assemble_chunks_kernel, dim3(2048), dim3(256), 0, stream, b, pen);
+    const uint32_t fix_blocks = std::min<uint32_t>((cap_nodes + b.cap_pairs + 255) / 256, 1024u);
     if (fp64_inverse)
-        hipLaunchKernelGGL(assemble_kernel<double>, dim3(cap_nodes), dim3(kAsmThreads), 0, stream, b, pen);
+        hipLaunchKernelGGL(assemble_fixup_kernel<double>, dim3(fix_blocks), dim3(256), 0, stream, b);
     else
-        hipLaunchKernelGGL(assemble_kernel<float>, dim3(cap_nodes), dim3(kAsmThreads), 0, stream, b, pen);
+        hipLaunchKernelGGL(assemble_fixup_kernel<float>, dim3(fix_blocks), dim3(256), 0, stream, b);
 }
 
 // Returns the number of partials written to slot kSlotCost.

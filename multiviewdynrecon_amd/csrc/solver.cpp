@@ -15,7 +15,6 @@ namespace mvdr {
 void launch_index_build(const Buffers& b, uint32_t cap_tiles, uint32_t cap_nodes, uint32_t cap_chunks,
                         bool use_foreground, bool use_cross, hipStream_t stream);
 uint32_t index_chunks(uint64_t max_terms);
-void launch_pair_terms(const Buffers& b, uint32_t* pair_terms, hipStream_t stream);
 void launch_eval_terms(const Buffers& b, const DQ* se3, bool fp64_geometry, hipStream_t stream);
 void launch_assemble(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream);
 void launch_spmv_once(const Buffers& b, bool fp64, uint32_t cap_nodes, hipStream_t stream);
@@ -43,15 +42,12 @@ struct mvdr_solver {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t graph_stream = nullptr;
-    uint32_t* pair_terms = nullptr;     // NodePair2Term lists, built on request (taps only)
-    size_t pair_terms_cap = 0;
     std::string error;
 
     ~mvdr_solver() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (done) (void)hipEventDestroy(done);
-        if (pair_terms) (void)hipFree(pair_terms);
         for (void* p : allocations) (void)hipFree(p);
     }
 };
@@ -304,11 +300,18 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     A(b.n2t_count, N1);
     A(b.n2t_offset, N1);
     A(b.n2t_index, b.cap_kv1);
+    A(b.n2t_node, b.cap_kv1);
     A(b.row_up, N1);
     A(b.row_low, N1);
+    A(b.row_kv2, N1);
     A(b.pair_offset, N1);
+    A(b.kv2_offset, N1);
     A(b.pair_key, b.cap_pairs);
     A(b.pair_range, b.cap_pairs);
+    A(b.pair_terms, b.cap_kv2);
+    A(b.kv2_pair, b.cap_kv2);
+    A(b.part_diag, ((size_t)b.cap_kv1 / 64 + 1) * 2 * 27);
+    A(b.part_pair, ((size_t)b.cap_kv2 / 64 + 1) * 2 * 36);
     A(b.low_offset, N1);
     A(b.low_ent, b.cap_pairs);
     A(b.diag, 36 * (size_t)c.max_nodes);
@@ -542,29 +545,8 @@ mvdr_status mvdr_solver_debug_get(mvdr_solver* s, int tap, void* dst, size_t byt
         case MVDR_TAP_NODE2TERM_OFFSET: src = b.n2t_offset; size = 4ull * (N + 1); break;
         case MVDR_TAP_NODE2TERM_INDEX: src = b.n2t_index; size = 4ull * c.kv1; break;
         case MVDR_TAP_NODEPAIR_KEY: src = b.pair_key; size = 4ull * c.pairs; break;
-        case MVDR_TAP_NODEPAIR_RANGE:
-        case MVDR_TAP_NODEPAIR_TERM_INDEX: {
-            // the solve does not use per-pair term lists: build them now
-            if (c.error) return fail(s, MVDR_E_CAPACITY, error_text(c.error));
-            if (s->pair_terms_cap < c.kv2) {
-                if (s->pair_terms) HIP_TRY(s, hipFree(s->pair_terms));
-                s->pair_terms = nullptr;
-                s->pair_terms_cap = 0;
-                HIP_TRY(s, hipMalloc(&s->pair_terms, 4ull * std::max<uint32_t>(c.kv2, 1)));
-                s->pair_terms_cap = std::max<uint32_t>(c.kv2, 1);
-            }
-            launch_pair_terms(b, s->pair_terms, nullptr);
-            HIP_TRY(s, hipStreamSynchronize(nullptr));
-            HIP_TRY(s, hipGetLastError());
-            if (tap == MVDR_TAP_NODEPAIR_RANGE) {
-                src = b.pair_range;
-                size = 8ull * c.pairs;
-            } else {
-                src = s->pair_terms;
-                size = 4ull * c.kv2;
-            }
-            break;
-        }
+        case MVDR_TAP_NODEPAIR_RANGE: src = b.pair_range; size = 8ull * c.pairs; break;
+        case MVDR_TAP_NODEPAIR_TERM_INDEX: src = b.pair_terms; size = 4ull * c.kv2; break;
         case MVDR_TAP_DENSE_TERMS: src = b.dense; size = sizeof(ScalarRec) * (size_t)c.dense; break;
         case MVDR_TAP_SMOOTH_TERMS: src = b.smooth; size = sizeof(SmoothRec) * (size_t)c.smooth; break;
         case MVDR_TAP_FEATURE_TERMS: src = b.feat; size = sizeof(P2PRec) * (size_t)c.feat; break;
