@@ -71,10 +71,13 @@ def algorithmic_bytes(sz, mixed):
         "assemble": r_terms + 4 * kv1 + 8 * (N + 1) + 4 * P + 16 * N + 144 * N + inv + 24 * N + 144 * P,
         # blocks + block-row structure + the direction vector in, A d out
         "spmv": 144 * (N + P) + 4 * (N + P) + 8 * (N + 1) + 8 * P + 24 * N + 24 * N,
+        # fused CG iteration: off-diagonal blocks of L^-1 A L^-T and the block-row
+        # structure; r, w, s, p, x read once, r, w, s, p, x written once
+        "cg_iteration": 144 * P + 12 * P + 8 * (N + 1) + 5 * 24 * N + 5 * 24 * N,
     }
 
 
-STAGE_IDS = {"eval_terms": 0, "assemble": 1, "pcg_update": 2, "spmv": 3, "index": 4}
+STAGE_IDS = {"eval_terms": 0, "assemble": 1, "pcg_update": 2, "spmv": 3, "index": 4, "cg_iteration": 5}
 
 
 def stage_time_ms(solver, stage, reps):
@@ -226,26 +229,31 @@ def main():
           file=sys.stderr, flush=True)
 
     # per-stage kernel timing on the solver's stream (hipEvents around repeated launches)
+    mixed = args.precision == "mixed"
+    # stages of this precision's solve and their launches per solve
+    if mixed:
+        per_solve = {"eval_terms": cfg.gn_iterations, "assemble": cfg.gn_iterations,
+                     "cg_iteration": cfg.gn_iterations * cfg.pcg_iterations, "index": 1}
+    else:
+        per_solve = {"eval_terms": cfg.gn_iterations, "assemble": cfg.gn_iterations,
+                     "pcg_update": cfg.gn_iterations * cfg.pcg_iterations,
+                     "spmv": cfg.gn_iterations * cfg.pcg_iterations, "index": 1}
     stages = {}
-    for st in STAGE_IDS:
+    for st in per_solve:
         stages[st] = stage_time_ms(solver, st, args.stage_reps)
     solver.SolveNonRigidAlignment()  # restore a consistent state after the stage loops
     torch.cuda.synchronize()
-    ab = algorithmic_bytes(sizes, args.precision == "mixed")
-    per_solve = {  # launches of each stage in one solve
-        "eval_terms": cfg.gn_iterations, "assemble": cfg.gn_iterations,
-        "pcg_update": cfg.gn_iterations * cfg.pcg_iterations,
-        "spmv": cfg.gn_iterations * (cfg.pcg_iterations + 2 * cfg.refine_steps), "index": 1,
-    }
+    ab = algorithmic_bytes(sizes, mixed)
     share = {k: stages[k] * per_solve[k] for k in stages}
     print("[bench] stage ms/launch " + json.dumps({k: round(v, 4) for k, v in stages.items()}), file=sys.stderr)
     print("[bench] stage ms/solve  " + json.dumps({k: round(v, 4) for k, v in share.items()}), file=sys.stderr)
-    dominant = max(("assemble", "spmv"), key=lambda k: share[k])
+    apply_stage = "cg_iteration" if mixed else "spmv"
+    dominant = max(("assemble", apply_stage), key=lambda k: share[k])
     achieved = ab[dominant] / (stages[dominant] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "bytes_per_launch": int(ab[dominant]), "launch_ms": round(stages[dominant], 5)}
-    spmv_gbs = ab["spmv"] / (stages["spmv"] * 1e-3) / 1e9
+    spmv_gbs = ab[apply_stage] / (stages[apply_stage] * 1e-3) / 1e9
     jtj_gbs = ab["assemble"] / (stages["assemble"] * 1e-3) / 1e9
 
     traffic = measured_traffic(dominant, args.workload)

@@ -56,7 +56,9 @@ typedef enum {
 typedef enum {
     MVDR_PRECISION_FP32 = 0,   /* everything FP32 (reference precision)          */
     MVDR_PRECISION_MIXED = 1   /* FP64 term geometry (J, r stored FP32), FP64
-                                  dots / alpha / beta / 6x6 inverses / SpMV sums */
+                                  dots / alpha / beta / 6x6 factors / SpMV sums;
+                                  block-Jacobi PCG run as one-pass CG on
+                                  L^-1 A L^-T (same iterates, one launch each) */
 } mvdr_precision;
 
 typedef enum {
@@ -213,7 +215,9 @@ typedef enum {
                                       block inverse (one fused pass per node row)   */
     MVDR_STAGE_PCG_UPDATE = 2,     /* one PCG vector update (x, r, M^-1 r, dot)       */
     MVDR_STAGE_SPMV = 3,           /* one block-sparse ApplyJtJ (q = A d) + dot(d, q) */
-    MVDR_STAGE_INDEX = 4           /* the whole index build                           */
+    MVDR_STAGE_INDEX = 4,          /* the whole index build                           */
+    MVDR_STAGE_CG_ITERATION = 5    /* one fused CG iteration of the mixed mode:
+                                      L^-1 A L^-T apply, vector updates, both dots  */
 } mvdr_stage;
 
 /* Kernel micro-benchmark (the reference keeps a commented 1000x loop around

@@ -338,6 +338,11 @@ struct Buffers {
     float* offdiag;          // 36 P
     // PCG
     float *x, *r, *s, *q, *d0, *d1;
+    // fused CG on A^ = L^-1 A L^-T (mixed precision mode)
+    double* linv64;          // 36 N: L^-1 of each diagonal block, row-major lower
+    float* offhat;           // 36 P: L_i^-1 B_ij L_j^-T
+    float *cg_r[2], *cg_w[2], *cg_s[2], *cg_p, *cg_x, *cg_b;
+    double* cg_scal;         // alpha_i, gamma_i per iteration
     double* partial;         // kPartialSlots x kPartialStride per-block partial sums
     // warp field double buffer
     DQ* se3[2];

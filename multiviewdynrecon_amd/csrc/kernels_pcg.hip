@@ -311,6 +311,286 @@ __global__ void __launch_bounds__(256) refine_step_kernel(Buffers b, uint32_t sp
         b.x[i] = (float)((Acc)b.x[i] + a * (Acc)b.d1[i]);
 }
 
+// ------------------------------------------- fused CG (mixed precision mode)
+// Block-Jacobi PCG on A x = b is CG on A^ x^ = b^ with A^ = L^-1 A L^-T,
+// b^ = L^-1 b, x = L^-T x^, where M = L L^T holds the Cholesky factors of the
+// diagonal blocks: the same iterates in exact arithmetic, and A^ has identity
+// diagonal blocks.  The Chronopoulos-Gear form of CG forms both inner
+// products of an iteration in one pass:
+//   x_{i+1} = x_i + alpha_i p_i,   r_{i+1} = r_i - alpha_i s_i,
+//   w_{i+1} = A^ r_{i+1},          gamma_{i+1} = (r, r), delta_{i+1} = (w, r),
+//   beta_{i+1} = gamma_{i+1} / gamma_i,
+//   alpha_{i+1} = gamma_{i+1} / (delta_{i+1} - beta_{i+1} gamma_{i+1} / alpha_i),
+//   p_{i+1} = r_{i+1} + beta_{i+1} p_i,   s_{i+1} = w_{i+1} + beta_{i+1} s_i,
+// so one launch per iteration suffices: launch i reduces gamma_i, delta_i from
+// the previous launch's per-block partials (in a fixed order, in every block),
+// forms s_i = w_i + beta_i s_{i-1} and r_{i+1} = r_i - alpha_i s_i for its rows
+// and, redundantly, for each column its rows touch, and applies A^.  Vectors
+// are FP32 in HBM; scalars, sums and the factors L^-1 are FP64.
+constexpr uint32_t kSlotCgGamma = 0;   // gamma_i partials: slot i
+constexpr uint32_t kSlotCgDelta = 72;  // delta_i partials: slot 72 + i
+
+// A^ blocks of the pairs and b^ of the nodes.
+__global__ void __launch_bounds__(256) cg_transform_kernel(Buffers b) {
+    const FrameDev& f = *b.frame;
+    const Counts& c = *b.counts;
+    if (c.error) return;
+    const uint32_t N = f.N, P = c.pairs;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < P + N; x += stride) {
+        if (x < P) {
+            const uint32_t key = b.pair_key[x];
+            const double* Li = b.linv64 + 36 * (size_t)(key >> 16);
+            const double* Lj = b.linv64 + 36 * (size_t)(key & 0xFFFF);
+            const float* B = b.offdiag + 36 * (size_t)x;
+            double T[36];
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+#pragma unroll
+                for (int cc = 0; cc < 6; ++cc) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int k = 0; k <= a; ++k) t += Li[6 * a + k] * (double)B[6 * k + cc];
+                    T[6 * a + cc] = t;
+                }
+            float* out = b.offhat + 36 * (size_t)x;
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+#pragma unroll
+                for (int bb = 0; bb < 6; ++bb) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int cc = 0; cc <= bb; ++cc) t += T[6 * a + cc] * Lj[6 * bb + cc];
+                    out[6 * a + bb] = (float)t;
+                }
+        } else {
+            const uint32_t n = x - P;
+            const double* L = b.linv64 + 36 * (size_t)n;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) {
+                double t = 0.0;
+#pragma unroll
+                for (int k = 0; k <= a; ++k) t += L[6 * a + k] * (double)b.jtr[6 * (size_t)n + k];
+                b.cg_b[6 * (size_t)n + a] = (float)t;
+                b.cg_r[0][6 * (size_t)n + a] = (float)t;
+            }
+        }
+    }
+}
+
+// y = sum over the off-diagonal blocks of row i of A^ v_j, with v_j = V(j)
+// supplied by the caller; split over the row's kRowLanes lanes and reduced.
+template <typename Load>
+__device__ inline void hat_row_apply(const Buffers& b, uint32_t i, uint32_t sub, const Load& load, double y[6]) {
+#pragma unroll
+    for (int a = 0; a < 6; ++a) y[a] = 0.0;
+    const uint32_t lo_beg = b.low_offset[i], lo_n = b.low_offset[i + 1] - lo_beg;
+    const uint32_t up_beg = b.pair_offset[i], up_n = b.pair_offset[i + 1] - up_beg;
+    for (uint32_t e = sub; e < lo_n + up_n; e += kRowLanes) {
+        const float* B;
+        bool tr;
+        uint32_t j;
+        if (e < lo_n) {
+            const uint2 le = b.low_ent[lo_beg + e];
+            B = b.offhat + 36 * (size_t)le.y;
+            tr = true;
+            j = le.x;
+        } else {
+            const uint32_t p = up_beg + (e - lo_n);
+            B = b.offhat + 36 * (size_t)p;
+            tr = false;
+            j = b.pair_key[p] & 0xFFFF;
+        }
+        float v[6];
+        load(j, v);
+        block_mv<double>(B, tr, v, y);
+    }
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+        for (int o = kRowLanes / 2; o > 0; o >>= 1) y[a] += __shfl_xor(y[a], o, 64);
+}
+
+// w_0 = A^ r_0 (r_0 = b^), x^ = p_{-1} = s_{-1} = 0, partials of gamma_0, delta_0.
+__global__ void __launch_bounds__(256) cg_init_kernel(Buffers b) {
+    const FrameDev& f = *b.frame;
+    const uint32_t sub = threadIdx.x % kRowLanes;
+    double g = 0.0, d = 0.0;
+    if (!b.counts->error) {
+        const float* R = b.cg_r[0];
+        auto load = [&](uint32_t j, float v[6]) {
+#pragma unroll
+            for (int a = 0; a < 6; ++a) v[a] = R[6 * (size_t)j + a];
+        };
+        const uint32_t groups = gridDim.x * kRowsPerBlock;
+        for (uint32_t base = blockIdx.x * kRowsPerBlock; base < f.N; base += groups) {
+            const uint32_t i = base + threadIdx.x / kRowLanes;
+            double y[6];
+            hat_row_apply(b, i < f.N ? i : 0, sub, load, y);
+            if (i < f.N && sub == 0) {
+#pragma unroll
+                for (int a = 0; a < 6; ++a) {
+                    const size_t k = 6 * (size_t)i + a;
+                    const double r = R[k];
+                    const double w = (double)(float)(y[a] + r);
+                    b.cg_w[0][k] = (float)w;
+                    b.cg_s[1][k] = 0.f;
+                    b.cg_p[k] = 0.f;
+                    b.cg_x[k] = 0.f;
+                    g += r * r;
+                    d += w * r;
+                }
+            }
+        }
+    }
+    write_partial<double>(b, kSlotCgGamma + 0, g);
+    write_partial<double>(b, kSlotCgDelta + 0, d);
+}
+
+// alpha_i, beta_i from the partials of gamma_i, delta_i and the stored
+// alpha_{i-1}, gamma_{i-1}; block 0 stores alpha_i, gamma_i for launch i + 1.
+__device__ inline void cg_scalars(const Buffers& b, uint32_t i, uint32_t blocks, double& alpha, double& beta) {
+    const double gamma = reduce_partials<double>(slot_ptr(b, kSlotCgGamma + i), blocks);
+    const double delta = reduce_partials<double>(slot_ptr(b, kSlotCgDelta + i), blocks);
+    beta = 0.0;
+    double den = delta;
+    if (i > 0) {
+        const double alpha_prev = b.cg_scal[2 * (i - 1)], gamma_prev = b.cg_scal[2 * (i - 1) + 1];
+        beta = gamma_prev != 0.0 ? gamma / gamma_prev : 0.0;
+        den = alpha_prev != 0.0 ? delta - beta * gamma / alpha_prev : 0.0;
+    }
+    alpha = den != 0.0 ? gamma / den : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        b.cg_scal[2 * i] = alpha;
+        b.cg_scal[2 * i + 1] = gamma;
+    }
+}
+
+// Launch i of the fused CG (see above).  Reads r_i, w_i, s_{i-1}; writes
+// r_{i+1}, w_{i+1}, s_i, p_i, x^_{i+1}.
+__global__ void __launch_bounds__(256) cg_iter_kernel(Buffers b, uint32_t i) {
+    const FrameDev& f = *b.frame;
+    const uint32_t sub = threadIdx.x % kRowLanes;
+    double alpha, beta;
+    cg_scalars(b, i, gridDim.x, alpha, beta);
+    double g = 0.0, d = 0.0;
+    if (!b.counts->error) {
+        const float* R = b.cg_r[i & 1];
+        const float* W = b.cg_w[i & 1];
+        const float* S = b.cg_s[(i + 1) & 1];
+        float* Ro = b.cg_r[(i + 1) & 1];
+        float* Wo = b.cg_w[(i + 1) & 1];
+        float* So = b.cg_s[i & 1];
+        // r_{i+1} of node j, rounded as it is stored
+        auto next_r = [&](uint32_t j, float v[6]) {
+#pragma unroll
+            for (int a = 0; a < 6; ++a) {
+                const size_t k = 6 * (size_t)j + a;
+                const float s_i = (float)((double)W[k] + beta * (double)S[k]);
+                v[a] = (float)((double)R[k] - alpha * (double)s_i);
+            }
+        };
+        const uint32_t groups = gridDim.x * kRowsPerBlock;
+        for (uint32_t base = blockIdx.x * kRowsPerBlock; base < f.N; base += groups) {
+            const uint32_t n = base + threadIdx.x / kRowLanes;
+            double y[6];
+            hat_row_apply(b, n < f.N ? n : 0, sub, next_r, y);
+            if (n < f.N && sub == 0) {
+#pragma unroll
+                for (int a = 0; a < 6; ++a) {
+                    const size_t k = 6 * (size_t)n + a;
+                    const float s_i = (float)((double)W[k] + beta * (double)S[k]);
+                    const float r_next = (float)((double)R[k] - alpha * (double)s_i);
+                    const float p_i = (float)((double)R[k] + beta * (double)b.cg_p[k]);
+                    const double w_next = (double)(float)(y[a] + (double)r_next);
+                    So[k] = s_i;
+                    Ro[k] = r_next;
+                    Wo[k] = (float)w_next;
+                    b.cg_p[k] = p_i;
+                    b.cg_x[k] = (float)((double)b.cg_x[k] + alpha * (double)p_i);
+                    g += (double)r_next * (double)r_next;
+                    d += w_next * (double)r_next;
+                }
+            }
+        }
+    }
+    write_partial<double>(b, kSlotCgGamma + i + 1, g);
+    write_partial<double>(b, kSlotCgDelta + i + 1, d);
+}
+
+// Residual refinement on the CG solution (preconditioned steepest descent on
+// the true residual, as refine_* above, in the A^ coordinates):
+// r^ = b^ - A^ x^, num = (r^, r^) ...
+__global__ void __launch_bounds__(256) cg_refine_residual_kernel(Buffers b) {
+    const FrameDev& f = *b.frame;
+    const uint32_t sub = threadIdx.x % kRowLanes;
+    double acc = 0.0;
+    if (!b.counts->error) {
+        const float* X = b.cg_x;
+        auto load = [&](uint32_t j, float v[6]) {
+#pragma unroll
+            for (int a = 0; a < 6; ++a) v[a] = X[6 * (size_t)j + a];
+        };
+        const uint32_t groups = gridDim.x * kRowsPerBlock;
+        for (uint32_t base = blockIdx.x * kRowsPerBlock; base < f.N; base += groups) {
+            const uint32_t n = base + threadIdx.x / kRowLanes;
+            double y[6];
+            hat_row_apply(b, n < f.N ? n : 0, sub, load, y);
+            if (n < f.N && sub == 0) {
+#pragma unroll
+                for (int a = 0; a < 6; ++a) {
+                    const size_t k = 6 * (size_t)n + a;
+                    const float rr = (float)((double)b.cg_b[k] - (y[a] + (double)X[k]));
+                    b.q[k] = rr;
+                    acc += (double)rr * (double)rr;
+                }
+            }
+        }
+    }
+    write_partial<double>(b, kSlotRefNum, acc);
+}
+
+// ... den = (r^, A^ r^) ...
+__global__ void __launch_bounds__(256) cg_refine_curvature_kernel(Buffers b) {
+    const FrameDev& f = *b.frame;
+    const uint32_t sub = threadIdx.x % kRowLanes;
+    double acc = 0.0;
+    if (!b.counts->error) {
+        const float* Q = b.q;
+        auto load = [&](uint32_t j, float v[6]) {
+#pragma unroll
+            for (int a = 0; a < 6; ++a) v[a] = Q[6 * (size_t)j + a];
+        };
+        const uint32_t groups = gridDim.x * kRowsPerBlock;
+        for (uint32_t base = blockIdx.x * kRowsPerBlock; base < f.N; base += groups) {
+            const uint32_t n = base + threadIdx.x / kRowLanes;
+            double y[6];
+            hat_row_apply(b, n < f.N ? n : 0, sub, load, y);
+            if (n < f.N && sub == 0) {
+#pragma unroll
+                for (int a = 0; a < 6; ++a) {
+                    const double q = Q[6 * (size_t)n + a];
+                    acc += q * (y[a] + q);
+                }
+            }
+        }
+    }
+    write_partial<double>(b, kSlotRefDen, acc);
+}
+
+// ... x^ += (num / den) r^ (all refinement steps but the last; the last is
+// folded into the warp update).
+__global__ void __launch_bounds__(256) cg_refine_step_kernel(Buffers b, uint32_t spmv_blocks) {
+    const FrameDev& f = *b.frame;
+    const double num = reduce_partials<double>(slot_ptr(b, kSlotRefNum), spmv_blocks);
+    const double den = reduce_partials<double>(slot_ptr(b, kSlotRefDen), spmv_blocks);
+    if (b.counts->error) return;
+    const double a = den > 0.0 ? num / den : 0.0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < 6 * f.N; k += stride)
+        b.cg_x[k] = (float)((double)b.cg_x[k] + a * (double)b.q[k]);
+}
+
 // ---------------------------------------------------------- warp update
 // DualQuaternion(const mat34&) (DualQuaternion.h:123-130) with Quaternion(mat33)
 // (Quaternion.h:46-77).
@@ -360,18 +640,44 @@ __device__ inline Mat34 mat34_of(DQ d) {
 
 // applyWarpFieldUpdateKernel (SolverIterationData.cu:9-57): Rodrigues twist
 // (MatUtils.cpp:10-40) left-composed with the node's SE3.
-__global__ void __launch_bounds__(256) apply_update_kernel(Buffers b, const DQ* __restrict__ se3_in, DQ* __restrict__ se3_out) {
+// kHat: the solution is x = L^-T (x^ + a r^) of the fused CG, a the last
+// refinement step (0 without refinement).
+template <bool kHat>
+__global__ void __launch_bounds__(256) apply_update_kernel(Buffers b, const DQ* __restrict__ se3_in, DQ* __restrict__ se3_out,
+                                                           uint32_t refine_blocks) {
     const FrameDev& f = *b.frame;
     const uint32_t stride = gridDim.x * blockDim.x;
+    double step = 0.0;
+    if (kHat && refine_blocks) {
+        const double num = reduce_partials<double>(slot_ptr(b, kSlotRefNum), refine_blocks);
+        const double den = reduce_partials<double>(slot_ptr(b, kSlotRefDen), refine_blocks);
+        step = den > 0.0 ? num / den : 0.0;
+    }
     const bool bad = b.counts->error != 0;
     for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < f.N; n += stride) {
         if (bad) {
             se3_out[n] = se3_in[n];
             continue;
         }
+        float* xi = b.x + 6 * (size_t)n;
+        if (kHat) {
+            double xh[6];
+#pragma unroll
+            for (int a = 0; a < 6; ++a) {
+                const size_t k = 6 * (size_t)n + a;
+                xh[a] = (double)b.cg_x[k] + (refine_blocks ? step * (double)b.q[k] : 0.0);
+            }
+            const double* L = b.linv64 + 36 * (size_t)n;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) {
+                double t = 0.0;
+#pragma unroll
+                for (int k = a; k < 6; ++k) t += L[6 * k + a] * xh[k];
+                xi[a] = (float)t;
+            }
+        }
         // node-centred solution (w, u) -> reference twist (w, v = u + g x w),
         // written back so that the twist buffer holds the reference quantity
-        float* xi = b.x + 6 * (size_t)n;
         const float4 g4 = f.nodes[n];
         const f3 w = mk(xi[0], xi[1], xi[2]);
         const f3 v = mk(xi[3], xi[4], xi[5]) + cross3(mk(g4.x, g4.y, g4.z), w);
@@ -449,7 +755,30 @@ void launch_pcg_update_once(const Buffers& b, bool fp64, hipStream_t stream) {
 }
 
 void launch_apply_update(const Buffers& b, const DQ* in, DQ* out, hipStream_t stream) {
-    hipLaunchKernelGGL(apply_update_kernel, dim3(64), dim3(256), 0, stream, b, in, out);
+    hipLaunchKernelGGL(apply_update_kernel<false>, dim3(64), dim3(256), 0, stream, b, in, out, 0u);
+}
+
+// Fused CG of the mixed precision mode: transform, init, one launch per
+// iteration, refinement, warp update.
+void launch_cg_solve(const Buffers& b, uint32_t iters, uint32_t refine, uint32_t cap_nodes, const DQ* in, DQ* out,
+                     hipStream_t stream) {
+    const uint32_t sg = spmv_grid(cap_nodes);
+    iters = std::min(iters, kMaxIters);
+    const uint32_t tgrid = std::min<uint32_t>((b.cap_pairs + cap_nodes + 255) / 256, 2048u);
+    hipLaunchKernelGGL(cg_transform_kernel, dim3(tgrid), dim3(256), 0, stream, b);
+    hipLaunchKernelGGL(cg_init_kernel, dim3(sg), dim3(256), 0, stream, b);
+    for (uint32_t i = 0; i < iters; ++i) hipLaunchKernelGGL(cg_iter_kernel, dim3(sg), dim3(256), 0, stream, b, i);
+    for (uint32_t r = 0; r < refine; ++r) {
+        hipLaunchKernelGGL(cg_refine_residual_kernel, dim3(sg), dim3(256), 0, stream, b);
+        hipLaunchKernelGGL(cg_refine_curvature_kernel, dim3(sg), dim3(256), 0, stream, b);
+        if (r + 1 < refine) hipLaunchKernelGGL(cg_refine_step_kernel, dim3(kVecBlocks), dim3(256), 0, stream, b, sg);
+    }
+    hipLaunchKernelGGL(apply_update_kernel<true>, dim3(64), dim3(256), 0, stream, b, in, out, refine ? sg : 0u);
+}
+
+// One fused CG iteration (stage timing).
+void launch_cg_iter_once(const Buffers& b, uint32_t cap_nodes, hipStream_t stream) {
+    hipLaunchKernelGGL(cg_iter_kernel, dim3(spmv_grid(cap_nodes)), dim3(256), 0, stream, b, 1u);
 }
 
 void launch_copy_se3(const Buffers& b, DQ* dst, hipStream_t stream) {

@@ -241,7 +241,7 @@ __device__ inline void add_node_frame_damping(float D[36], const float4 g) {
 // inv(A) = L^-T L^-1 with A = L L^T; A is symmetric positive definite (the
 // reference adds 1e-2 to every diagonal, PreconditionerBlocked.cu:272-275).
 template <typename T>
-__device__ inline void spd_inverse6(const float* A, float* out, double* out64) {
+__device__ inline void spd_inverse6(const float* A, float* out, double* out64, double* linv64 = nullptr) {
     T L[21];  // packed lower triangle, row-major
 #define LI(r, c) L[(r) * ((r) + 1) / 2 + (c)]
     for (int r = 0; r < 6; ++r) {
@@ -261,6 +261,8 @@ __device__ inline void spd_inverse6(const float* A, float* out, double* out64) {
             X[6 * r + c2] = s / LI(r, r);
         }
 #undef LI
+    if (linv64)
+        for (int q = 0; q < 36; ++q) linv64[q] = (double)X[q];
     // inv = X^T X
     for (int r = 0; r < 6; ++r)
         for (int col = 0; col < 6; ++col) {
@@ -523,7 +525,8 @@ __global__ void __launch_bounds__(256) assemble_fixup_kernel(Buffers b) {
 #pragma unroll
             for (int q = 0; q < 6; ++q) b.jtr[6 * (size_t)n + q] = jtr[q];
             spd_inverse6<InvT>(D, b.diag_inv + 36 * (size_t)n,
-                               sizeof(InvT) == 8 ? b.diag_inv64 + 36 * (size_t)n : nullptr);
+                               sizeof(InvT) == 8 ? b.diag_inv64 + 36 * (size_t)n : nullptr,
+                               sizeof(InvT) == 8 ? b.linv64 + 36 * (size_t)n : nullptr);
         } else {
             const uint32_t p = x - N;
             const uint2 r = b.pair_range[p];

@@ -22,6 +22,9 @@ void launch_pcg_update_once(const Buffers& b, bool fp64, hipStream_t stream);
 uint32_t launch_cost(const Buffers& b, const Penalty& pen, bool density_in_jtj, hipStream_t stream);
 void launch_pcg(const Buffers& b, uint32_t iters, uint32_t refine, bool fp64, uint32_t cap_nodes, hipStream_t stream);
 void launch_apply_update(const Buffers& b, const DQ* in, DQ* out, hipStream_t stream);
+void launch_cg_solve(const Buffers& b, uint32_t iters, uint32_t refine, uint32_t cap_nodes, const DQ* in, DQ* out,
+                     hipStream_t stream);
+void launch_cg_iter_once(const Buffers& b, uint32_t cap_nodes, hipStream_t stream);
 void launch_copy_se3(const Buffers& b, DQ* dst, hipStream_t stream);
 }  // namespace mvdr
 
@@ -121,8 +124,12 @@ void enqueue_solve(mvdr_solver* s, hipStream_t stream, bool index, uint32_t firs
         DQ* next = s->b.se3[(k + 1) & 1];
         launch_eval_terms(s->b, cur, fp64, stream);
         launch_assemble(s->b, pen, fp64, c.max_nodes, stream);
-        launch_pcg(s->b, c.pcg_iterations, fp64 ? c.refine_steps : 0, fp64, c.max_nodes, stream);
-        launch_apply_update(s->b, cur, next, stream);
+        if (fp64) {
+            launch_cg_solve(s->b, c.pcg_iterations, c.refine_steps, c.max_nodes, cur, next, stream);
+        } else {
+            launch_pcg(s->b, c.pcg_iterations, 0, false, c.max_nodes, stream);
+            launch_apply_update(s->b, cur, next, stream);
+        }
     }
 }
 
@@ -321,6 +328,17 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     A(b.jtr, 6 * (size_t)c.max_nodes);
     A(b.offdiag, 36 * (size_t)b.cap_pairs);
     for (float** v : {&b.x, &b.r, &b.s, &b.q, &b.d0, &b.d1}) A(*v, 6 * (size_t)c.max_nodes);
+    b.linv64 = nullptr;
+    b.offhat = nullptr;
+    b.cg_scal = nullptr;
+    if (c.precision == MVDR_PRECISION_MIXED) {
+        A(b.linv64, 36 * (size_t)c.max_nodes);
+        A(b.offhat, 36 * (size_t)b.cap_pairs);
+        for (float** v : {&b.cg_r[0], &b.cg_r[1], &b.cg_w[0], &b.cg_w[1], &b.cg_s[0], &b.cg_s[1], &b.cg_p, &b.cg_x,
+                          &b.cg_b})
+            A(*v, 6 * (size_t)c.max_nodes);
+        A(b.cg_scal, 2 * 72);
+    }
     A(b.partial, (size_t)kPartialSlots * kPartialStride);
     A(b.se3[0], c.max_nodes);
     A(b.se3[1], c.max_nodes);
@@ -609,6 +627,14 @@ mvdr_status mvdr_solver_bench_stage(mvdr_solver* s, int stage, uint32_t reps, vo
             case MVDR_STAGE_PCG_UPDATE: launch_pcg_update_once(s->b, fp64, st); break;
             case MVDR_STAGE_SPMV: launch_spmv_once(s->b, fp64, c.max_nodes, st); break;
             case MVDR_STAGE_INDEX: enqueue_solve(s, st, true, 0, 0); break;
+            case MVDR_STAGE_CG_ITERATION:
+                if (!fp64) {
+                    (void)hipEventDestroy(e0);
+                    (void)hipEventDestroy(e1);
+                    return fail(s, MVDR_E_INVALID, "the fused CG iteration runs in the mixed precision mode");
+                }
+                launch_cg_iter_once(s->b, c.max_nodes, st);
+                break;
             default:
                 (void)hipEventDestroy(e0);
                 (void)hipEventDestroy(e1);

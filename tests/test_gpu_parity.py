@@ -206,7 +206,7 @@ def test_stage_bench_entry_point(run1):
 
     _, solver, _, _ = run1
     out = C.c_float()
-    for stage in range(5):
+    for stage in range(6):  # run1 is mixed precision: the fused CG iteration stage exists
         solver._check(lib().mvdr_solver_bench_stage(solver.h, stage, 3, _stream_handle(None), C.byref(out)))
         assert out.value > 0
 
