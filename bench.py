@@ -5,9 +5,11 @@ Workload (default): the three-view body scene -- ~150k surfels, ~4k nodes,
 3 x 640x480 depth + foreground, ~300 feature pairs per view, ~2000 cross-view
 pairs -- solved with 5 Gauss-Newton x 10 PCG iterations in the mixed-precision
 mode (FP32 Jacobians / J^T J blocks, FP64 dots, alpha/beta and 6x6 inverses,
-one residual-refinement step per GN iteration).  A "step" is one full solve of
-one frame: index build + all GN iterations + warp updates, with every input
-already resident in HBM.
+one residual-refinement step per GN iteration).  A "step" is one batched solve
+of --batch independent frames of that workload (distinct seeds), one launch per
+stage for all of them: index build + all GN iterations + warp updates, with
+every input already resident in HBM.  `value` is ms per frame (step time /
+frames); the latency of one frame solved alone is reported beside it.
 
 The JSON line carries the contract fields plus:
   roofline      the dominant kernel's algorithmic bytes / launch time vs 8 TB/s
@@ -49,6 +51,7 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=8, help="frames in the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage-reps", type=int, default=50)
+    ap.add_argument("--batch", type=int, default=16, help="frames solved per step (one launch per stage)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first: the known-size kernel that "
                          "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -159,10 +162,11 @@ def time_solves(solver, steps, warmup):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
-def measured_traffic(kernel, workload):
+def measured_traffic(kernel, workload, frames):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
-    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{WORKLOAD_NAMES[workload]}.md")
+    (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated) for this many frames per
+    launch, or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{WORKLOAD_NAMES[workload]}_x{frames}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -182,10 +186,11 @@ def main():
     from multiviewdynrecon_amd.config import workload_config
     from multiviewdynrecon_amd.frame import DeviceFrame
     from multiviewdynrecon_amd.scene import make_scene
-    from multiviewdynrecon_amd.solver import NonRigidSolver
+    from multiviewdynrecon_amd.solver import NonRigidBatch, NonRigidSolver
 
     overrides = dict(use_graph=args.graph)
-    if args.precision == "mixed":
+    mixed = args.precision == "mixed"
+    if mixed:
         overrides.update(precision=1, refine_steps=1)
     else:
         overrides.update(precision=0, refine_steps=0)
@@ -200,37 +205,47 @@ def main():
         print(f"[bench] calibration copy: {args.copy_mib} MiB read + {args.copy_mib} MiB written", file=sys.stderr)
         del src, dst
 
-    scene = make_scene(args.workload)
-    frame = DeviceFrame(scene)
-    solver = NonRigidSolver(cfg)
-    solver.SetSolverInput(frame)
-    for _ in range(args.warmup):
-        solver.SolveNonRigidAlignment()
-    torch.cuda.synchronize()
-    print(f"[bench] warmup done ({args.warmup} solves)", file=sys.stderr, flush=True)
+    # B independent frames of the workload (distinct seeds); frame 0 is the
+    # default-seed scene that the latency, error and CPU legs use
+    B = max(1, args.batch)
+    scenes = [make_scene(args.workload)] + [make_scene(args.workload, seed=1000 + k) for k in range(1, B)]
+    frames = [DeviceFrame(sc) for sc in scenes]
+    scene, frame = scenes[0], frames[0]
+    solvers = []
+    for fr in frames:
+        s = NonRigidSolver(cfg)
+        s.SetSolverInput(fr)
+        solvers.append(s)
+    batch = NonRigidBatch(solvers)
+    print(f"[bench] {B} frames resident", file=sys.stderr, flush=True)
 
-    # timed region: K full solves, synchronised on both sides
-    per = []
+    # timed region: K steps, one step = one batched solve of the B frames
+    for _ in range(args.warmup):
+        batch.SolveNonRigidAlignment()
     torch.cuda.synchronize()
+    per = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
-        solver.SolveNonRigidAlignment()
+        batch.SolveNonRigidAlignment()
         e1.record()
         per.append((e0, e1))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     event_ms = [a.elapsed_time(b) for a, b in per]
     ms_per_step = wall * 1e3 / args.steps
-    sizes = solver.sizes()
-    print(f"[bench] {args.steps} solves: wall {ms_per_step:.3f} ms/frame, event median {np.median(event_ms):.3f}",
-          file=sys.stderr, flush=True)
+    ms_per_frame = ms_per_step / B
+    print(f"[bench] {args.steps} steps of {B} frames: {ms_per_step:.3f} ms/step, {ms_per_frame:.4f} ms/frame "
+          f"(event median {np.median(event_ms):.3f} ms/step)", file=sys.stderr, flush=True)
 
-    # per-stage kernel timing on the solver's stream (hipEvents around repeated launches)
-    mixed = args.precision == "mixed"
-    # stages of this precision's solve and their launches per solve
+    # one frame alone: latency of a single solve
+    single = solvers[0]
+    latency_ms = time_solves(single, args.steps, 2)
+    print(f"[bench] single-frame latency {latency_ms:.3f} ms", file=sys.stderr, flush=True)
+
+    # per-stage kernel timing of the batched launches (hipEvents on the stream)
     if mixed:
         per_solve = {"eval_terms": cfg.gn_iterations, "assemble": cfg.gn_iterations,
                      "cg_iteration": cfg.gn_iterations * cfg.pcg_iterations, "index": 1}
@@ -238,28 +253,30 @@ def main():
         per_solve = {"eval_terms": cfg.gn_iterations, "assemble": cfg.gn_iterations,
                      "pcg_update": cfg.gn_iterations * cfg.pcg_iterations,
                      "spmv": cfg.gn_iterations * cfg.pcg_iterations, "index": 1}
-    stages = {}
-    for st in per_solve:
-        stages[st] = stage_time_ms(solver, st, args.stage_reps)
-    solver.SolveNonRigidAlignment()  # restore a consistent state after the stage loops
+    batch.SolveNonRigidAlignment()
+    stages = {st: batch.bench_stage(STAGE_IDS[st], args.stage_reps) for st in per_solve}
+    batch.SolveNonRigidAlignment()  # restore a consistent state after the stage loops
     torch.cuda.synchronize()
-    ab = algorithmic_bytes(sizes, mixed)
+    all_sizes = [s.sizes() for s in solvers]
+    sizes = all_sizes[0]
+    ab = {}
+    for sz in all_sizes:
+        for k, v in algorithmic_bytes(sz, mixed).items():
+            ab[k] = ab.get(k, 0) + v
     share = {k: stages[k] * per_solve[k] for k in stages}
     print("[bench] stage ms/launch " + json.dumps({k: round(v, 4) for k, v in stages.items()}), file=sys.stderr)
-    print("[bench] stage ms/solve  " + json.dumps({k: round(v, 4) for k, v in share.items()}), file=sys.stderr)
+    print("[bench] stage ms/step   " + json.dumps({k: round(v, 4) for k, v in share.items()}), file=sys.stderr)
     apply_stage = "cg_iteration" if mixed else "spmv"
     dominant = max(("assemble", apply_stage), key=lambda k: share[k])
     achieved = ab[dominant] / (stages[dominant] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "bytes_per_launch": int(ab[dominant]), "launch_ms": round(stages[dominant], 5)}
-    spmv_gbs = ab[apply_stage] / (stages[apply_stage] * 1e-3) / 1e9
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": measured_traffic(dominant, args.workload, B),
+                "bytes_per_launch": int(ab[dominant]), "launch_ms": round(stages[dominant], 5), "frames_per_launch": B}
+    apply_gbs = ab[apply_stage] / (stages[apply_stage] * 1e-3) / 1e9
     jtj_gbs = ab["assemble"] / (stages["assemble"] * 1e-3) / 1e9
 
-    traffic = measured_traffic(dominant, args.workload)
-    roofline["traffic"] = traffic
-
-    # the plain FP32 GPU path on the same frame (reference precision, no graph)
+    # the plain FP32 GPU path on frame 0 (reference precision, one frame, no graph)
     plain_cfg = workload_config(args.workload, precision=0, refine_steps=0, use_graph=0)
     plain = NonRigidSolver(plain_cfg)
     plain.SetSolverInput(frame)
@@ -279,7 +296,7 @@ def main():
 
     out = {
         "metric": "non-rigid solve ms/frame (3 views, ~4k nodes) at <=1e-4 rel err vs FP64",
-        "value": round(ms_per_step, 4),
+        "value": round(ms_per_frame, 4),
         "unit": "ms/frame",
         "n_gpus": 1,
         "steps": args.steps,
@@ -288,21 +305,23 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (f64 reductions / 6x6 inverses)" if args.precision == "mixed" else "f32",
+        "dtype": "f32 (f64 reductions / 6x6 factors)" if mixed else "f32",
         "data": "synthetic (seeded scene generator)",
-        "config": {"workload": WORKLOAD_NAMES[args.workload], "views": scene.num_views, "width": scene.width,
-                   "height": scene.height, "nodes": scene.num_nodes, "surfels": scene.num_surfels,
-                   "gn_iterations": cfg.gn_iterations, "pcg_iterations": cfg.pcg_iterations,
-                   "precision": args.precision, "hip_graph": bool(args.graph), "dense_terms": sizes["num_dense_terms"],
-                   "node_pairs": sizes["num_node_pairs"], "parallelism": "single GPU"},
-        "event_ms_median": round(float(np.median(event_ms)), 4),
+        "config": {"workload": WORKLOAD_NAMES[args.workload], "frames_per_step": B, "views": scene.num_views,
+                   "width": scene.width, "height": scene.height, "nodes": scene.num_nodes,
+                   "surfels": scene.num_surfels, "gn_iterations": cfg.gn_iterations,
+                   "pcg_iterations": cfg.pcg_iterations, "precision": args.precision, "hip_graph": bool(args.graph),
+                   "dense_terms": sizes["num_dense_terms"], "node_pairs": sizes["num_node_pairs"],
+                   "parallelism": "single GPU, frames batched per launch"},
+        "event_ms_median_per_step": round(float(np.median(event_ms)), 4),
+        "latency_ms_single_frame": round(latency_ms, 4),
         "roofline": roofline,
-        "apply_jtj_gbs": round(spmv_gbs, 2),
+        "apply_jtj_gbs": round(apply_gbs, 2),
         "jtj_assembly_gbs": round(jtj_gbs, 2),
         "stage_ms": {k: round(v, 5) for k, v in stages.items()},
         "cpu_baseline": cpu,
         "twist_rel_err": max(errs) if errs else None,
-        "plain_fp32": {"ms_per_frame": round(plain_ms, 4), "speedup": round(plain_ms / ms_per_step, 3),
+        "plain_fp32": {"ms_per_frame": round(plain_ms, 4), "speedup": round(plain_ms / ms_per_frame, 3),
                        "twist_rel_err": max(plain_errs) if plain_errs else None},
     }
     print(json.dumps(out), flush=True)

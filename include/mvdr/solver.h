@@ -228,6 +228,26 @@ typedef enum {
  * overwrites.  Synchronises. */
 mvdr_status mvdr_solver_bench_stage(mvdr_solver* solver, int stage, uint32_t reps, void* stream, float* avg_ms);
 
+/* ------------------------------------------------------------------ batches
+ * No counterpart in the reference, which solves one frame at a time: a batch
+ * solves the current frames of several solver instances in one launch per
+ * stage (kernels index the frame by blockIdx.y), for many-scene throughput.
+ * Members must share gn/pcg iterations, penalties, term switches, precision
+ * and refinement (MVDR_E_SHAPE otherwise); capacities and frame sizes may
+ * differ.  The members must outlive the batch.  After mvdr_batch_solve every
+ * member's warp field is solved exactly as mvdr_solver_solve would, and its
+ * SolvedNodeSE3 / taps / sizes read as after a single solve. */
+typedef struct mvdr_batch mvdr_batch;
+
+mvdr_status mvdr_batch_create(mvdr_solver* const* solvers, uint32_t count, mvdr_batch** batch);
+void mvdr_batch_destroy(mvdr_batch* batch);
+const char* mvdr_batch_last_error(const mvdr_batch* batch);
+/* SolveNonRigidAlignment of every member (use_graph of the first member's
+ * config: the whole batch solve is captured once and replayed). */
+mvdr_status mvdr_batch_solve(mvdr_batch* batch, void* stream);
+/* mvdr_solver_bench_stage over all members at once (one launch per rep). */
+mvdr_status mvdr_batch_bench_stage(mvdr_batch* batch, int stage, uint32_t reps, void* stream, float* avg_ms);
+
 #ifdef __cplusplus
 }
 #endif

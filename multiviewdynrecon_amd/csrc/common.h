@@ -348,4 +348,13 @@ struct Buffers {
     DQ* se3[2];
 };
 
+// One launch over `frames` solver instances: the device array of their
+// Buffers (kernels read bs[blockIdx.y]) and the largest capacities among them,
+// which size the grids; every kernel bounds its work by its own frame's sizes.
+struct Batch {
+    const Buffers* dev;
+    uint32_t frames;
+    uint32_t cap_tiles, cap_nodes, cap_chunks, cap_pairs;
+};
+
 }  // namespace mvdr

@@ -75,7 +75,8 @@ struct ForegroundPred {
 // Pass 1: valid pixels per 4096-pixel tile.  Block 0 also clears the
 // counters the rest of the index build accumulates into.
 template <typename Pred>
-__global__ void __launch_bounds__(kTileThreads) compact_count_kernel(Buffers b, Pred pred, int clear) {
+__global__ void __launch_bounds__(kTileThreads) compact_count_kernel(const Buffers* __restrict__ bs, Pred pred, int clear) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     const uint32_t M = f.V * f.W * f.H;
     if (clear) {
@@ -105,7 +106,8 @@ __global__ void __launch_bounds__(kTileThreads) compact_count_kernel(Buffers b, 
 }
 
 // Pass 2: exclusive scan of the tile counts (one block).
-__global__ void __launch_bounds__(1024) compact_scan_kernel(Buffers b, int which) {
+__global__ void __launch_bounds__(1024) compact_scan_kernel(const Buffers* __restrict__ bs, int which) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     const uint32_t M = f.V * f.W * f.H;
     const uint32_t tiles = (M + kTilePixels - 1) / kTilePixels;
@@ -124,7 +126,8 @@ __global__ void __launch_bounds__(1024) compact_scan_kernel(Buffers b, int which
 // Pass 3: scatter in ascending pixel order (view, y, x), the order of the
 // reference's inclusive-scan compaction (ImageTermKNNFetcher.cu:33-68).
 template <typename Pred>
-__global__ void __launch_bounds__(kTileThreads) compact_scatter_kernel(Buffers b, Pred pred, int which) {
+__global__ void __launch_bounds__(kTileThreads) compact_scatter_kernel(const Buffers* __restrict__ bs, Pred pred, int which) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     const uint32_t M = f.V * f.W * f.H;
     const uint32_t base = blockIdx.x * kTilePixels;
@@ -296,7 +299,8 @@ __device__ inline bool choose_cross(const FrameDev& f, uint32_t k, P2PRec& rec, 
 
 // Block 0 compacts feature pairs, block 1 cross-view pairs, each in candidate
 // order with a running block-wide prefix.
-__global__ void __launch_bounds__(1024) select_pairs_kernel(Buffers b, int use_cross) {
+__global__ void __launch_bounds__(1024) select_pairs_kernel(const Buffers* __restrict__ bs, int use_cross) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     const bool is_cross = blockIdx.x == 1;
     if (is_cross && !use_cross) {
@@ -388,7 +392,8 @@ uint32_t hist_tile(uint32_t cap_nodes) { return std::min<uint32_t>(cap_nodes, kH
 
 __device__ inline uint32_t num_chunks(const Counts& c) { return (c.off[4] + kChunkTerms - 1) / kChunkTerms; }
 
-__global__ void __launch_bounds__(64) n2t_hist_kernel(Buffers b, uint32_t tile) {
+__global__ void __launch_bounds__(64) n2t_hist_kernel(const Buffers* __restrict__ bs, uint32_t tile) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     Counts& c = *b.counts;
     uint32_t off[5];
@@ -431,7 +436,8 @@ __global__ void __launch_bounds__(64) n2t_hist_kernel(Buffers b, uint32_t tile) 
 // number of node-n entries in chunks before c; n2t_count[n] the node's total.
 constexpr uint32_t kScanWaves = 16;
 
-__global__ void __launch_bounds__(64 * kScanWaves) n2t_colscan_kernel(Buffers b) {
+__global__ void __launch_bounds__(64 * kScanWaves) n2t_colscan_kernel(const Buffers* __restrict__ bs) {
+    const Buffers& b = bs[blockIdx.y];
     const uint32_t N = b.frame->N;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t n = blockIdx.x * 64 + lane;
@@ -462,7 +468,8 @@ __global__ void __launch_bounds__(64 * kScanWaves) n2t_colscan_kernel(Buffers b)
     if (w == kScanWaves - 1) b.n2t_count[n] = total;
 }
 
-__global__ void __launch_bounds__(64) n2t_place_kernel(Buffers b, uint32_t tile) {
+__global__ void __launch_bounds__(64) n2t_place_kernel(const Buffers* __restrict__ bs, uint32_t tile) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
@@ -529,7 +536,8 @@ __global__ void __launch_bounds__(64) n2t_place_kernel(Buffers b, uint32_t tile)
 // Exclusive scans over the N node counters.  which = 0: Node2Term lists;
 // which = 1: pairs (i, j > i) per row (block 0), lower entries per row
 // (block 1), pair-term entries per row (block 2).
-__global__ void __launch_bounds__(1024) node_scan_kernel(Buffers b, int which) {
+__global__ void __launch_bounds__(1024) node_scan_kernel(const Buffers* __restrict__ bs, int which) {
+    const Buffers& b = bs[blockIdx.y];
     const uint32_t N = b.frame->N;
     Counts& c = *b.counts;
     const int job = which == 0 ? 0 : 1 + (int)blockIdx.x;
@@ -592,7 +600,8 @@ __device__ inline uint32_t lower_bound_lds(const uint32_t* a, uint32_t n, uint32
 constexpr uint32_t kRowWaves = 4;
 
 template <bool kWrite>
-__global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(Buffers b) {
+__global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers* __restrict__ bs) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     Counts& c = *b.counts;
     if (c.error) return;
@@ -764,7 +773,8 @@ __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(Buffers b) {
 
 // Pair index of every lower entry (k, .) of row j: the position of (k, j) in
 // row k's ascending upper list.
-__global__ void __launch_bounds__(256) pair_link_kernel(Buffers b) {
+__global__ void __launch_bounds__(256) pair_link_kernel(const Buffers* __restrict__ bs) {
+    const Buffers& b = bs[blockIdx.y];
     const Counts& c = *b.counts;
     if (c.error) return;
     const uint32_t N = b.frame->N;
@@ -786,28 +796,30 @@ __global__ void __launch_bounds__(256) pair_link_kernel(Buffers b) {
 }
 
 // ------------------------------------------------------------- host launch
-void launch_index_build(const Buffers& b, uint32_t cap_tiles, uint32_t cap_nodes, uint32_t cap_chunks,
-                        bool use_foreground, bool use_cross, hipStream_t stream) {
-    hipLaunchKernelGGL(compact_count_kernel<DensePred>, dim3(cap_tiles), dim3(kTileThreads), 0, stream, b, DensePred{}, 1);
-    hipLaunchKernelGGL(compact_scan_kernel, dim3(1), dim3(1024), 0, stream, b, 0);
-    hipLaunchKernelGGL(compact_scatter_kernel<DensePred>, dim3(cap_tiles), dim3(kTileThreads), 0, stream, b, DensePred{}, 0);
+void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, hipStream_t stream) {
+    const uint32_t F = B.frames;
+    hipLaunchKernelGGL(compact_count_kernel<DensePred>, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream, B.dev,
+                       DensePred{}, 1);
+    hipLaunchKernelGGL(compact_scan_kernel, dim3(1, F), dim3(1024), 0, stream, B.dev, 0);
+    hipLaunchKernelGGL(compact_scatter_kernel<DensePred>, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream, B.dev,
+                       DensePred{}, 0);
     if (use_foreground) {
-        hipLaunchKernelGGL(compact_count_kernel<ForegroundPred>, dim3(cap_tiles), dim3(kTileThreads), 0, stream, b,
-                           ForegroundPred{}, 0);
-        hipLaunchKernelGGL(compact_scan_kernel, dim3(1), dim3(1024), 0, stream, b, 1);
-        hipLaunchKernelGGL(compact_scatter_kernel<ForegroundPred>, dim3(cap_tiles), dim3(kTileThreads), 0, stream, b,
-                           ForegroundPred{}, 1);
+        hipLaunchKernelGGL(compact_count_kernel<ForegroundPred>, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream,
+                           B.dev, ForegroundPred{}, 0);
+        hipLaunchKernelGGL(compact_scan_kernel, dim3(1, F), dim3(1024), 0, stream, B.dev, 1);
+        hipLaunchKernelGGL(compact_scatter_kernel<ForegroundPred>, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream,
+                           B.dev, ForegroundPred{}, 1);
     }
-    hipLaunchKernelGGL(select_pairs_kernel, dim3(2), dim3(1024), 0, stream, b, use_cross ? 1 : 0);
-    const uint32_t tile = hist_tile(cap_nodes);
-    hipLaunchKernelGGL(n2t_hist_kernel, dim3(cap_chunks), dim3(64), tile * sizeof(uint32_t), stream, b, tile);
-    hipLaunchKernelGGL(n2t_colscan_kernel, dim3((cap_nodes + 63) / 64), dim3(64 * kScanWaves), 0, stream, b);
-    hipLaunchKernelGGL(node_scan_kernel, dim3(1), dim3(1024), 0, stream, b, 0);
-    hipLaunchKernelGGL(n2t_place_kernel, dim3(cap_chunks), dim3(64), tile * sizeof(uint32_t), stream, b, tile);
-    hipLaunchKernelGGL(pair_rows_kernel<false>, dim3(cap_nodes), dim3(64 * kRowWaves), 0, stream, b);
-    hipLaunchKernelGGL(node_scan_kernel, dim3(3), dim3(1024), 0, stream, b, 1);
-    hipLaunchKernelGGL(pair_rows_kernel<true>, dim3(cap_nodes), dim3(64 * kRowWaves), 0, stream, b);
-    hipLaunchKernelGGL(pair_link_kernel, dim3((cap_nodes + 255) / 256), dim3(256), 0, stream, b);
+    hipLaunchKernelGGL(select_pairs_kernel, dim3(2, F), dim3(1024), 0, stream, B.dev, use_cross ? 1 : 0);
+    const uint32_t tile = hist_tile(B.cap_nodes);
+    hipLaunchKernelGGL(n2t_hist_kernel, dim3(B.cap_chunks, F), dim3(64), tile * sizeof(uint32_t), stream, B.dev, tile);
+    hipLaunchKernelGGL(n2t_colscan_kernel, dim3((B.cap_nodes + 63) / 64, F), dim3(64 * kScanWaves), 0, stream, B.dev);
+    hipLaunchKernelGGL(node_scan_kernel, dim3(1, F), dim3(1024), 0, stream, B.dev, 0);
+    hipLaunchKernelGGL(n2t_place_kernel, dim3(B.cap_chunks, F), dim3(64), tile * sizeof(uint32_t), stream, B.dev, tile);
+    hipLaunchKernelGGL(pair_rows_kernel<false>, dim3(B.cap_nodes, F), dim3(64 * kRowWaves), 0, stream, B.dev);
+    hipLaunchKernelGGL(node_scan_kernel, dim3(3, F), dim3(1024), 0, stream, B.dev, 1);
+    hipLaunchKernelGGL(pair_rows_kernel<true>, dim3(B.cap_nodes, F), dim3(64 * kRowWaves), 0, stream, B.dev);
+    hipLaunchKernelGGL(pair_link_kernel, dim3((B.cap_nodes + 255) / 256, F), dim3(256), 0, stream, B.dev);
 }
 
 uint32_t index_chunks(uint64_t max_terms) { return (uint32_t)((max_terms + kChunkTerms - 1) / kChunkTerms); }

@@ -157,7 +157,8 @@ __device__ inline void row_apply(const Buffers& b, uint32_t i, uint32_t sub, con
 // r = b, d = M^-1 b, x = 0, partials of delta_0 = dot(r, d)
 // (blockPCGZeroInitKernel, BlockPCG.cuh:65-111, then cublasSdot).
 template <typename Acc>
-__global__ void __launch_bounds__(256) pcg_init_kernel(Buffers b) {
+__global__ void __launch_bounds__(256) pcg_init_kernel(const Buffers* __restrict__ bs) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     Acc acc = 0;
     if (!b.counts->error) {
@@ -181,8 +182,10 @@ __global__ void __launch_bounds__(256) pcg_init_kernel(Buffers b) {
 
 // Iteration k: q = A d with d = s + beta d_prev (k > 0); partials of dot(d, q).
 template <typename Acc>
-__global__ void __launch_bounds__(256) pcg_spmv_kernel(Buffers b, uint32_t k, const float* __restrict__ d_prev,
-                                                       float* __restrict__ d_out) {
+__global__ void __launch_bounds__(256) pcg_spmv_kernel(const Buffers* __restrict__ bs, uint32_t k, uint32_t cur) {
+    const Buffers& b = bs[blockIdx.y];
+    const float* __restrict__ d_prev = cur ? b.d1 : b.d0;
+    float* __restrict__ d_out = cur ? b.d0 : b.d1;
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     Acc beta = 0;
@@ -217,7 +220,9 @@ __global__ void __launch_bounds__(256) pcg_spmv_kernel(Buffers b, uint32_t k, co
 // s = M^-1 r; partials of delta_{k+1} = dot(r, s) (blockPCGSecondUpdateKernel,
 // BlockPCG.cuh:120-161).  A zero denominator gives a zero step.
 template <typename Acc>
-__global__ void __launch_bounds__(256) pcg_update_kernel(Buffers b, uint32_t k, uint32_t spmv_blocks, const float* __restrict__ d) {
+__global__ void __launch_bounds__(256) pcg_update_kernel(const Buffers* __restrict__ bs, uint32_t k, uint32_t spmv_blocks, uint32_t dsel) {
+    const Buffers& b = bs[blockIdx.y];
+    const float* __restrict__ d = dsel ? b.d1 : b.d0;
     const FrameDev& f = *b.frame;
     const Acc delta = reduce_partials<Acc>(slot_ptr(b, kSlotDelta + k), kVecBlocks);
     const Acc dq = reduce_partials<Acc>(slot_ptr(b, kSlotDQ + k), spmv_blocks);
@@ -248,7 +253,8 @@ __global__ void __launch_bounds__(256) pcg_update_kernel(Buffers b, uint32_t k, 
 // Residual refinement (mixed mode), one preconditioned steepest-descent step
 // on the true residual: r = b - A x, z = M^-1 r, num = dot(r, z) ...
 template <typename Acc>
-__global__ void __launch_bounds__(256) refine_residual_kernel(Buffers b) {
+__global__ void __launch_bounds__(256) refine_residual_kernel(const Buffers* __restrict__ bs) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     Acc acc = 0;
@@ -279,7 +285,8 @@ __global__ void __launch_bounds__(256) refine_residual_kernel(Buffers b) {
 
 // ... den = dot(z, A z) ...
 template <typename Acc>
-__global__ void __launch_bounds__(256) refine_curvature_kernel(Buffers b) {
+__global__ void __launch_bounds__(256) refine_curvature_kernel(const Buffers* __restrict__ bs) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     Acc acc = 0;
@@ -300,7 +307,8 @@ __global__ void __launch_bounds__(256) refine_curvature_kernel(Buffers b) {
 
 // ... x += (num / den) z
 template <typename Acc>
-__global__ void __launch_bounds__(256) refine_step_kernel(Buffers b, uint32_t spmv_blocks) {
+__global__ void __launch_bounds__(256) refine_step_kernel(const Buffers* __restrict__ bs, uint32_t spmv_blocks) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     const Acc num = reduce_partials<Acc>(slot_ptr(b, kSlotRefNum), spmv_blocks);
     const Acc den = reduce_partials<Acc>(slot_ptr(b, kSlotRefDen), spmv_blocks);
@@ -331,7 +339,8 @@ constexpr uint32_t kSlotCgGamma = 0;   // gamma_i partials: slot i
 constexpr uint32_t kSlotCgDelta = 72;  // delta_i partials: slot 72 + i
 
 // A^ blocks of the pairs and b^ of the nodes.
-__global__ void __launch_bounds__(256) cg_transform_kernel(Buffers b) {
+__global__ void __launch_bounds__(256) cg_transform_kernel(const Buffers* __restrict__ bs) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
@@ -412,7 +421,8 @@ __device__ inline void hat_row_apply(const Buffers& b, uint32_t i, uint32_t sub,
 }
 
 // w_0 = A^ r_0 (r_0 = b^), x^ = p_{-1} = s_{-1} = 0, partials of gamma_0, delta_0.
-__global__ void __launch_bounds__(256) cg_init_kernel(Buffers b) {
+__global__ void __launch_bounds__(256) cg_init_kernel(const Buffers* __restrict__ bs) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     double g = 0.0, d = 0.0;
@@ -468,7 +478,8 @@ __device__ inline void cg_scalars(const Buffers& b, uint32_t i, uint32_t blocks,
 
 // Launch i of the fused CG (see above).  Reads r_i, w_i, s_{i-1}; writes
 // r_{i+1}, w_{i+1}, s_i, p_i, x^_{i+1}.
-__global__ void __launch_bounds__(256) cg_iter_kernel(Buffers b, uint32_t i) {
+__global__ void __launch_bounds__(256) cg_iter_kernel(const Buffers* __restrict__ bs, uint32_t i) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     double alpha, beta;
@@ -521,7 +532,8 @@ __global__ void __launch_bounds__(256) cg_iter_kernel(Buffers b, uint32_t i) {
 // Residual refinement on the CG solution (preconditioned steepest descent on
 // the true residual, as refine_* above, in the A^ coordinates):
 // r^ = b^ - A^ x^, num = (r^, r^) ...
-__global__ void __launch_bounds__(256) cg_refine_residual_kernel(Buffers b) {
+__global__ void __launch_bounds__(256) cg_refine_residual_kernel(const Buffers* __restrict__ bs) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     double acc = 0.0;
@@ -551,7 +563,8 @@ __global__ void __launch_bounds__(256) cg_refine_residual_kernel(Buffers b) {
 }
 
 // ... den = (r^, A^ r^) ...
-__global__ void __launch_bounds__(256) cg_refine_curvature_kernel(Buffers b) {
+__global__ void __launch_bounds__(256) cg_refine_curvature_kernel(const Buffers* __restrict__ bs) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     double acc = 0.0;
@@ -580,7 +593,8 @@ __global__ void __launch_bounds__(256) cg_refine_curvature_kernel(Buffers b) {
 
 // ... x^ += (num / den) r^ (all refinement steps but the last; the last is
 // folded into the warp update).
-__global__ void __launch_bounds__(256) cg_refine_step_kernel(Buffers b, uint32_t spmv_blocks) {
+__global__ void __launch_bounds__(256) cg_refine_step_kernel(const Buffers* __restrict__ bs, uint32_t spmv_blocks) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     const double num = reduce_partials<double>(slot_ptr(b, kSlotRefNum), spmv_blocks);
     const double den = reduce_partials<double>(slot_ptr(b, kSlotRefDen), spmv_blocks);
@@ -643,8 +657,10 @@ __device__ inline Mat34 mat34_of(DQ d) {
 // kHat: the solution is x = L^-T (x^ + a r^) of the fused CG, a the last
 // refinement step (0 without refinement).
 template <bool kHat>
-__global__ void __launch_bounds__(256) apply_update_kernel(Buffers b, const DQ* __restrict__ se3_in, DQ* __restrict__ se3_out,
-                                                           uint32_t refine_blocks) {
+__global__ void __launch_bounds__(256) apply_update_kernel(const Buffers* __restrict__ bs, uint32_t k, uint32_t refine_blocks) {
+    const Buffers& b = bs[blockIdx.y];
+    const DQ* __restrict__ se3_in = b.se3[k & 1];
+    DQ* __restrict__ se3_out = b.se3[(k + 1) & 1];
     const FrameDev& f = *b.frame;
     const uint32_t stride = gridDim.x * blockDim.x;
     double step = 0.0;
@@ -706,7 +722,9 @@ __global__ void __launch_bounds__(256) apply_update_kernel(Buffers b, const DQ* 
 }
 
 // Start of a frame: the warp field the GN loop reads first is the input one.
-__global__ void copy_se3_kernel(Buffers b, DQ* __restrict__ dst) {
+__global__ void copy_se3_kernel(const Buffers* __restrict__ bs) {
+    const Buffers& b = bs[blockIdx.y];
+    DQ* __restrict__ dst = b.se3[0];
     const FrameDev& f = *b.frame;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < f.N; n += stride) dst[n] = f.node_se3_in[n];
@@ -718,71 +736,74 @@ static uint32_t spmv_grid(uint32_t cap_nodes) {
 }
 
 template <typename Acc>
-static void pcg_impl(const Buffers& b, uint32_t iters, uint32_t refine, uint32_t cap_nodes, hipStream_t stream) {
-    const uint32_t sg = spmv_grid(cap_nodes);
+static void pcg_impl(const Batch& B, uint32_t iters, uint32_t refine, hipStream_t stream) {
+    const uint32_t sg = spmv_grid(B.cap_nodes);
     iters = std::min(iters, kMaxIters);
-    hipLaunchKernelGGL(pcg_init_kernel<Acc>, dim3(kVecBlocks), dim3(256), 0, stream, b);
-    float* dbuf[2] = {b.d0, b.d1};
-    int cur = 0;
+    hipLaunchKernelGGL(pcg_init_kernel<Acc>, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev);
+    uint32_t cur = 0;
     for (uint32_t k = 0; k < iters; ++k) {
-        hipLaunchKernelGGL(pcg_spmv_kernel<Acc>, dim3(sg), dim3(256), 0, stream, b, k, dbuf[cur], dbuf[cur ^ 1]);
+        hipLaunchKernelGGL(pcg_spmv_kernel<Acc>, dim3(sg, B.frames), dim3(256), 0, stream, B.dev, k, cur);
         cur ^= 1;
-        hipLaunchKernelGGL(pcg_update_kernel<Acc>, dim3(kVecBlocks), dim3(256), 0, stream, b, k, sg, dbuf[cur]);
+        hipLaunchKernelGGL(pcg_update_kernel<Acc>, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev, k, sg, cur);
     }
     for (uint32_t r = 0; r < refine; ++r) {
-        hipLaunchKernelGGL(refine_residual_kernel<Acc>, dim3(sg), dim3(256), 0, stream, b);
-        hipLaunchKernelGGL(refine_curvature_kernel<Acc>, dim3(sg), dim3(256), 0, stream, b);
-        hipLaunchKernelGGL(refine_step_kernel<Acc>, dim3(kVecBlocks), dim3(256), 0, stream, b, sg);
+        hipLaunchKernelGGL(refine_residual_kernel<Acc>, dim3(sg, B.frames), dim3(256), 0, stream, B.dev);
+        hipLaunchKernelGGL(refine_curvature_kernel<Acc>, dim3(sg, B.frames), dim3(256), 0, stream, B.dev);
+        hipLaunchKernelGGL(refine_step_kernel<Acc>, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev, sg);
     }
 }
 
-void launch_pcg(const Buffers& b, uint32_t iters, uint32_t refine, bool fp64, uint32_t cap_nodes, hipStream_t stream) {
-    if (fp64) pcg_impl<double>(b, iters, refine, cap_nodes, stream);
-    else pcg_impl<float>(b, iters, refine, cap_nodes, stream);
+void launch_pcg(const Batch& B, uint32_t iters, uint32_t refine, bool fp64, hipStream_t stream) {
+    if (fp64) pcg_impl<double>(B, iters, refine, stream);
+    else pcg_impl<float>(B, iters, refine, stream);
 }
 
 // One ApplyJtJ (q = A d0, no direction update) for stage timing.
-void launch_spmv_once(const Buffers& b, bool fp64, uint32_t cap_nodes, hipStream_t stream) {
-    const uint32_t sg = spmv_grid(cap_nodes);
-    if (fp64) hipLaunchKernelGGL(pcg_spmv_kernel<double>, dim3(sg), dim3(256), 0, stream, b, 0u, b.d0, b.d1);
-    else hipLaunchKernelGGL(pcg_spmv_kernel<float>, dim3(sg), dim3(256), 0, stream, b, 0u, b.d0, b.d1);
+void launch_spmv_once(const Batch& B, bool fp64, hipStream_t stream) {
+    const uint32_t sg = spmv_grid(B.cap_nodes);
+    if (fp64) hipLaunchKernelGGL(pcg_spmv_kernel<double>, dim3(sg, B.frames), dim3(256), 0, stream, B.dev, 0u, 0u);
+    else hipLaunchKernelGGL(pcg_spmv_kernel<float>, dim3(sg, B.frames), dim3(256), 0, stream, B.dev, 0u, 0u);
 }
 
-void launch_pcg_update_once(const Buffers& b, bool fp64, hipStream_t stream) {
-    const uint32_t sg = spmv_grid(b.cap_nodes);
-    if (fp64) hipLaunchKernelGGL(pcg_update_kernel<double>, dim3(kVecBlocks), dim3(256), 0, stream, b, 0u, sg, b.d1);
-    else hipLaunchKernelGGL(pcg_update_kernel<float>, dim3(kVecBlocks), dim3(256), 0, stream, b, 0u, sg, b.d1);
+void launch_pcg_update_once(const Batch& B, bool fp64, hipStream_t stream) {
+    const uint32_t sg = spmv_grid(B.cap_nodes);
+    if (fp64)
+        hipLaunchKernelGGL(pcg_update_kernel<double>, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev, 0u, sg, 1u);
+    else
+        hipLaunchKernelGGL(pcg_update_kernel<float>, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev, 0u, sg, 1u);
 }
 
-void launch_apply_update(const Buffers& b, const DQ* in, DQ* out, hipStream_t stream) {
-    hipLaunchKernelGGL(apply_update_kernel<false>, dim3(64), dim3(256), 0, stream, b, in, out, 0u);
+// Warp update of GN iteration k: se3[k & 1] -> se3[(k + 1) & 1].
+void launch_apply_update(const Batch& B, uint32_t k, hipStream_t stream) {
+    hipLaunchKernelGGL(apply_update_kernel<false>, dim3(64, B.frames), dim3(256), 0, stream, B.dev, k, 0u);
 }
 
 // Fused CG of the mixed precision mode: transform, init, one launch per
-// iteration, refinement, warp update.
-void launch_cg_solve(const Buffers& b, uint32_t iters, uint32_t refine, uint32_t cap_nodes, const DQ* in, DQ* out,
-                     hipStream_t stream) {
-    const uint32_t sg = spmv_grid(cap_nodes);
+// iteration, refinement, warp update of GN iteration k.
+void launch_cg_solve(const Batch& B, uint32_t iters, uint32_t refine, uint32_t k, hipStream_t stream) {
+    const uint32_t sg = spmv_grid(B.cap_nodes);
     iters = std::min(iters, kMaxIters);
-    const uint32_t tgrid = std::min<uint32_t>((b.cap_pairs + cap_nodes + 255) / 256, 2048u);
-    hipLaunchKernelGGL(cg_transform_kernel, dim3(tgrid), dim3(256), 0, stream, b);
-    hipLaunchKernelGGL(cg_init_kernel, dim3(sg), dim3(256), 0, stream, b);
-    for (uint32_t i = 0; i < iters; ++i) hipLaunchKernelGGL(cg_iter_kernel, dim3(sg), dim3(256), 0, stream, b, i);
+    const uint32_t tgrid = std::min<uint32_t>((B.cap_pairs + B.cap_nodes + 255) / 256, 2048u);
+    hipLaunchKernelGGL(cg_transform_kernel, dim3(tgrid, B.frames), dim3(256), 0, stream, B.dev);
+    hipLaunchKernelGGL(cg_init_kernel, dim3(sg, B.frames), dim3(256), 0, stream, B.dev);
+    for (uint32_t i = 0; i < iters; ++i)
+        hipLaunchKernelGGL(cg_iter_kernel, dim3(sg, B.frames), dim3(256), 0, stream, B.dev, i);
     for (uint32_t r = 0; r < refine; ++r) {
-        hipLaunchKernelGGL(cg_refine_residual_kernel, dim3(sg), dim3(256), 0, stream, b);
-        hipLaunchKernelGGL(cg_refine_curvature_kernel, dim3(sg), dim3(256), 0, stream, b);
-        if (r + 1 < refine) hipLaunchKernelGGL(cg_refine_step_kernel, dim3(kVecBlocks), dim3(256), 0, stream, b, sg);
+        hipLaunchKernelGGL(cg_refine_residual_kernel, dim3(sg, B.frames), dim3(256), 0, stream, B.dev);
+        hipLaunchKernelGGL(cg_refine_curvature_kernel, dim3(sg, B.frames), dim3(256), 0, stream, B.dev);
+        if (r + 1 < refine)
+            hipLaunchKernelGGL(cg_refine_step_kernel, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev, sg);
     }
-    hipLaunchKernelGGL(apply_update_kernel<true>, dim3(64), dim3(256), 0, stream, b, in, out, refine ? sg : 0u);
+    hipLaunchKernelGGL(apply_update_kernel<true>, dim3(64, B.frames), dim3(256), 0, stream, B.dev, k, refine ? sg : 0u);
 }
 
 // One fused CG iteration (stage timing).
-void launch_cg_iter_once(const Buffers& b, uint32_t cap_nodes, hipStream_t stream) {
-    hipLaunchKernelGGL(cg_iter_kernel, dim3(spmv_grid(cap_nodes)), dim3(256), 0, stream, b, 1u);
+void launch_cg_iter_once(const Batch& B, hipStream_t stream) {
+    hipLaunchKernelGGL(cg_iter_kernel, dim3(spmv_grid(B.cap_nodes), B.frames), dim3(256), 0, stream, B.dev, 1u);
 }
 
-void launch_copy_se3(const Buffers& b, DQ* dst, hipStream_t stream) {
-    hipLaunchKernelGGL(copy_se3_kernel, dim3(64), dim3(256), 0, stream, b, dst);
+void launch_copy_se3(const Batch& B, hipStream_t stream) {
+    hipLaunchKernelGGL(copy_se3_kernel, dim3(64, B.frames), dim3(256), 0, stream, B.dev);
 }
 
 }  // namespace mvdr

@@ -99,7 +99,9 @@ __device__ inline void eval_image(const FrameDev& f, const DQ* se3, ushort4 px, 
 // (NodeGraphSmoothHandler.cu:10-43), feature / cross terms their canonical
 // vertex (SparseCorrespondenceHandler.cu:193-220).
 template <typename R>
-__global__ void __launch_bounds__(256) eval_terms_kernel(Buffers b, const DQ* __restrict__ se3) {
+__global__ void __launch_bounds__(256) eval_terms_kernel(const Buffers* __restrict__ bs, uint32_t which) {
+    const Buffers& b = bs[blockIdx.y];
+    const DQ* __restrict__ se3 = b.se3[which];
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
@@ -459,7 +461,8 @@ __device__ inline void pair_chunk(const Buffers& b, const Counts& c, const Penal
     for (int q = 0; q < (int)kPairVals; ++q) out[q] = v[q];
 }
 
-__global__ void __launch_bounds__(256) assemble_chunks_kernel(Buffers b, Penalty pen) {
+__global__ void __launch_bounds__(256) assemble_chunks_kernel(const Buffers* __restrict__ bs, Penalty pen) {
+    const Buffers& b = bs[blockIdx.y];
     const Counts& c = *b.counts;
     if (c.error) return;
     const uint32_t lane = threadIdx.x & 63;
@@ -476,7 +479,8 @@ __global__ void __launch_bounds__(256) assemble_chunks_kernel(Buffers b, Penalty
 // slot A.  Then the node rows get the damping and their block-Jacobi inverse
 // (blockDiagonalInverseKernel, BlockDiagonalPreconditionerInverse.cu:6-41).
 template <typename InvT>
-__global__ void __launch_bounds__(256) assemble_fixup_kernel(Buffers b) {
+__global__ void __launch_bounds__(256) assemble_fixup_kernel(const Buffers* __restrict__ bs) {
+    const Buffers& b = bs[blockIdx.y];
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
@@ -554,7 +558,8 @@ __global__ void __launch_bounds__(256) assemble_fixup_kernel(Buffers b) {
 // ---------------------------------------------------------- total residual
 // computeTermResidualKernel (ResidualEvaluator.cu:71-142): sum of squared
 // lambda-weighted residuals, accumulated in double.
-__global__ void __launch_bounds__(256) cost_kernel(Buffers b, Penalty pen, int density_in_jtj) {
+__global__ void __launch_bounds__(256) cost_kernel(const Buffers* __restrict__ bs, Penalty pen, int density_in_jtj) {
+    const Buffers& b = bs[blockIdx.y];
     const Counts& c = *b.counts;
     double acc = 0.0;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -591,26 +596,27 @@ __global__ void __launch_bounds__(256) cost_kernel(Buffers b, Penalty pen, int d
 }
 
 // ------------------------------------------------------------- host launch
-void launch_eval_terms(const Buffers& b, const DQ* se3, bool fp64_geometry, hipStream_t stream) {
+// Terms at the warp field se3[which].
+void launch_eval_terms(const Batch& B, uint32_t which, bool fp64_geometry, hipStream_t stream) {
     if (fp64_geometry)
-        hipLaunchKernelGGL(eval_terms_kernel<double>, dim3(1024), dim3(256), 0, stream, b, se3);
+        hipLaunchKernelGGL(eval_terms_kernel<double>, dim3(1024, B.frames), dim3(256), 0, stream, B.dev, which);
     else
-        hipLaunchKernelGGL(eval_terms_kernel<float>, dim3(1024), dim3(256), 0, stream, b, se3);
+        hipLaunchKernelGGL(eval_terms_kernel<float>, dim3(1024, B.frames), dim3(256), 0, stream, B.dev, which);
 }
 
-void launch_assemble(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream) {
-    hipLaunchKernelGGL(assemble_chunks_kernel, dim3(2048), dim3(256), 0, stream, b, pen);
-    const uint32_t fix_blocks = std::min<uint32_t>((cap_nodes + b.cap_pairs + 255) / 256, 1024u);
+void launch_assemble(const Batch& B, const Penalty& pen, bool fp64_inverse, hipStream_t stream) {
+    hipLaunchKernelGGL(assemble_chunks_kernel, dim3(2048, B.frames), dim3(256), 0, stream, B.dev, pen);
+    const uint32_t fix_blocks = std::min<uint32_t>((B.cap_nodes + B.cap_pairs + 255) / 256, 1024u);
     if (fp64_inverse)
-        hipLaunchKernelGGL(assemble_fixup_kernel<double>, dim3(fix_blocks), dim3(256), 0, stream, b);
+        hipLaunchKernelGGL(assemble_fixup_kernel<double>, dim3(fix_blocks, B.frames), dim3(256), 0, stream, B.dev);
     else
-        hipLaunchKernelGGL(assemble_fixup_kernel<float>, dim3(fix_blocks), dim3(256), 0, stream, b);
+        hipLaunchKernelGGL(assemble_fixup_kernel<float>, dim3(fix_blocks, B.frames), dim3(256), 0, stream, B.dev);
 }
 
-// Returns the number of partials written to slot kSlotCost.
-uint32_t launch_cost(const Buffers& b, const Penalty& pen, bool density_in_jtj, hipStream_t stream) {
+// Returns the number of partials written to slot kSlotCost (per frame).
+uint32_t launch_cost(const Batch& B, const Penalty& pen, bool density_in_jtj, hipStream_t stream) {
     const uint32_t blocks = 256;
-    hipLaunchKernelGGL(cost_kernel, dim3(blocks), dim3(256), 0, stream, b, pen, density_in_jtj ? 1 : 0);
+    hipLaunchKernelGGL(cost_kernel, dim3(blocks, B.frames), dim3(256), 0, stream, B.dev, pen, density_in_jtj ? 1 : 0);
     return blocks;
 }
 

@@ -12,20 +12,18 @@
 #include "common.h"
 
 namespace mvdr {
-void launch_index_build(const Buffers& b, uint32_t cap_tiles, uint32_t cap_nodes, uint32_t cap_chunks,
-                        bool use_foreground, bool use_cross, hipStream_t stream);
+void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, hipStream_t stream);
 uint32_t index_chunks(uint64_t max_terms);
-void launch_eval_terms(const Buffers& b, const DQ* se3, bool fp64_geometry, hipStream_t stream);
-void launch_assemble(const Buffers& b, const Penalty& pen, bool fp64_inverse, uint32_t cap_nodes, hipStream_t stream);
-void launch_spmv_once(const Buffers& b, bool fp64, uint32_t cap_nodes, hipStream_t stream);
-void launch_pcg_update_once(const Buffers& b, bool fp64, hipStream_t stream);
-uint32_t launch_cost(const Buffers& b, const Penalty& pen, bool density_in_jtj, hipStream_t stream);
-void launch_pcg(const Buffers& b, uint32_t iters, uint32_t refine, bool fp64, uint32_t cap_nodes, hipStream_t stream);
-void launch_apply_update(const Buffers& b, const DQ* in, DQ* out, hipStream_t stream);
-void launch_cg_solve(const Buffers& b, uint32_t iters, uint32_t refine, uint32_t cap_nodes, const DQ* in, DQ* out,
-                     hipStream_t stream);
-void launch_cg_iter_once(const Buffers& b, uint32_t cap_nodes, hipStream_t stream);
-void launch_copy_se3(const Buffers& b, DQ* dst, hipStream_t stream);
+void launch_eval_terms(const Batch& B, uint32_t which, bool fp64_geometry, hipStream_t stream);
+void launch_assemble(const Batch& B, const Penalty& pen, bool fp64_inverse, hipStream_t stream);
+void launch_spmv_once(const Batch& B, bool fp64, hipStream_t stream);
+void launch_pcg_update_once(const Batch& B, bool fp64, hipStream_t stream);
+uint32_t launch_cost(const Batch& B, const Penalty& pen, bool density_in_jtj, hipStream_t stream);
+void launch_pcg(const Batch& B, uint32_t iters, uint32_t refine, bool fp64, hipStream_t stream);
+void launch_apply_update(const Batch& B, uint32_t k, hipStream_t stream);
+void launch_cg_solve(const Batch& B, uint32_t iters, uint32_t refine, uint32_t k, hipStream_t stream);
+void launch_cg_iter_once(const Batch& B, hipStream_t stream);
+void launch_copy_se3(const Batch& B, hipStream_t stream);
 }  // namespace mvdr
 
 using namespace mvdr;
@@ -38,6 +36,8 @@ struct mvdr_solver {
     std::vector<void*> allocations;
     uint32_t cap_tiles = 0;
     uint32_t cap_chunks = 0;            // Node2Term counting-sort chunks
+    Buffers* b_dev = nullptr;           // device copy of b (a batch of one)
+    Batch self{};
     bool has_input = false;
     bool indexed = false;
     uint32_t iter = 0;                  // GN iterations done on the current frame
@@ -52,6 +52,25 @@ struct mvdr_solver {
         if (graph) (void)hipGraphDestroy(graph);
         if (done) (void)hipEventDestroy(done);
         for (void* p : allocations) (void)hipFree(p);
+    }
+};
+
+struct mvdr_batch {
+    std::vector<mvdr_solver*> members;
+    mvdr_solver_config cfg{};
+    Buffers* dev = nullptr;             // the members' Buffers, one per frame
+    Batch batch{};
+    hipEvent_t done = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::string error;
+
+    ~mvdr_batch() {
+        if (done) (void)hipEventSynchronize(done);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (done) (void)hipEventDestroy(done);
+        if (dev) (void)hipFree(dev);
     }
 };
 
@@ -110,27 +129,30 @@ const char* error_text(uint32_t bits) {
     return "capacity exceeded";
 }
 
-void enqueue_solve(mvdr_solver* s, hipStream_t stream, bool index, uint32_t first_iter, uint32_t iters) {
-    const mvdr_solver_config& c = s->cfg;
+// Index build and/or GN iterations [first_iter, first_iter + iters) of every
+// frame of the batch B, all sharing the schedule of config c.
+void enqueue_batch(const Batch& B, const mvdr_solver_config& c, hipStream_t stream, bool index, uint32_t first_iter,
+                   uint32_t iters) {
     const bool fp64 = c.precision == MVDR_PRECISION_MIXED;
     if (index) {
-        launch_copy_se3(s->b, s->b.se3[0], stream);
-        launch_index_build(s->b, s->cap_tiles, c.max_nodes, s->cap_chunks, c.use_foreground != 0,
-                           c.use_cross_view != 0, stream);
+        launch_copy_se3(B, stream);
+        launch_index_build(B, c.use_foreground != 0, c.use_cross_view != 0, stream);
     }
     for (uint32_t k = first_iter; k < first_iter + iters; ++k) {
         const Penalty pen = kernel_penalty(c, k);
-        const DQ* cur = s->b.se3[k & 1];
-        DQ* next = s->b.se3[(k + 1) & 1];
-        launch_eval_terms(s->b, cur, fp64, stream);
-        launch_assemble(s->b, pen, fp64, c.max_nodes, stream);
+        launch_eval_terms(B, k & 1, fp64, stream);
+        launch_assemble(B, pen, fp64, stream);
         if (fp64) {
-            launch_cg_solve(s->b, c.pcg_iterations, c.refine_steps, c.max_nodes, cur, next, stream);
+            launch_cg_solve(B, c.pcg_iterations, c.refine_steps, k, stream);
         } else {
-            launch_pcg(s->b, c.pcg_iterations, 0, false, c.max_nodes, stream);
-            launch_apply_update(s->b, cur, next, stream);
+            launch_pcg(B, c.pcg_iterations, 0, false, stream);
+            launch_apply_update(B, k, stream);
         }
     }
+}
+
+void enqueue_solve(mvdr_solver* s, hipStream_t stream, bool index, uint32_t first_iter, uint32_t iters) {
+    enqueue_batch(s->self, s->cfg, stream, index, first_iter, iters);
 }
 
 // Node-centred <-> reference coordinates (see to_node_frame in kernels_solve.hip):
@@ -351,6 +373,14 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
         return MVDR_E_HIP;
     }
     b.frame = s->frame_dev;
+    if (e == hipSuccess) e = alloc(s, s->b_dev, 1);
+    if (e == hipSuccess) e = hipMemcpy(s->b_dev, &b, sizeof(Buffers), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "mvdr_solver_create: %s\n", hipGetErrorString(e));
+        delete s;
+        return MVDR_E_HIP;
+    }
+    s->self = Batch{s->b_dev, 1, s->cap_tiles, c.max_nodes, s->cap_chunks, b.cap_pairs};
     *out = s;
     return MVDR_OK;
 }
@@ -509,10 +539,10 @@ mvdr_status mvdr_solver_total_residual(mvdr_solver* s, void* stream, double* cos
     if (!s->indexed) return fail(s, MVDR_E_STATE, "total_residual before build_index");
     hipStream_t st = static_cast<hipStream_t>(stream);
     const Penalty pen = penalty_for(s->cfg, s->iter);
-    launch_eval_terms(s->b, s->b.se3[s->iter & 1], s->cfg.precision == MVDR_PRECISION_MIXED, st);
+    launch_eval_terms(s->self, s->iter & 1, s->cfg.precision == MVDR_PRECISION_MIXED, st);
     Penalty kp = pen;
     if (!s->cfg.density_in_jtj) kp.density = 0.f;
-    const uint32_t parts = launch_cost(s->b, kp, s->cfg.density_in_jtj != 0, st);
+    const uint32_t parts = launch_cost(s->self, kp, s->cfg.density_in_jtj != 0, st);
     HIP_TRY(s, hipGetLastError());
     HIP_TRY(s, hipEventRecord(s->done, st));
     HIP_TRY(s, hipStreamSynchronize(st));
@@ -608,48 +638,163 @@ mvdr_status mvdr_solver_debug_get(mvdr_solver* s, int tap, void* dst, size_t byt
     return MVDR_OK;
 }
 
+// Stage micro-benchmark over a batch (a solver is a batch of one): `reps`
+// launches of one stage on the data of GN iteration k, timed with hipEvents
+// on `st`.
+static mvdr_status bench_stage_impl(const Batch& B, const mvdr_solver_config& c, uint32_t k, int stage, uint32_t reps,
+                                    hipStream_t st, float* avg_ms, std::string& err) {
+    const bool fp64 = c.precision == MVDR_PRECISION_MIXED;
+    if (stage < MVDR_STAGE_EVAL_TERMS || stage > MVDR_STAGE_CG_ITERATION) {
+        err = "unknown stage";
+        return MVDR_E_INVALID;
+    }
+    if (stage == MVDR_STAGE_CG_ITERATION && !fp64) {
+        err = "the fused CG iteration runs in the mixed precision mode";
+        return MVDR_E_INVALID;
+    }
+    const Penalty pen = kernel_penalty(c, k);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) {
+        switch (stage) {
+            case MVDR_STAGE_EVAL_TERMS: launch_eval_terms(B, k & 1, fp64, st); break;
+            case MVDR_STAGE_ASSEMBLE: launch_assemble(B, pen, fp64, st); break;
+            case MVDR_STAGE_PCG_UPDATE: launch_pcg_update_once(B, fp64, st); break;
+            case MVDR_STAGE_SPMV: launch_spmv_once(B, fp64, st); break;
+            case MVDR_STAGE_INDEX: enqueue_batch(B, c, st, true, 0, 0); break;
+            default: launch_cg_iter_once(B, st); break;
+        }
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) {
+        err = std::string("HIP: ") + hipGetErrorString(e);
+        return MVDR_E_HIP;
+    }
+    *avg_ms = ms / (float)reps;
+    return MVDR_OK;
+}
+
 mvdr_status mvdr_solver_bench_stage(mvdr_solver* s, int stage, uint32_t reps, void* stream, float* avg_ms) {
     if (!s || !avg_ms || reps == 0) return MVDR_E_INVALID;
     if (!s->indexed || s->iter == 0) return fail(s, MVDR_E_STATE, "bench_stage needs a completed solve");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const mvdr_solver_config& c = s->cfg;
-    const bool fp64 = c.precision == MVDR_PRECISION_MIXED;
-    const uint32_t k = s->iter - 1;
-    const Penalty pen = kernel_penalty(c, k);
-    hipEvent_t e0, e1;
-    HIP_TRY(s, hipEventCreate(&e0));
-    HIP_TRY(s, hipEventCreate(&e1));
-    HIP_TRY(s, hipEventRecord(e0, st));
-    for (uint32_t r = 0; r < reps; ++r) {
-        switch (stage) {
-            case MVDR_STAGE_EVAL_TERMS: launch_eval_terms(s->b, s->b.se3[k & 1], s->cfg.precision == MVDR_PRECISION_MIXED, st); break;
-            case MVDR_STAGE_ASSEMBLE: launch_assemble(s->b, pen, fp64, c.max_nodes, st); break;
-            case MVDR_STAGE_PCG_UPDATE: launch_pcg_update_once(s->b, fp64, st); break;
-            case MVDR_STAGE_SPMV: launch_spmv_once(s->b, fp64, c.max_nodes, st); break;
-            case MVDR_STAGE_INDEX: enqueue_solve(s, st, true, 0, 0); break;
-            case MVDR_STAGE_CG_ITERATION:
-                if (!fp64) {
-                    (void)hipEventDestroy(e0);
-                    (void)hipEventDestroy(e1);
-                    return fail(s, MVDR_E_INVALID, "the fused CG iteration runs in the mixed precision mode");
-                }
-                launch_cg_iter_once(s->b, c.max_nodes, st);
-                break;
-            default:
-                (void)hipEventDestroy(e0);
-                (void)hipEventDestroy(e1);
-                return fail(s, MVDR_E_INVALID, "unknown stage");
-        }
+    const mvdr_status rc = bench_stage_impl(s->self, s->cfg, s->iter - 1, stage, reps, st, avg_ms, s->error);
+    if (rc == MVDR_OK) HIP_TRY(s, hipEventRecord(s->done, st));
+    return rc;
+}
+
+// ------------------------------------------------------------------ batches
+// Members must agree on everything that shapes the launch sequence; their
+// capacities may differ (grids take the largest, kernels their own sizes).
+static bool same_schedule(const mvdr_solver_config& a, const mvdr_solver_config& b) {
+    return a.gn_iterations == b.gn_iterations && a.pcg_iterations == b.pcg_iterations &&
+           a.global_iterations == b.global_iterations && a.use_elastic_penalty == b.use_elastic_penalty &&
+           a.use_density == b.use_density && a.use_foreground == b.use_foreground &&
+           a.use_cross_view == b.use_cross_view && a.density_in_jtj == b.density_in_jtj &&
+           a.precision == b.precision && a.jtj_mode == b.jtj_mode && a.refine_steps == b.refine_steps;
+}
+
+mvdr_status mvdr_batch_create(mvdr_solver* const* solvers, uint32_t count, mvdr_batch** out) {
+    if (!out) return MVDR_E_INVALID;
+    *out = nullptr;
+    if (!solvers || count == 0 || count > 65535) return MVDR_E_INVALID;
+    for (uint32_t k = 0; k < count; ++k) {
+        if (!solvers[k]) return MVDR_E_INVALID;
+        if (!same_schedule(solvers[k]->cfg, solvers[0]->cfg)) return MVDR_E_SHAPE;
+        for (uint32_t q = 0; q < k; ++q)
+            if (solvers[q] == solvers[k]) return MVDR_E_INVALID;
     }
-    HIP_TRY(s, hipEventRecord(e1, st));
-    HIP_TRY(s, hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIP_TRY(s, hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    HIP_TRY(s, hipGetLastError());
-    HIP_TRY(s, hipEventRecord(s->done, st));
-    *avg_ms = ms / (float)reps;
+    auto* bt = new mvdr_batch();
+    bt->members.assign(solvers, solvers + count);
+    bt->cfg = solvers[0]->cfg;
+    std::vector<Buffers> host(count);
+    Batch B{nullptr, count, 0, 0, 0, 0};
+    for (uint32_t k = 0; k < count; ++k) {
+        const mvdr_solver* m = solvers[k];
+        host[k] = m->b;
+        B.cap_tiles = std::max(B.cap_tiles, m->self.cap_tiles);
+        B.cap_nodes = std::max(B.cap_nodes, m->self.cap_nodes);
+        B.cap_chunks = std::max(B.cap_chunks, m->self.cap_chunks);
+        B.cap_pairs = std::max(B.cap_pairs, m->self.cap_pairs);
+    }
+    hipError_t e = hipMalloc(&bt->dev, sizeof(Buffers) * count);
+    if (e == hipSuccess) e = hipMemcpy(bt->dev, host.data(), sizeof(Buffers) * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "mvdr_batch_create: %s\n", hipGetErrorString(e));
+        delete bt;
+        return MVDR_E_HIP;
+    }
+    B.dev = bt->dev;
+    bt->batch = B;
+    *out = bt;
+    return MVDR_OK;
+}
+
+void mvdr_batch_destroy(mvdr_batch* bt) { delete bt; }
+
+mvdr_status mvdr_batch_bench_stage(mvdr_batch* bt, int stage, uint32_t reps, void* stream, float* avg_ms) {
+    if (!bt || !avg_ms || reps == 0) return MVDR_E_INVALID;
+    const mvdr_solver* m0 = bt->members[0];
+    for (const mvdr_solver* m : bt->members)
+        if (!m->indexed || m->iter != m0->iter || m->iter == 0) {
+            bt->error = "bench_stage needs a completed batch solve";
+            return MVDR_E_STATE;
+        }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const mvdr_status rc = bench_stage_impl(bt->batch, bt->cfg, m0->iter - 1, stage, reps, st, avg_ms, bt->error);
+    if (rc == MVDR_OK)
+        for (mvdr_solver* m : bt->members) (void)hipEventRecord(m->done, st);
+    return rc;
+}
+
+const char* mvdr_batch_last_error(const mvdr_batch* bt) { return bt ? bt->error.c_str() : "null batch"; }
+
+mvdr_status mvdr_batch_solve(mvdr_batch* bt, void* stream) {
+    if (!bt) return MVDR_E_INVALID;
+    for (mvdr_solver* m : bt->members)
+        if (!m->has_input) {
+            bt->error = "a member solver has no input";
+            return MVDR_E_STATE;
+        }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const uint32_t gn = bt->cfg.gn_iterations;
+    hipError_t e = hipSuccess;
+    if (bt->cfg.use_graph) {
+        if (!bt->graph_exec) {
+            hipStream_t cap = nullptr;
+            e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                enqueue_batch(bt->batch, bt->cfg, cap, true, 0, gn);
+                e = hipStreamEndCapture(cap, &bt->graph);
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&bt->graph_exec, bt->graph, nullptr, nullptr, 0);
+            if (cap) (void)hipStreamDestroy(cap);
+        }
+        if (e == hipSuccess) e = hipGraphLaunch(bt->graph_exec, st);
+    } else {
+        enqueue_batch(bt->batch, bt->cfg, st, true, 0, gn);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(bt->done, st);
+    for (mvdr_solver* m : bt->members) {
+        if (e == hipSuccess) e = hipEventRecord(m->done, st);
+        m->indexed = true;
+        m->iter = gn;
+    }
+    if (e != hipSuccess) {
+        bt->error = std::string("HIP: ") + hipGetErrorString(e);
+        return MVDR_E_HIP;
+    }
     return MVDR_OK;
 }
 

@@ -63,6 +63,13 @@ def lib():
         L.mvdr_solver_debug_get.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
         L.mvdr_solver_default_config.argtypes = [C.POINTER(_abi.SolverConfig)]
         L.mvdr_status_string.restype = C.c_char_p
+        L.mvdr_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_void_p)]
+        L.mvdr_batch_destroy.argtypes = [C.c_void_p]
+        L.mvdr_batch_last_error.argtypes = [C.c_void_p]
+        L.mvdr_batch_last_error.restype = C.c_char_p
+        L.mvdr_batch_solve.argtypes = [C.c_void_p, C.c_void_p]
+        L.mvdr_batch_bench_stage.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_void_p, C.POINTER(C.c_float)]
+        L.mvdr_solver_bench_stage.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_void_p, C.POINTER(C.c_float)]
         _LIB = L
     return _LIB
 
@@ -182,3 +189,39 @@ def decode_p2p_terms(raw):
     f = raw.view(np.float32)
     knn = np.stack([raw[:, 0] & 0xFFFF, raw[:, 0] >> 16, raw[:, 1] & 0xFFFF, raw[:, 1] >> 16], 1).astype(np.uint16)
     return {"knn": knn, "w": f[:, 4:8].copy(), "target": f[:, 8:11].copy(), "warped": f[:, 12:15].copy()}
+
+
+class NonRigidBatch:
+    """Solves the current frames of several NonRigidSolver instances with one
+    launch per stage (no counterpart in the reference, which solves one frame at
+    a time).  The members must share their iteration schedule and precision and
+    must outlive the batch; each member reads as after its own solve."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
+        self.h = C.c_void_p()
+        rc = lib().mvdr_batch_create(arr, len(self.solvers), C.byref(self.h))
+        if rc != 0:
+            raise SolverError(rc, "mvdr_batch_create failed")
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().mvdr_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise SolverError(rc, lib().mvdr_batch_last_error(self.h).decode())
+
+    def SolveNonRigidAlignment(self, stream=None):
+        self._check(lib().mvdr_batch_solve(self.h, _stream_handle(stream)))
+
+    def bench_stage(self, stage, reps, stream=None):
+        """Mean ms per launch of one stage over all members (hipEvents on the stream)."""
+        out = C.c_float()
+        self._check(lib().mvdr_batch_bench_stage(self.h, stage, reps, _stream_handle(stream), C.byref(out)))
+        return out.value
