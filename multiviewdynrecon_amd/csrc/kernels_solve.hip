@@ -336,39 +336,54 @@ __device__ inline int term_rows(const Buffers& b, const Counts& c, const Penalty
 constexpr uint32_t kDiagVals = 27;  // 21 upper-triangle entries + 6 of J^T r
 constexpr uint32_t kPairVals = 36;
 
-// Inclusive segmented scan across the wave by DPP moves (row shifts 1, 2, 4, 8
-// inside each 16-lane row, then the row_bcast15 / row_bcast31 steps across
-// rows); a lane adds what it receives only when it comes from a lane with the
-// same key.  Runs of a key are contiguous, so equal keys mean the same run.
-template <int CTRL, int ROW_MASK>
-__device__ inline uint32_t dpp_u32(uint32_t v, uint32_t old) {
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROW_MASK, 0xF, false);
-}
-template <int CTRL, int ROW_MASK, int V>
-__device__ inline void seg_step(float (&v)[V], uint32_t key) {
-    const bool same = dpp_u32<CTRL, ROW_MASK>(key, 0xFFFFFFFEu) == key;
+// Run sums of a chunk: lane e holds the contribution v of entry e (key: its
+// node or pair; runs of a key are contiguous).  The wave stores v in LDS, then
+// lane q adds component q down each run in entry order -- a fixed order -- and
+// emit(key, first, last, sum) is called once per run by every lane (lanes
+// q >= V carry no sum).  Column reads use an odd row stride: no bank conflicts.
+constexpr uint32_t kRedStride = 37;
+
+template <int V, typename Emit>
+__device__ inline void chunk_runs(const float (&v)[V], uint32_t key, bool valid, uint32_t lane, float* lds, Emit emit) {
 #pragma unroll
-    for (int q = 0; q < V; ++q) {
-        const float o = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v[q]), CTRL, ROW_MASK, 0xF, false));
-        v[q] += same ? o : 0.f;
+    for (int q = 0; q < V; ++q) lds[lane * kRedStride + q] = v[q];
+    const uint32_t key_next = (uint32_t)__shfl_down((int)key, 1, 64);
+    uint64_t ends = __ballot(valid && (lane == 63 || key_next != key));
+    wave_lds_sync();
+    uint32_t first = 0;
+    while (ends) {
+        const uint32_t last = (uint32_t)__builtin_ctzll(ends);
+        ends &= ends - 1;
+        const uint32_t rkey = (uint32_t)__shfl((int)key, (int)last, 64);
+        float acc = 0.f;
+        if (lane < (uint32_t)V)
+            for (uint32_t e = first; e <= last; ++e) acc += lds[e * kRedStride + lane];
+        emit(rkey, first, last, acc);
+        first = last + 1;
     }
-}
-template <int V>
-__device__ inline void segmented_scan(float (&v)[V], uint32_t key) {
-    seg_step<0x111, 0xF>(v, key);  // row_shr:1
-    seg_step<0x112, 0xF>(v, key);  // row_shr:2
-    seg_step<0x114, 0xF>(v, key);  // row_shr:4
-    seg_step<0x118, 0xF>(v, key);  // row_shr:8
-    seg_step<0x142, 0xA>(v, key);  // row_bcast:15 into rows 1, 3
-    seg_step<0x143, 0xC>(v, key);  // row_bcast:31 into rows 2, 3
+    wave_lds_sync();  // the wave's LDS rows are rewritten by its next chunk
 }
 
-__device__ inline void diag_chunk(const Buffers& b, const Counts& c, const Penalty& pen, uint32_t ch, uint32_t lane) {
+// Position (a, k), a <= k, of the q-th entry of a row-major upper triangle of 6x6.
+__device__ inline void triu_index(uint32_t q, uint32_t& a, uint32_t& k) {
+    a = 0;
+    uint32_t rest = q;
+    while (rest >= 6 - a) {
+        rest -= 6 - a;
+        ++a;
+    }
+    k = a + rest;
+}
+
+__device__ inline void diag_chunk(const Buffers& b, const Counts& c, const Penalty& pen, uint32_t ch, uint32_t lane,
+                                  float* lds) {
     const FrameDev& f = *b.frame;
     const uint32_t total = c.kv1;
     const uint32_t e = ch * 64 + lane;
     const bool valid = e < total;
     const uint32_t key = valid ? b.n2t_node[e] : 0xFFFFFFFFu;
+    const uint32_t prev_key = ch > 0 ? b.n2t_node[ch * 64 - 1] : 0xFFFFFFFFu;
+    const uint32_t next_key = ch * 64 + 64 < total ? b.n2t_node[ch * 64 + 64] : 0xFFFFFFFFu;
     float v[kDiagVals];
 #pragma unroll
     for (int q = 0; q < (int)kDiagVals; ++q) v[q] = 0.f;
@@ -390,40 +405,33 @@ __device__ inline void diag_chunk(const Buffers& b, const Counts& c, const Penal
             }
         }
     }
-    segmented_scan(v, key);
-    const uint32_t key_next = (uint32_t)__shfl_down((int)key, 1, 64);
-    const uint32_t first_key = (uint32_t)__shfl((int)key, 0, 64);
-    const bool last = valid && (lane == 63 || key_next != key);
-    if (!last) return;
-    const bool has_lane0 = key == first_key;
-    const bool open_head = has_lane0 && ch > 0 && b.n2t_node[ch * 64 - 1] == key;
-    const bool open_tail = lane == 63 && e + 1 < total && b.n2t_node[e + 1] == key;
-    if (open_head || open_tail) {
-        float* out = b.part_diag + ((size_t)ch * 2 + (has_lane0 ? 0 : 1)) * kDiagVals;
-#pragma unroll
-        for (int q = 0; q < (int)kDiagVals; ++q) out[q] = v[q];
-        return;
-    }
-    float* D = b.diag + 36 * (size_t)key;
-    int o = 0;
-#pragma unroll
-    for (int a = 0; a < 6; ++a)
-#pragma unroll
-        for (int k2 = a; k2 < 6; ++k2) {
-            D[6 * a + k2] = v[o];
-            D[6 * k2 + a] = v[o];
-            ++o;
+    uint32_t ta = 0, tk = 0;
+    if (lane < 21) triu_index(lane, ta, tk);
+    chunk_runs(v, key, valid, lane, lds, [&](uint32_t rkey, uint32_t first, uint32_t last, float sum) {
+        if (lane >= kDiagVals) return;
+        const bool open_head = first == 0 && prev_key == rkey;
+        const bool open_tail = last == 63 && next_key == rkey;
+        if (open_head || open_tail) {
+            b.part_diag[((size_t)ch * 2 + (first == 0 ? 0 : 1)) * kDiagVals + lane] = sum;
+        } else if (lane < 21) {
+            float* D = b.diag + 36 * (size_t)rkey;
+            D[6 * ta + tk] = sum;
+            D[6 * tk + ta] = sum;
+        } else {
+            b.jtr[6 * (size_t)rkey + lane - 21] = sum;
         }
-#pragma unroll
-    for (int a = 0; a < 6; ++a) b.jtr[6 * (size_t)key + a] = v[21 + a];
+    });
 }
 
-__device__ inline void pair_chunk(const Buffers& b, const Counts& c, const Penalty& pen, uint32_t ch, uint32_t lane) {
+__device__ inline void pair_chunk(const Buffers& b, const Counts& c, const Penalty& pen, uint32_t ch, uint32_t lane,
+                                  float* lds) {
     const FrameDev& f = *b.frame;
     const uint32_t total = c.kv2;
     const uint32_t e = ch * 64 + lane;
     const bool valid = e < total;
     const uint32_t key = valid ? b.kv2_pair[e] : 0xFFFFFFFFu;
+    const uint32_t prev_key = ch > 0 ? b.kv2_pair[ch * 64 - 1] : 0xFFFFFFFFu;
+    const uint32_t next_key = ch * 64 + 64 < total ? b.kv2_pair[ch * 64 + 64] : 0xFFFFFFFFu;
     float v[kPairVals];
 #pragma unroll
     for (int q = 0; q < (int)kPairVals; ++q) v[q] = 0.f;
@@ -447,18 +455,13 @@ __device__ inline void pair_chunk(const Buffers& b, const Counts& c, const Penal
             }
         }
     }
-    segmented_scan(v, key);
-    const uint32_t key_next = (uint32_t)__shfl_down((int)key, 1, 64);
-    const uint32_t first_key = (uint32_t)__shfl((int)key, 0, 64);
-    const bool last = valid && (lane == 63 || key_next != key);
-    if (!last) return;
-    const bool has_lane0 = key == first_key;
-    const bool open_head = has_lane0 && ch > 0 && b.kv2_pair[ch * 64 - 1] == key;
-    const bool open_tail = lane == 63 && e + 1 < total && b.kv2_pair[e + 1] == key;
-    float* out = (open_head || open_tail) ? b.part_pair + ((size_t)ch * 2 + (has_lane0 ? 0 : 1)) * kPairVals
-                                          : b.offdiag + 36 * (size_t)key;
-#pragma unroll
-    for (int q = 0; q < (int)kPairVals; ++q) out[q] = v[q];
+    chunk_runs(v, key, valid, lane, lds, [&](uint32_t rkey, uint32_t first, uint32_t last, float sum) {
+        if (lane >= kPairVals) return;
+        const bool open = (first == 0 && prev_key == rkey) || (last == 63 && next_key == rkey);
+        float* out = open ? b.part_pair + ((size_t)ch * 2 + (first == 0 ? 0 : 1)) * kPairVals
+                          : b.offdiag + 36 * (size_t)rkey;
+        out[lane] = sum;
+    });
 }
 
 __global__ void __launch_bounds__(256) assemble_chunks_kernel(const Buffers* __restrict__ bs, Penalty pen) {
@@ -466,11 +469,13 @@ __global__ void __launch_bounds__(256) assemble_chunks_kernel(const Buffers* __r
     const Counts& c = *b.counts;
     if (c.error) return;
     const uint32_t lane = threadIdx.x & 63;
+    __shared__ float red[4 * 64 * kRedStride];
+    float* lds = red + (threadIdx.x >> 6) * 64 * kRedStride;
     const uint32_t c1 = (c.kv1 + 63) / 64, c2 = (c.kv2 + 63) / 64;
     const uint32_t waves = gridDim.x * (blockDim.x >> 6);
     for (uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < c1 + c2; w += waves) {
-        if (w < c1) diag_chunk(b, c, pen, w, lane);
-        else pair_chunk(b, c, pen, w - c1, lane);
+        if (w < c1) diag_chunk(b, c, pen, w, lane, lds);
+        else pair_chunk(b, c, pen, w - c1, lane, lds);
     }
 }
 
