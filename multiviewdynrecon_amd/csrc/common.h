@@ -348,6 +348,19 @@ struct Buffers {
     DQ* se3[2];
 };
 
+// The Buffers of this workgroup's frame (bs[blockIdx.y]).  Read through the
+// constant address space: pointers loaded from constant memory are known to
+// point to global memory, so the accesses through them compile to global
+// loads / stores with scalar bases instead of flat ones.  (The host pass of
+// the compiler only parses device code; it takes the plain read.)
+__device__ inline Buffers frame_buffers(const Buffers* bs) {
+#if __HIP_DEVICE_COMPILE__
+    return *((const __attribute__((address_space(4))) Buffers*)bs + blockIdx.y);
+#else
+    return bs[blockIdx.y];
+#endif
+}
+
 // One launch over `frames` solver instances: the device array of their
 // Buffers (kernels read bs[blockIdx.y]) and the largest capacities among them,
 // which size the grids; every kernel bounds its work by its own frame's sizes.

@@ -19,6 +19,28 @@ __device__ inline void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// XCD-aware work mapping.  Workgroups are dispatched round-robin over the 8
+// XCDs (linear workgroup id mod 8), each with its own L2.  For a grid-stride
+// loop over `used` work items with `grid` workgroups (a multiple of 8), the
+// v-th virtual item (v = blockIdx.x + s * grid) maps to a work item such that
+// XCD x handles a contiguous share of every sweep of `grid` items: neighbouring
+// rows / chunks, and the data they share, stay in one L2.  Returns 0xFFFFFFFF
+// for a virtual item without work.
+constexpr uint32_t kXcds = 8;
+
+__device__ inline uint32_t xcd_remap(uint32_t v, uint32_t grid, uint32_t used) {
+    const uint32_t base = (v / grid) * grid, w = v % grid;
+    if (base >= used) return 0xFFFFFFFFu;
+    const uint32_t n = min(grid, used - base);
+    const uint32_t per = (n + kXcds - 1) / kXcds;
+    const uint32_t x = w % kXcds, k = w / kXcds;
+    const uint32_t g = x * per + k;
+    return (k < per && g < n) ? base + g : 0xFFFFFFFFu;
+}
+
+// Number of virtual items a grid-stride loop over `used` items visits.
+__device__ inline uint32_t xcd_span(uint32_t grid, uint32_t used) { return ((used + grid - 1) / grid) * grid; }
+
 template <typename T>
 __device__ inline T wave_sum(T v) {
 #pragma unroll

@@ -76,7 +76,7 @@ struct ForegroundPred {
 // counters the rest of the index build accumulates into.
 template <typename Pred>
 __global__ void __launch_bounds__(kTileThreads) compact_count_kernel(const Buffers* __restrict__ bs, Pred pred, int clear) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const uint32_t M = f.V * f.W * f.H;
     if (clear) {
@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(kTileThreads) compact_count_kernel(const Buffe
 
 // Pass 2: exclusive scan of the tile counts (one block).
 __global__ void __launch_bounds__(1024) compact_scan_kernel(const Buffers* __restrict__ bs, int which) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const uint32_t M = f.V * f.W * f.H;
     const uint32_t tiles = (M + kTilePixels - 1) / kTilePixels;
@@ -127,7 +127,7 @@ __global__ void __launch_bounds__(1024) compact_scan_kernel(const Buffers* __res
 // reference's inclusive-scan compaction (ImageTermKNNFetcher.cu:33-68).
 template <typename Pred>
 __global__ void __launch_bounds__(kTileThreads) compact_scatter_kernel(const Buffers* __restrict__ bs, Pred pred, int which) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const uint32_t M = f.V * f.W * f.H;
     const uint32_t base = blockIdx.x * kTilePixels;
@@ -300,7 +300,7 @@ __device__ inline bool choose_cross(const FrameDev& f, uint32_t k, P2PRec& rec, 
 // Block 0 compacts feature pairs, block 1 cross-view pairs, each in candidate
 // order with a running block-wide prefix.
 __global__ void __launch_bounds__(1024) select_pairs_kernel(const Buffers* __restrict__ bs, int use_cross) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const bool is_cross = blockIdx.x == 1;
     if (is_cross && !use_cross) {
@@ -393,7 +393,7 @@ uint32_t hist_tile(uint32_t cap_nodes) { return std::min<uint32_t>(cap_nodes, kH
 __device__ inline uint32_t num_chunks(const Counts& c) { return (c.off[4] + kChunkTerms - 1) / kChunkTerms; }
 
 __global__ void __launch_bounds__(64) n2t_hist_kernel(const Buffers* __restrict__ bs, uint32_t tile) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     Counts& c = *b.counts;
     uint32_t off[5];
@@ -437,7 +437,7 @@ __global__ void __launch_bounds__(64) n2t_hist_kernel(const Buffers* __restrict_
 constexpr uint32_t kScanWaves = 16;
 
 __global__ void __launch_bounds__(64 * kScanWaves) n2t_colscan_kernel(const Buffers* __restrict__ bs) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const uint32_t N = b.frame->N;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t n = blockIdx.x * 64 + lane;
@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(64 * kScanWaves) n2t_colscan_kernel(const Buff
 }
 
 __global__ void __launch_bounds__(64) n2t_place_kernel(const Buffers* __restrict__ bs, uint32_t tile) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
@@ -537,7 +537,7 @@ __global__ void __launch_bounds__(64) n2t_place_kernel(const Buffers* __restrict
 // which = 1: pairs (i, j > i) per row (block 0), lower entries per row
 // (block 1), pair-term entries per row (block 2).
 __global__ void __launch_bounds__(1024) node_scan_kernel(const Buffers* __restrict__ bs, int which) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const uint32_t N = b.frame->N;
     Counts& c = *b.counts;
     const int job = which == 0 ? 0 : 1 + (int)blockIdx.x;
@@ -601,12 +601,14 @@ constexpr uint32_t kRowWaves = 4;
 
 template <bool kWrite>
 __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers* __restrict__ bs) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     Counts& c = *b.counts;
     if (c.error) return;
-    const uint32_t i = blockIdx.x;
-    if (i >= f.N) return;
+    // XCD-aware row order: an XCD takes a contiguous range of rows, whose term
+    // records overlap, into its L2
+    const uint32_t i = xcd_remap(blockIdx.x, gridDim.x, f.N);
+    if (i == 0xFFFFFFFFu) return;
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const uint64_t below = (1ull << lane) - 1ull;
     __shared__ uint32_t set[kRowHash];
@@ -774,7 +776,7 @@ __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers
 // Pair index of every lower entry (k, .) of row j: the position of (k, j) in
 // row k's ascending upper list.
 __global__ void __launch_bounds__(256) pair_link_kernel(const Buffers* __restrict__ bs) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const Counts& c = *b.counts;
     if (c.error) return;
     const uint32_t N = b.frame->N;
@@ -816,9 +818,10 @@ void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, hip
     hipLaunchKernelGGL(n2t_colscan_kernel, dim3((B.cap_nodes + 63) / 64, F), dim3(64 * kScanWaves), 0, stream, B.dev);
     hipLaunchKernelGGL(node_scan_kernel, dim3(1, F), dim3(1024), 0, stream, B.dev, 0);
     hipLaunchKernelGGL(n2t_place_kernel, dim3(B.cap_chunks, F), dim3(64), tile * sizeof(uint32_t), stream, B.dev, tile);
-    hipLaunchKernelGGL(pair_rows_kernel<false>, dim3(B.cap_nodes, F), dim3(64 * kRowWaves), 0, stream, B.dev);
+    const uint32_t row_grid = (B.cap_nodes + kXcds - 1) / kXcds * kXcds;
+    hipLaunchKernelGGL(pair_rows_kernel<false>, dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
     hipLaunchKernelGGL(node_scan_kernel, dim3(3, F), dim3(1024), 0, stream, B.dev, 1);
-    hipLaunchKernelGGL(pair_rows_kernel<true>, dim3(B.cap_nodes, F), dim3(64 * kRowWaves), 0, stream, B.dev);
+    hipLaunchKernelGGL(pair_rows_kernel<true>, dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
     hipLaunchKernelGGL(pair_link_kernel, dim3((B.cap_nodes + 255) / 256, F), dim3(256), 0, stream, B.dev);
 }
 

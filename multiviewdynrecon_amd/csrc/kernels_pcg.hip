@@ -158,7 +158,7 @@ __device__ inline void row_apply(const Buffers& b, uint32_t i, uint32_t sub, con
 // (blockPCGZeroInitKernel, BlockPCG.cuh:65-111, then cublasSdot).
 template <typename Acc>
 __global__ void __launch_bounds__(256) pcg_init_kernel(const Buffers* __restrict__ bs) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     Acc acc = 0;
     if (!b.counts->error) {
@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(256) pcg_init_kernel(const Buffers* __restrict
 // Iteration k: q = A d with d = s + beta d_prev (k > 0); partials of dot(d, q).
 template <typename Acc>
 __global__ void __launch_bounds__(256) pcg_spmv_kernel(const Buffers* __restrict__ bs, uint32_t k, uint32_t cur) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const float* __restrict__ d_prev = cur ? b.d1 : b.d0;
     float* __restrict__ d_out = cur ? b.d0 : b.d1;
     const FrameDev& f = *b.frame;
@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(256) pcg_spmv_kernel(const Buffers* __restrict
 // BlockPCG.cuh:120-161).  A zero denominator gives a zero step.
 template <typename Acc>
 __global__ void __launch_bounds__(256) pcg_update_kernel(const Buffers* __restrict__ bs, uint32_t k, uint32_t spmv_blocks, uint32_t dsel) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const float* __restrict__ d = dsel ? b.d1 : b.d0;
     const FrameDev& f = *b.frame;
     const Acc delta = reduce_partials<Acc>(slot_ptr(b, kSlotDelta + k), kVecBlocks);
@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(256) pcg_update_kernel(const Buffers* __restri
 // on the true residual: r = b - A x, z = M^-1 r, num = dot(r, z) ...
 template <typename Acc>
 __global__ void __launch_bounds__(256) refine_residual_kernel(const Buffers* __restrict__ bs) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     Acc acc = 0;
@@ -286,7 +286,7 @@ __global__ void __launch_bounds__(256) refine_residual_kernel(const Buffers* __r
 // ... den = dot(z, A z) ...
 template <typename Acc>
 __global__ void __launch_bounds__(256) refine_curvature_kernel(const Buffers* __restrict__ bs) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     Acc acc = 0;
@@ -308,7 +308,7 @@ __global__ void __launch_bounds__(256) refine_curvature_kernel(const Buffers* __
 // ... x += (num / den) z
 template <typename Acc>
 __global__ void __launch_bounds__(256) refine_step_kernel(const Buffers* __restrict__ bs, uint32_t spmv_blocks) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const Acc num = reduce_partials<Acc>(slot_ptr(b, kSlotRefNum), spmv_blocks);
     const Acc den = reduce_partials<Acc>(slot_ptr(b, kSlotRefDen), spmv_blocks);
@@ -340,7 +340,7 @@ constexpr uint32_t kSlotCgDelta = 72;  // delta_i partials: slot 72 + i
 
 // A^ blocks of the pairs and b^ of the nodes.
 __global__ void __launch_bounds__(256) cg_transform_kernel(const Buffers* __restrict__ bs) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
@@ -422,7 +422,7 @@ __device__ inline void hat_row_apply(const Buffers& b, uint32_t i, uint32_t sub,
 
 // w_0 = A^ r_0 (r_0 = b^), x^ = p_{-1} = s_{-1} = 0, partials of gamma_0, delta_0.
 __global__ void __launch_bounds__(256) cg_init_kernel(const Buffers* __restrict__ bs) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     double g = 0.0, d = 0.0;
@@ -479,19 +479,20 @@ __device__ inline void cg_scalars(const Buffers& b, uint32_t i, uint32_t blocks,
 // Launch i of the fused CG (see above).  Reads r_i, w_i, s_{i-1}; writes
 // r_{i+1}, w_{i+1}, s_i, p_i, x^_{i+1}.
 __global__ void __launch_bounds__(256) cg_iter_kernel(const Buffers* __restrict__ bs, uint32_t i) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     double alpha, beta;
     cg_scalars(b, i, gridDim.x, alpha, beta);
     double g = 0.0, d = 0.0;
     if (!b.counts->error) {
-        const float* R = b.cg_r[i & 1];
-        const float* W = b.cg_w[i & 1];
-        const float* S = b.cg_s[(i + 1) & 1];
-        float* Ro = b.cg_r[(i + 1) & 1];
-        float* Wo = b.cg_w[(i + 1) & 1];
-        float* So = b.cg_s[i & 1];
+        const bool odd = (i & 1) != 0;
+        const float* R = odd ? b.cg_r[1] : b.cg_r[0];
+        const float* W = odd ? b.cg_w[1] : b.cg_w[0];
+        const float* S = odd ? b.cg_s[0] : b.cg_s[1];
+        float* Ro = odd ? b.cg_r[0] : b.cg_r[1];
+        float* Wo = odd ? b.cg_w[0] : b.cg_w[1];
+        float* So = odd ? b.cg_s[1] : b.cg_s[0];
         // r_{i+1} of node j, rounded as it is stored
         auto next_r = [&](uint32_t j, float v[6]) {
 #pragma unroll
@@ -507,20 +508,38 @@ __global__ void __launch_bounds__(256) cg_iter_kernel(const Buffers* __restrict_
             double y[6];
             hat_row_apply(b, n < f.N ? n : 0, sub, next_r, y);
             if (n < f.N && sub == 0) {
+                // every input first (no aliasing stalls), then the updates
+                float rk[6], wk[6], sk[6], pk[6], xk[6];
+                const size_t k0 = 6 * (size_t)n;
 #pragma unroll
                 for (int a = 0; a < 6; ++a) {
-                    const size_t k = 6 * (size_t)n + a;
-                    const float s_i = (float)((double)W[k] + beta * (double)S[k]);
-                    const float r_next = (float)((double)R[k] - alpha * (double)s_i);
-                    const float p_i = (float)((double)R[k] + beta * (double)b.cg_p[k]);
+                    rk[a] = R[k0 + a];
+                    wk[a] = W[k0 + a];
+                    sk[a] = S[k0 + a];
+                    pk[a] = b.cg_p[k0 + a];
+                    xk[a] = b.cg_x[k0 + a];
+                }
+#pragma unroll
+                for (int a = 0; a < 6; ++a) {
+                    const float s_i = (float)((double)wk[a] + beta * (double)sk[a]);
+                    const float r_next = (float)((double)rk[a] - alpha * (double)s_i);
+                    const float p_i = (float)((double)rk[a] + beta * (double)pk[a]);
                     const double w_next = (double)(float)(y[a] + (double)r_next);
-                    So[k] = s_i;
-                    Ro[k] = r_next;
-                    Wo[k] = (float)w_next;
-                    b.cg_p[k] = p_i;
-                    b.cg_x[k] = (float)((double)b.cg_x[k] + alpha * (double)p_i);
+                    sk[a] = s_i;
+                    rk[a] = r_next;
+                    wk[a] = (float)w_next;
+                    pk[a] = p_i;
+                    xk[a] = (float)((double)xk[a] + alpha * (double)p_i);
                     g += (double)r_next * (double)r_next;
                     d += w_next * (double)r_next;
+                }
+#pragma unroll
+                for (int a = 0; a < 6; ++a) {
+                    So[k0 + a] = sk[a];
+                    Ro[k0 + a] = rk[a];
+                    Wo[k0 + a] = wk[a];
+                    b.cg_p[k0 + a] = pk[a];
+                    b.cg_x[k0 + a] = xk[a];
                 }
             }
         }
@@ -533,7 +552,7 @@ __global__ void __launch_bounds__(256) cg_iter_kernel(const Buffers* __restrict_
 // the true residual, as refine_* above, in the A^ coordinates):
 // r^ = b^ - A^ x^, num = (r^, r^) ...
 __global__ void __launch_bounds__(256) cg_refine_residual_kernel(const Buffers* __restrict__ bs) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     double acc = 0.0;
@@ -564,7 +583,7 @@ __global__ void __launch_bounds__(256) cg_refine_residual_kernel(const Buffers* 
 
 // ... den = (r^, A^ r^) ...
 __global__ void __launch_bounds__(256) cg_refine_curvature_kernel(const Buffers* __restrict__ bs) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     double acc = 0.0;
@@ -594,7 +613,7 @@ __global__ void __launch_bounds__(256) cg_refine_curvature_kernel(const Buffers*
 // ... x^ += (num / den) r^ (all refinement steps but the last; the last is
 // folded into the warp update).
 __global__ void __launch_bounds__(256) cg_refine_step_kernel(const Buffers* __restrict__ bs, uint32_t spmv_blocks) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const double num = reduce_partials<double>(slot_ptr(b, kSlotRefNum), spmv_blocks);
     const double den = reduce_partials<double>(slot_ptr(b, kSlotRefDen), spmv_blocks);
@@ -658,9 +677,9 @@ __device__ inline Mat34 mat34_of(DQ d) {
 // refinement step (0 without refinement).
 template <bool kHat>
 __global__ void __launch_bounds__(256) apply_update_kernel(const Buffers* __restrict__ bs, uint32_t k, uint32_t refine_blocks) {
-    const Buffers& b = bs[blockIdx.y];
-    const DQ* __restrict__ se3_in = b.se3[k & 1];
-    DQ* __restrict__ se3_out = b.se3[(k + 1) & 1];
+    const Buffers b = frame_buffers(bs);
+    const DQ* __restrict__ se3_in = (k & 1) ? b.se3[1] : b.se3[0];
+    DQ* __restrict__ se3_out = (k & 1) ? b.se3[0] : b.se3[1];
     const FrameDev& f = *b.frame;
     const uint32_t stride = gridDim.x * blockDim.x;
     double step = 0.0;
@@ -723,7 +742,7 @@ __global__ void __launch_bounds__(256) apply_update_kernel(const Buffers* __rest
 
 // Start of a frame: the warp field the GN loop reads first is the input one.
 __global__ void copy_se3_kernel(const Buffers* __restrict__ bs) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     DQ* __restrict__ dst = b.se3[0];
     const FrameDev& f = *b.frame;
     const uint32_t stride = gridDim.x * blockDim.x;

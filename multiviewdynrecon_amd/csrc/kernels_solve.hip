@@ -100,8 +100,8 @@ __device__ inline void eval_image(const FrameDev& f, const DQ* se3, ushort4 px, 
 // vertex (SparseCorrespondenceHandler.cu:193-220).
 template <typename R>
 __global__ void __launch_bounds__(256) eval_terms_kernel(const Buffers* __restrict__ bs, uint32_t which) {
-    const Buffers& b = bs[blockIdx.y];
-    const DQ* __restrict__ se3 = b.se3[which];
+    const Buffers b = frame_buffers(bs);
+    const DQ* __restrict__ se3 = which ? b.se3[1] : b.se3[0];
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
@@ -356,8 +356,18 @@ __device__ inline void chunk_runs(const float (&v)[V], uint32_t key, bool valid,
         ends &= ends - 1;
         const uint32_t rkey = (uint32_t)__shfl((int)key, (int)last, 64);
         float acc = 0.f;
-        if (lane < (uint32_t)V)
-            for (uint32_t e = first; e <= last; ++e) acc += lds[e * kRedStride + lane];
+        if (lane < (uint32_t)V) {
+            // eight reads in flight, added in entry order
+            uint32_t e = first;
+            for (; e + 8 <= last + 1; e += 8) {
+                float x[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) x[u] = lds[(e + u) * kRedStride + lane];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) acc += x[u];
+            }
+            for (; e <= last; ++e) acc += lds[e * kRedStride + lane];
+        }
         emit(rkey, first, last, acc);
         first = last + 1;
     }
@@ -465,17 +475,23 @@ __device__ inline void pair_chunk(const Buffers& b, const Counts& c, const Penal
 }
 
 __global__ void __launch_bounds__(256) assemble_chunks_kernel(const Buffers* __restrict__ bs, Penalty pen) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const Counts& c = *b.counts;
     if (c.error) return;
     const uint32_t lane = threadIdx.x & 63;
     __shared__ float red[4 * 64 * kRedStride];
     float* lds = red + (threadIdx.x >> 6) * 64 * kRedStride;
     const uint32_t c1 = (c.kv1 + 63) / 64, c2 = (c.kv2 + 63) / 64;
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-    for (uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < c1 + c2; w += waves) {
+    // groups of 4 consecutive chunks (one per wave), XCD-aware: an XCD works
+    // through a contiguous range of node rows / pairs and keeps their term
+    // records in its L2
+    const uint32_t groups = (c1 + c2 + 3) / 4;
+    for (uint32_t v = blockIdx.x; v < xcd_span(gridDim.x, groups); v += gridDim.x) {
+        const uint32_t grp = xcd_remap(v, gridDim.x, groups);
+        if (grp == 0xFFFFFFFFu) continue;
+        const uint32_t w = grp * 4 + (threadIdx.x >> 6);
         if (w < c1) diag_chunk(b, c, pen, w, lane, lds);
-        else pair_chunk(b, c, pen, w - c1, lane, lds);
+        else if (w < c1 + c2) pair_chunk(b, c, pen, w - c1, lane, lds);
     }
 }
 
@@ -485,7 +501,7 @@ __global__ void __launch_bounds__(256) assemble_chunks_kernel(const Buffers* __r
 // (blockDiagonalInverseKernel, BlockDiagonalPreconditionerInverse.cu:6-41).
 template <typename InvT>
 __global__ void __launch_bounds__(256) assemble_fixup_kernel(const Buffers* __restrict__ bs) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
@@ -564,7 +580,7 @@ __global__ void __launch_bounds__(256) assemble_fixup_kernel(const Buffers* __re
 // computeTermResidualKernel (ResidualEvaluator.cu:71-142): sum of squared
 // lambda-weighted residuals, accumulated in double.
 __global__ void __launch_bounds__(256) cost_kernel(const Buffers* __restrict__ bs, Penalty pen, int density_in_jtj) {
-    const Buffers& b = bs[blockIdx.y];
+    const Buffers b = frame_buffers(bs);
     const Counts& c = *b.counts;
     double acc = 0.0;
     const uint32_t stride = gridDim.x * blockDim.x;
