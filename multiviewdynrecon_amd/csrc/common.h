@@ -324,6 +324,8 @@ struct Buffers {
     uint2* pair_range;       // cap_pairs: [begin, end) into pair_terms
     uint32_t* pair_terms;    // kv2: NodePair2Term term lists
     uint32_t* kv2_pair;      // kv2: the pair of each entry
+    uint32_t* pair_edge_n;   // cap_pairs: graph edges joining the pair (row Gram assembly)
+    uint32_t* pair_edge;     // kPairEdges x cap_pairs: their edge indices
     // assembly partials of runs crossing 64-entry chunk boundaries
     float* part_diag;        // (kv1 / 64 + 1) x 2 x 27
     float* part_pair;        // (kv2 / 64 + 1) x 2 x 36
@@ -362,12 +364,16 @@ __device__ inline Buffers frame_buffers(const Buffers* bs) {
 }
 
 // One launch over `frames` solver instances: the device array of their
-// Buffers (kernels read bs[blockIdx.y]) and the largest capacities among them,
-// which size the grids; every kernel bounds its work by its own frame's sizes.
+// Buffers (kernels read bs[blockIdx.y]), the largest capacities among them and
+// the largest node count of their current frames (rounded up to 64), which size
+// the grids; every kernel bounds its work by its own frame's sizes.
 struct Batch {
     const Buffers* dev;
     uint32_t frames;
-    uint32_t cap_tiles, cap_nodes, cap_chunks, cap_pairs;
+    uint32_t cap_tiles, nodes, cap_chunks;
 };
+
+// Graph edges per node pair the row Gram assembly can hold.
+constexpr uint32_t kPairEdges = 4;
 
 }  // namespace mvdr

@@ -388,7 +388,7 @@ constexpr uint32_t kChunkTerms = 512;
 constexpr uint32_t kHistNodes = 16384;  // largest LDS node tile (64 KB of u32 counters)
 
 // LDS node tile of the counting-sort kernels (dynamic shared memory).
-uint32_t hist_tile(uint32_t cap_nodes) { return std::min<uint32_t>(cap_nodes, kHistNodes); }
+uint32_t hist_tile(uint32_t nodes) { return std::min<uint32_t>(nodes, kHistNodes); }
 
 __device__ inline uint32_t num_chunks(const Counts& c) { return (c.off[4] + kChunkTerms - 1) / kChunkTerms; }
 
@@ -591,15 +591,18 @@ __device__ inline uint32_t lower_bound_lds(const uint32_t* a, uint32_t n, uint32
     return lo;
 }
 
-// kWrite = false: pairs above / below the diagonal and pair-term entries of
-// row i.  kWrite = true: the row's pair keys, lower entries, and its pair-term
-// lists (NodePair2Term) with their ranges: for pair (i, j) every Node2Term
-// entry of row i once per knn slot holding j, in Node2Term order -- the
-// order of the reference's stable key sort.  kRowWaves waves per row; wave w
-// places a contiguous w-th share of the row's entries.
+// kRowCount: pairs above / below the diagonal and pair-term entries of row i.
+// kRowKeys: the row's pair keys and lower entries, and a cleared graph-edge
+// count per pair (the row Gram assembly needs no pair-term lists).
+// kRowTerms: pair keys, lower entries, and the pair-term lists (NodePair2Term)
+// with their ranges: for pair (i, j) every Node2Term entry of row i once per
+// knn slot holding j, in Node2Term order -- the order of the reference's stable
+// key sort.  kRowWaves waves per row; wave w places a contiguous w-th share of
+// the row's entries.
 constexpr uint32_t kRowWaves = 4;
+enum : int { kRowCount = 0, kRowKeys = 1, kRowTerms = 2 };
 
-template <bool kWrite>
+template <int kMode>
 __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers* __restrict__ bs) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
@@ -680,7 +683,7 @@ __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers
     }
     __syncthreads();
     const uint32_t up = nup, low = nlow;
-    if (!kWrite) {
+    if (kMode == kRowCount) {
         if (tid == 0) {
             b.row_up[i] = up;
             b.row_low[i] = low;
@@ -694,7 +697,7 @@ __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers
     const uint32_t len = end - beg, per = (len + kRowWaves - 1) / kRowWaves;
     const uint32_t wb = beg + min(len, w * per), we = beg + min(len, (w + 1) * per);
     uint32_t* run = cnt[w];
-    if (up > 0) {
+    if (kMode == kRowTerms && up > 0) {
         // pass 1: this wave's entries per pair
         for (uint32_t u = lane; u < up; u += 64) run[u] = 0;
         wave_lds_sync();
@@ -769,8 +772,42 @@ __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers
             }
         }
     }
-    for (uint32_t u = tid; u < up; u += 64 * kRowWaves) b.pair_key[up_base + u] = (i << 16) | ul[u];
+    for (uint32_t u = tid; u < up; u += 64 * kRowWaves) {
+        b.pair_key[up_base + u] = (i << 16) | ul[u];
+        if (kMode == kRowKeys) b.pair_edge_n[up_base + u] = 0;
+    }
     for (uint32_t k = tid; k < low; k += 64 * kRowWaves) b.low_ent[low_base + k] = make_uint2(ll[k], 0u);
+}
+
+// Graph edges of every pair (i, j): the row Gram assembly adds their pair
+// blocks in the fix-up, in edge order.  The first kPairEdges edges found are
+// kept; the count tells the fix-up when a pair has more (a node graph with
+// repeated edges) and it walks the row's Node2Term list instead.
+__global__ void __launch_bounds__(256) pair_edges_kernel(const Buffers* __restrict__ bs) {
+    const Buffers b = frame_buffers(bs);
+    const FrameDev& f = *b.frame;
+    Counts& c = *b.counts;
+    if (c.error) return;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < f.E; e += stride) {
+        const ushort2 g = f.graph[e];
+        if (g.x == g.y) continue;
+        const uint32_t i = min(g.x, g.y), j = max(g.x, g.y);
+        const uint32_t key = (i << 16) | j;
+        uint32_t lo = b.pair_offset[i], hi = b.pair_offset[i + 1];
+        const uint32_t end = hi;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (b.pair_key[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo == end || b.pair_key[lo] != key) {
+            atomicOr(&c.error, kErrPairs);
+            continue;
+        }
+        const uint32_t at = atomicAdd(&b.pair_edge_n[lo], 1u);
+        if (at < kPairEdges) b.pair_edge[kPairEdges * (size_t)lo + at] = e;
+    }
 }
 
 // Pair index of every lower entry (k, .) of row j: the position of (k, j) in
@@ -798,7 +835,9 @@ __global__ void __launch_bounds__(256) pair_link_kernel(const Buffers* __restric
 }
 
 // ------------------------------------------------------------- host launch
-void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, hipStream_t stream) {
+// gram: index for the row Gram assembly (pair keys and graph edges per pair);
+// otherwise the NodePair2Term lists of the reference's assembly.
+void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, bool gram, hipStream_t stream) {
     const uint32_t F = B.frames;
     hipLaunchKernelGGL(compact_count_kernel<DensePred>, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream, B.dev,
                        DensePred{}, 1);
@@ -813,16 +852,29 @@ void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, hip
                            B.dev, ForegroundPred{}, 1);
     }
     hipLaunchKernelGGL(select_pairs_kernel, dim3(2, F), dim3(1024), 0, stream, B.dev, use_cross ? 1 : 0);
-    const uint32_t tile = hist_tile(B.cap_nodes);
+    const uint32_t tile = hist_tile(B.nodes);
     hipLaunchKernelGGL(n2t_hist_kernel, dim3(B.cap_chunks, F), dim3(64), tile * sizeof(uint32_t), stream, B.dev, tile);
-    hipLaunchKernelGGL(n2t_colscan_kernel, dim3((B.cap_nodes + 63) / 64, F), dim3(64 * kScanWaves), 0, stream, B.dev);
+    hipLaunchKernelGGL(n2t_colscan_kernel, dim3((B.nodes + 63) / 64, F), dim3(64 * kScanWaves), 0, stream, B.dev);
     hipLaunchKernelGGL(node_scan_kernel, dim3(1, F), dim3(1024), 0, stream, B.dev, 0);
     hipLaunchKernelGGL(n2t_place_kernel, dim3(B.cap_chunks, F), dim3(64), tile * sizeof(uint32_t), stream, B.dev, tile);
-    const uint32_t row_grid = (B.cap_nodes + kXcds - 1) / kXcds * kXcds;
-    hipLaunchKernelGGL(pair_rows_kernel<false>, dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
+    const uint32_t row_grid = (B.nodes + kXcds - 1) / kXcds * kXcds;
+    hipLaunchKernelGGL(pair_rows_kernel<kRowCount>, dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
     hipLaunchKernelGGL(node_scan_kernel, dim3(3, F), dim3(1024), 0, stream, B.dev, 1);
-    hipLaunchKernelGGL(pair_rows_kernel<true>, dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
-    hipLaunchKernelGGL(pair_link_kernel, dim3((B.cap_nodes + 255) / 256, F), dim3(256), 0, stream, B.dev);
+    if (gram)
+        hipLaunchKernelGGL(pair_rows_kernel<kRowKeys>, dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
+    else
+        hipLaunchKernelGGL(pair_rows_kernel<kRowTerms>, dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
+    hipLaunchKernelGGL(pair_link_kernel, dim3((B.nodes + 255) / 256, F), dim3(256), 0, stream, B.dev);
+    if (gram) hipLaunchKernelGGL(pair_edges_kernel, dim3((8 * B.nodes + 255) / 256, F), dim3(256), 0, stream, B.dev);
+}
+
+// The NodePair2Term lists of an index built for the row Gram assembly (which
+// does not need them), for inspection: the pair keys and graph-edge lists stay
+// as they are.
+void launch_pair_term_lists(const Batch& B, hipStream_t stream) {
+    const uint32_t row_grid = (B.nodes + kXcds - 1) / kXcds * kXcds;
+    hipLaunchKernelGGL(pair_rows_kernel<kRowTerms>, dim3(row_grid, B.frames), dim3(64 * kRowWaves), 0, stream, B.dev);
+    hipLaunchKernelGGL(pair_link_kernel, dim3((B.nodes + 255) / 256, B.frames), dim3(256), 0, stream, B.dev);
 }
 
 uint32_t index_chunks(uint64_t max_terms) { return (uint32_t)((max_terms + kChunkTerms - 1) / kChunkTerms); }

@@ -750,13 +750,16 @@ __global__ void copy_se3_kernel(const Buffers* __restrict__ bs) {
 }
 
 // ------------------------------------------------------------- host launch
-static uint32_t spmv_grid(uint32_t cap_nodes) {
-    return std::max<uint32_t>(1, std::min<uint32_t>((cap_nodes + kRowsPerBlock - 1) / kRowsPerBlock, kPartialStride));
+// Row-group grid for `nodes` block rows: kRowsPerBlock rows per workgroup, a
+// multiple of the XCD count, at most one partial slot row.
+static uint32_t spmv_grid(uint32_t nodes) {
+    const uint32_t g = (nodes + kRowsPerBlock - 1) / kRowsPerBlock;
+    return std::max<uint32_t>(kXcds, std::min<uint32_t>((g + kXcds - 1) / kXcds * kXcds, kPartialStride));
 }
 
 template <typename Acc>
 static void pcg_impl(const Batch& B, uint32_t iters, uint32_t refine, hipStream_t stream) {
-    const uint32_t sg = spmv_grid(B.cap_nodes);
+    const uint32_t sg = spmv_grid(B.nodes);
     iters = std::min(iters, kMaxIters);
     hipLaunchKernelGGL(pcg_init_kernel<Acc>, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev);
     uint32_t cur = 0;
@@ -779,13 +782,13 @@ void launch_pcg(const Batch& B, uint32_t iters, uint32_t refine, bool fp64, hipS
 
 // One ApplyJtJ (q = A d0, no direction update) for stage timing.
 void launch_spmv_once(const Batch& B, bool fp64, hipStream_t stream) {
-    const uint32_t sg = spmv_grid(B.cap_nodes);
+    const uint32_t sg = spmv_grid(B.nodes);
     if (fp64) hipLaunchKernelGGL(pcg_spmv_kernel<double>, dim3(sg, B.frames), dim3(256), 0, stream, B.dev, 0u, 0u);
     else hipLaunchKernelGGL(pcg_spmv_kernel<float>, dim3(sg, B.frames), dim3(256), 0, stream, B.dev, 0u, 0u);
 }
 
 void launch_pcg_update_once(const Batch& B, bool fp64, hipStream_t stream) {
-    const uint32_t sg = spmv_grid(B.cap_nodes);
+    const uint32_t sg = spmv_grid(B.nodes);
     if (fp64)
         hipLaunchKernelGGL(pcg_update_kernel<double>, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev, 0u, sg, 1u);
     else
@@ -800,9 +803,9 @@ void launch_apply_update(const Batch& B, uint32_t k, hipStream_t stream) {
 // Fused CG of the mixed precision mode: transform, init, one launch per
 // iteration, refinement, warp update of GN iteration k.
 void launch_cg_solve(const Batch& B, uint32_t iters, uint32_t refine, uint32_t k, hipStream_t stream) {
-    const uint32_t sg = spmv_grid(B.cap_nodes);
+    const uint32_t sg = spmv_grid(B.nodes);
     iters = std::min(iters, kMaxIters);
-    const uint32_t tgrid = std::min<uint32_t>((B.cap_pairs + B.cap_nodes + 255) / 256, 2048u);
+    const uint32_t tgrid = std::min<uint32_t>((8 * B.nodes + 255) / 256, 2048u);  // grid-stride over P + N
     hipLaunchKernelGGL(cg_transform_kernel, dim3(tgrid, B.frames), dim3(256), 0, stream, B.dev);
     hipLaunchKernelGGL(cg_init_kernel, dim3(sg, B.frames), dim3(256), 0, stream, B.dev);
     for (uint32_t i = 0; i < iters; ++i)
@@ -818,7 +821,7 @@ void launch_cg_solve(const Batch& B, uint32_t iters, uint32_t refine, uint32_t k
 
 // One fused CG iteration (stage timing).
 void launch_cg_iter_once(const Batch& B, hipStream_t stream) {
-    hipLaunchKernelGGL(cg_iter_kernel, dim3(spmv_grid(B.cap_nodes), B.frames), dim3(256), 0, stream, B.dev, 1u);
+    hipLaunchKernelGGL(cg_iter_kernel, dim3(spmv_grid(B.nodes), B.frames), dim3(256), 0, stream, B.dev, 1u);
 }
 
 void launch_copy_se3(const Batch& B, hipStream_t stream) {

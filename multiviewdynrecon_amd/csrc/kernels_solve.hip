@@ -633,6 +633,352 @@ __global__ void __launch_bounds__(256) assemble_fixup_kernel(const Buffers* __re
     }
 }
 
+// ------------------------------------------- row Gram assembly (mixed mode)
+// For the knn-weighted terms (dense, foreground, density, feature, cross) the
+// Jacobian rows of node n are the term's rows J scaled by the node's knn slot
+// weight w_n.  In node-centred coordinates the rows of node i are w_i u_i,
+// u_i = (J_w + J_v x g_i, J_v), and those of node j are w_j u_i M_ij with
+//   M_ij: (x_w, x_v) -> (x_w + x_v x (g_j - g_i), x_v)   (a change of centre).
+// Every block of row i is therefore a weighted sum of the same per-term values
+// G_t = (lambda^2 u_i^T u_i [21 upper-triangle entries], -lambda^2 u_i^T r [6]):
+//   D_i  = sum_t w_i^2 G_t[0:21]          (PreconditionerBlocked.cu:172-284)
+//   b_i  = sum_t w_i   G_t[21:27]         (JtDotResidualIndexed.cu:116-224)
+//   S_ij = sum_t w_i w_j G_t[0:21],  B_ij = S_ij M_ij
+//                                         (JtJMaterializer.cu:270-365)
+// where w_j sums over the knn slots holding j (the reference's pair
+// multiplicity) and each Node2Term entry of i contributes once (its slot
+// multiplicity).  For row i that is the product G^T W of the row's terms (K)
+// with a weight matrix W whose columns are the diagonal (w_i^2), J^T r (w_i)
+// and the upper partners j > i (w_i w_j) -- run on the matrix cores with
+// v_mfma_f32_16x16x4_f32 (FP32 products and accumulation), four terms per
+// instruction, two 16-row tiles for the 27 values, one 16-column tile per 16
+// weight columns.  Graph-edge (smooth) terms have different rows per node: they
+// enter the diagonal and J^T r here and their pair blocks in the fix-up.
+// One wave per row; rows with more than 62 upper partners take several passes
+// of 64 weight columns.  No NodePair2Term lists are read.
+constexpr uint32_t kGramVals = 27;
+constexpr uint32_t kGramStride = 28;    // LDS floats per term
+constexpr uint32_t kGramCols = 64;      // weight columns per pass (4 tiles of 16)
+constexpr uint32_t kGramPartners = 512; // a row's upper partners (pair_rows caps a row at 512 partners)
+constexpr uint16_t kNoCol = 0xFFFF;
+
+typedef float gram_f4 __attribute__((ext_vector_type(4)));
+
+// G_t of term t for row node i; knn partners j > i and their weights w_i w_j.
+__device__ inline void gram_entry(const Buffers& b, const Counts& c, const Penalty& pen, uint32_t t, uint32_t i,
+                                  const float4 g, float G[kGramVals], float& wi, ushort4& part, float4& wq) {
+    uint32_t k;
+    const int type = type_of(c, t, k);
+    float J[3][6], res[3], lam2[3];
+    int rows = 3;
+    ushort4 knn = make_ushort4(kNoCol, kNoCol, kNoCol, kNoCol);
+    float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+    wi = 1.f;
+    if (type == kSmooth) {
+        const SmoothRec sm = b.smooth[k];
+        smooth_rows(sm, i == (sm.ij & 0xFFFF), J, res);
+        lam2[0] = lam2[1] = lam2[2] = pen.smooth * pen.smooth;
+    } else if (type == kDense || type == kForeground) {
+        const ScalarRec& rec = type == kDense ? b.dense[k] : b.fg[k];
+        knn = rec.knn;
+        w = rec.w;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) J[0][q] = rec.J[q];
+        res[0] = rec.r;
+        lam2[0] = type == kDense ? 1.0f : pen.foreground * pen.foreground;
+        rows = 1;
+        if (type == kDense && b.density && pen.density != 0.f) {
+            const ScalarRec& d = b.density[k];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) J[1][q] = d.J[q];
+            res[1] = d.r;
+            lam2[1] = pen.density * pen.density;
+            rows = 2;
+        }
+    } else {
+        const P2PRec& rec = type == kFeature ? b.feat[k] : b.cross[k];
+        knn = rec.knn;
+        w = rec.w;
+        p2p_rows(rec, J, res);
+        const float l = type == kFeature ? pen.feature : pen.cross;
+        lam2[0] = lam2[1] = lam2[2] = l * l;
+    }
+#pragma unroll
+    for (int q = 0; q < (int)kGramVals; ++q) G[q] = 0.f;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        if (r < rows) {
+            to_node_frame(J[r], g);
+            int o = 0;
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+#pragma unroll
+                for (int k2 = a; k2 < 6; ++k2) G[o++] += lam2[r] * J[r][a] * J[r][k2];
+#pragma unroll
+            for (int a = 0; a < 6; ++a) G[21 + a] -= lam2[r] * J[r][a] * res[r];
+        }
+    }
+    part = make_ushort4(kNoCol, kNoCol, kNoCol, kNoCol);
+    wq = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (type != kSmooth) {
+        wi = slot_weight(knn, w, i);
+        const uint32_t n[4] = {knn.x, knn.y, knn.z, knn.w};
+        float v[4];
+        uint16_t pp[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const bool up = n[q] > i && n[q] != kNoCol;
+            pp[q] = up ? (uint16_t)n[q] : kNoCol;
+            v[q] = up ? wi * slot_weight(knn, w, n[q]) : 0.f;
+        }
+        part = make_ushort4(pp[0], pp[1], pp[2], pp[3]);
+        wq = make_float4(v[0], v[1], v[2], v[3]);
+    }
+}
+
+__device__ inline uint32_t lower_bound_u32(const uint32_t* a, uint32_t n, uint32_t v) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Weight of column n (global: 0 = diagonal, 1 = J^T r, 2 + u = upper partner u)
+// for a term with packed partner columns cols (4 x u16) and weights.
+__device__ inline float gram_weight(uint32_t n, uint2 cols, float4 wq, float wi, float wi2) {
+    float v = n == 0 ? wi2 : (n == 1 ? wi : 0.f);
+    v += (cols.x & 0xFFFF) == n ? wq.x : 0.f;
+    v += (cols.x >> 16) == n ? wq.y : 0.f;
+    v += (cols.y & 0xFFFF) == n ? wq.z : 0.f;
+    v += (cols.y >> 16) == n ? wq.w : 0.f;
+    return v;
+}
+
+__global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict__ bs, Penalty pen) {
+    const Buffers b = frame_buffers(bs);
+    const FrameDev& f = *b.frame;
+    const Counts& c = *b.counts;
+    if (c.error) return;
+    __shared__ float Gl[64 * kGramStride];
+    __shared__ uint2 Cl[64];      // partner columns of each term
+    __shared__ float4 Wl[64];     // their weights
+    __shared__ float2 Dl[64];     // w_i^2, w_i
+    __shared__ uint32_t plist[kGramPartners];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t m = lane & 15, kq = lane >> 4;
+    for (uint32_t v = blockIdx.x; v < xcd_span(gridDim.x, f.N); v += gridDim.x) {
+        const uint32_t i = xcd_remap(v, gridDim.x, f.N);
+        if (i == 0xFFFFFFFFu) continue;
+        const uint32_t beg = b.n2t_offset[i], end = b.n2t_offset[i + 1];
+        const uint32_t ubeg = b.pair_offset[i], U = min(b.pair_offset[i + 1] - ubeg, kGramPartners);
+        const float4 g = f.nodes[i];
+        for (uint32_t u = lane; u < U; u += 64) plist[u] = b.pair_key[ubeg + u] & 0xFFFF;
+        wave_lds_sync();
+        const uint32_t cols = 2 + U;
+        for (uint32_t c0 = 0; c0 < cols; c0 += kGramCols) {
+            const uint32_t tiles = min(4u, (cols - c0 + 15) / 16);
+            gram_f4 acc[4][2];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t][0] = acc[t][1] = (gram_f4){0.f, 0.f, 0.f, 0.f};
+            for (uint32_t e0 = beg; e0 < end; e0 += 64) {
+                // phase 1: lane = Node2Term entry; G_t and weight columns into LDS
+                const uint32_t e = e0 + lane;
+                float G[kGramVals];
+                float wi = 0.f;
+                ushort4 part = make_ushort4(kNoCol, kNoCol, kNoCol, kNoCol);
+                float4 wq = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (e < end) {
+                    gram_entry(b, c, pen, b.n2t_index[e], i, g, G, wi, part, wq);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < (int)kGramVals; ++q) G[q] = 0.f;
+                }
+                uint32_t pc[4] = {part.x, part.y, part.z, part.w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (pc[q] != kNoCol) {
+                        const uint32_t r = lower_bound_u32(plist, U, pc[q]);
+                        pc[q] = (r < U && plist[r] == pc[q]) ? 2 + r : kNoCol;
+                    }
+                }
+                float* gl = Gl + lane * kGramStride;
+#pragma unroll
+                for (int q = 0; q < (int)kGramVals; ++q) gl[q] = G[q];
+                gl[kGramVals] = 0.f;
+                Cl[lane] = make_uint2(pc[0] | (pc[1] << 16), pc[2] | (pc[3] << 16));
+                Wl[lane] = wq;
+                Dl[lane] = make_float2(wi * wi, wi);
+                wave_lds_sync();
+                // phase 2: acc[t] += G^T W over 4 terms per step
+                const uint32_t steps = (min(end - e0, 64u) + 3) / 4;
+                for (uint32_t st = 0; st < steps; ++st) {
+                    const uint32_t k = 4 * st + kq;
+                    const float a0 = Gl[k * kGramStride + m];
+                    const float a1 = m < kGramVals - 16 ? Gl[k * kGramStride + 16 + m] : 0.f;
+                    const uint2 cl = Cl[k];
+                    const float4 wk = Wl[k];
+                    const float2 dk = Dl[k];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        if ((uint32_t)t < tiles) {
+                            const float bv = gram_weight(c0 + 16 * t + m, cl, wk, dk.y, dk.x);
+                            acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bv, acc[t][0], 0, 0, 0);
+                            acc[t][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bv, acc[t][1], 0, 0, 0);
+                        }
+                    }
+                }
+                wave_lds_sync();  // LDS is rewritten by the next group
+            }
+            // lane (m, kq) holds rows 4 kq + r of both value tiles for column m of each tile
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                if ((uint32_t)t >= tiles) continue;
+                const uint32_t n = c0 + 16 * t + m;
+                const gram_f4 lo = acc[t][0], hi = acc[t][1];
+                if (n == 0) {
+                    float* D = b.diag + 36 * (size_t)i;  // 32 values; the fix-up reads the first 21
+                    *reinterpret_cast<float4*>(D + 4 * kq) = make_float4(lo[0], lo[1], lo[2], lo[3]);
+                    *reinterpret_cast<float4*>(D + 16 + 4 * kq) = make_float4(hi[0], hi[1], hi[2], hi[3]);
+                } else if (n == 1) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const uint32_t q = 16 + 4 * kq + r;
+                        if (q >= 21 && q < kGramVals) b.jtr[6 * (size_t)i + q - 21] = hi[r];
+                    }
+                } else if (n < cols) {
+                    float* S = b.offdiag + 36 * (size_t)(ubeg + n - 2);  // 24 values; the fix-up reads 21
+                    *reinterpret_cast<float4*>(S + 4 * kq) = make_float4(lo[0], lo[1], lo[2], lo[3]);
+                    if (kq < 2) *reinterpret_cast<float4*>(S + 16 + 4 * kq) = make_float4(hi[0], hi[1], hi[2], hi[3]);
+                }
+            }
+        }
+        wave_lds_sync();  // plist is rewritten by the next row
+    }
+}
+
+// Fix-up of the row Gram assembly: node rows get the damping and their
+// block-Jacobi inverse with the FP64 Cholesky factor
+// (blockDiagonalInverseKernel, BlockDiagonalPreconditionerInverse.cu:6-41);
+// pair blocks get the change of centre B_ij = S_ij M_ij and the pair blocks of
+// the graph edges joining them, in edge order.
+__global__ void __launch_bounds__(256) gram_fixup_kernel(const Buffers* __restrict__ bs, Penalty pen) {
+    const Buffers b = frame_buffers(bs);
+    const FrameDev& f = *b.frame;
+    const Counts& c = *b.counts;
+    if (c.error) return;
+    const uint32_t N = f.N, P = c.pairs;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N + P; x += stride) {
+        if (x < N) {
+            const uint32_t n = x;
+            const float* Dn = b.diag + 36 * (size_t)n;
+            float v[21];
+#pragma unroll
+            for (int q = 0; q < 21; ++q) v[q] = Dn[q];
+            float D[36];
+            int o = 0;
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+#pragma unroll
+                for (int k2 = a; k2 < 6; ++k2) {
+                    D[6 * a + k2] = v[o];
+                    D[6 * k2 + a] = v[o];
+                    ++o;
+                }
+            add_node_frame_damping(D, f.nodes[n]);
+#pragma unroll
+            for (int q = 0; q < 36; ++q) b.diag[36 * (size_t)n + q] = D[q];
+            spd_inverse6<double>(D, b.diag_inv + 36 * (size_t)n, b.diag_inv64 + 36 * (size_t)n,
+                                 b.linv64 + 36 * (size_t)n);
+        } else {
+            const uint32_t p = x - N;
+            const uint32_t key = b.pair_key[p];
+            const uint32_t i = key >> 16, j = key & 0xFFFF;
+            float* Bp = b.offdiag + 36 * (size_t)p;
+            float v[21];
+#pragma unroll
+            for (int q = 0; q < 21; ++q) v[q] = Bp[q];
+            float S[36];
+            int o = 0;
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+#pragma unroll
+                for (int k2 = a; k2 < 6; ++k2) {
+                    S[6 * a + k2] = v[o];
+                    S[6 * k2 + a] = v[o];
+                    ++o;
+                }
+            const float4 gi = f.nodes[i], gj = f.nodes[j];
+            const float d[3] = {gj.x - gi.x, gj.y - gi.y, gj.z - gi.z};
+            // rows e_c x d of the change of centre (translation -> rotation part)
+            const float K[3][3] = {{0.f, -d[2], d[1]}, {d[2], 0.f, -d[0]}, {-d[1], d[0], 0.f}};
+            float B[36];
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+#pragma unroll
+                for (int k2 = 0; k2 < 6; ++k2) {
+                    float s = S[6 * a + k2];
+                    if (k2 < 3) s += S[6 * a + 3] * K[0][k2] + S[6 * a + 4] * K[1][k2] + S[6 * a + 5] * K[2][k2];
+                    B[6 * a + k2] = s;
+                }
+            auto add_edge = [&](uint32_t e) {
+                float Ji[3][6], Jj[3][6], li[3];
+                const int rows = pair_term_rows(b, c, pen, c.off[0] + e, i, j, Ji, Jj, li);
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    if (r < rows) {
+                        to_node_frame(Ji[r], gi);
+                        to_node_frame(Jj[r], gj);
+#pragma unroll
+                        for (int a = 0; a < 6; ++a)
+#pragma unroll
+                            for (int k2 = 0; k2 < 6; ++k2) B[6 * a + k2] += li[r] * Ji[r][a] * Jj[r][k2];
+                    }
+                }
+            };
+            const uint32_t ne = b.pair_edge_n[p];
+            if (ne <= kPairEdges) {
+                uint32_t ed[kPairEdges];
+#pragma unroll
+                for (int q = 0; q < (int)kPairEdges; ++q)
+                    ed[q] = (uint32_t)q < ne ? b.pair_edge[kPairEdges * (size_t)p + q] : 0xFFFFFFFFu;
+                // edge order (the edges were appended in arbitrary order)
+#pragma unroll
+                for (int q = 1; q < (int)kPairEdges; ++q)
+#pragma unroll
+                    for (int r = q; r > 0; --r)
+                        if (ed[r] < ed[r - 1]) {
+                            const uint32_t tmp = ed[r];
+                            ed[r] = ed[r - 1];
+                            ed[r - 1] = tmp;
+                        }
+                for (uint32_t q = 0; q < ne; ++q) add_edge(ed[q]);
+            } else {
+                // repeated graph edges: row i's Node2Term list holds each edge
+                // term of i once, in edge order
+                const uint32_t last = b.n2t_offset[i + 1];
+                uint32_t lo = b.n2t_offset[i], hi = last;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (b.n2t_index[mid] < c.off[0]) lo = mid + 1;
+                    else hi = mid;
+                }
+                for (uint32_t e = lo; e < last; ++e) {
+                    const uint32_t t = b.n2t_index[e];
+                    if (t >= c.off[1]) break;
+                    const ushort2 ge = f.graph[t - c.off[0]];
+                    if (ge.x != ge.y && (ge.x == j || ge.y == j)) add_edge(t - c.off[0]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 36; ++q) Bp[q] = B[q];
+        }
+    }
+}
+
 // ---------------------------------------------------------- total residual
 // computeTermResidualKernel (ResidualEvaluator.cu:71-142): sum of squared
 // lambda-weighted residuals, accumulated in double.
@@ -682,13 +1028,19 @@ void launch_eval_terms(const Batch& B, uint32_t which, bool fp64_geometry, hipSt
         hipLaunchKernelGGL(eval_terms_kernel<float>, dim3(1024, B.frames), dim3(256), 0, stream, B.dev, which);
 }
 
-void launch_assemble(const Batch& B, const Penalty& pen, bool fp64_inverse, hipStream_t stream) {
-    hipLaunchKernelGGL(assemble_chunks_kernel, dim3(2048, B.frames), dim3(256), 0, stream, B.dev, pen);
-    const uint32_t fix_blocks = std::min<uint32_t>((B.cap_nodes + B.cap_pairs + 255) / 256, 1024u);
-    if (fp64_inverse)
-        hipLaunchKernelGGL(assemble_fixup_kernel<double>, dim3(fix_blocks, B.frames), dim3(256), 0, stream, B.dev);
-    else
+// gram: the row Gram assembly with FP64 factors (mixed precision mode);
+// otherwise the reference's structure -- segmented sums over Node2Term and
+// NodePair2Term -- with FP32 inverses.
+void launch_assemble(const Batch& B, const Penalty& pen, bool gram, hipStream_t stream) {
+    const uint32_t fix_blocks = std::min<uint32_t>((8 * B.nodes + 255) / 256, 1024u);  // grid-stride over N + P
+    if (gram) {
+        const uint32_t rows = (B.nodes + kXcds - 1) / kXcds * kXcds;
+        hipLaunchKernelGGL(gram_rows_kernel, dim3(rows, B.frames), dim3(64), 0, stream, B.dev, pen);
+        hipLaunchKernelGGL(gram_fixup_kernel, dim3(fix_blocks, B.frames), dim3(256), 0, stream, B.dev, pen);
+    } else {
+        hipLaunchKernelGGL(assemble_chunks_kernel, dim3(2048, B.frames), dim3(256), 0, stream, B.dev, pen);
         hipLaunchKernelGGL(assemble_fixup_kernel<float>, dim3(fix_blocks, B.frames), dim3(256), 0, stream, B.dev);
+    }
 }
 
 // Returns the number of partials written to slot kSlotCost (per frame).

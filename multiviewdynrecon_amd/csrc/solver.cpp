@@ -12,10 +12,11 @@
 #include "common.h"
 
 namespace mvdr {
-void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, hipStream_t stream);
+void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, bool gram, hipStream_t stream);
+void launch_pair_term_lists(const Batch& B, hipStream_t stream);
 uint32_t index_chunks(uint64_t max_terms);
 void launch_eval_terms(const Batch& B, uint32_t which, bool fp64_geometry, hipStream_t stream);
-void launch_assemble(const Batch& B, const Penalty& pen, bool fp64_inverse, hipStream_t stream);
+void launch_assemble(const Batch& B, const Penalty& pen, bool gram, hipStream_t stream);
 void launch_spmv_once(const Batch& B, bool fp64, hipStream_t stream);
 void launch_pcg_update_once(const Batch& B, bool fp64, hipStream_t stream);
 uint32_t launch_cost(const Batch& B, const Penalty& pen, bool density_in_jtj, hipStream_t stream);
@@ -40,6 +41,8 @@ struct mvdr_solver {
     Batch self{};
     bool has_input = false;
     bool indexed = false;
+    bool pair_terms = false;            // NodePair2Term lists of the current index present
+    uint32_t graph_nodes = 0;           // Batch::nodes the captured graph was sized for
     uint32_t iter = 0;                  // GN iterations done on the current frame
     hipEvent_t done = nullptr;          // end of the last enqueued work
     hipGraph_t graph = nullptr;
@@ -60,6 +63,7 @@ struct mvdr_batch {
     mvdr_solver_config cfg{};
     Buffers* dev = nullptr;             // the members' Buffers, one per frame
     Batch batch{};
+    uint32_t graph_nodes = 0;
     hipEvent_t done = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -136,7 +140,7 @@ void enqueue_batch(const Batch& B, const mvdr_solver_config& c, hipStream_t stre
     const bool fp64 = c.precision == MVDR_PRECISION_MIXED;
     if (index) {
         launch_copy_se3(B, stream);
-        launch_index_build(B, c.use_foreground != 0, c.use_cross_view != 0, stream);
+        launch_index_build(B, c.use_foreground != 0, c.use_cross_view != 0, fp64, stream);
     }
     for (uint32_t k = first_iter; k < first_iter + iters; ++k) {
         const Penalty pen = kernel_penalty(c, k);
@@ -151,7 +155,12 @@ void enqueue_batch(const Batch& B, const mvdr_solver_config& c, hipStream_t stre
     }
 }
 
+// Grid sizing node count: the frame's node count rounded up to 64.
+uint32_t live_nodes(uint32_t n) { return std::max<uint32_t>(64, (n + 63) / 64 * 64); }
+
 void enqueue_solve(mvdr_solver* s, hipStream_t stream, bool index, uint32_t first_iter, uint32_t iters) {
+    s->self.nodes = live_nodes(s->frame_host.N);
+    if (index) s->pair_terms = s->cfg.precision != MVDR_PRECISION_MIXED;
     enqueue_batch(s->self, s->cfg, stream, index, first_iter, iters);
 }
 
@@ -339,8 +348,15 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     A(b.pair_range, b.cap_pairs);
     A(b.pair_terms, b.cap_kv2);
     A(b.kv2_pair, b.cap_kv2);
-    A(b.part_diag, ((size_t)b.cap_kv1 / 64 + 1) * 2 * 27);
-    A(b.part_pair, ((size_t)b.cap_kv2 / 64 + 1) * 2 * 36);
+    b.part_diag = b.part_pair = nullptr;
+    b.pair_edge_n = b.pair_edge = nullptr;
+    if (c.precision == MVDR_PRECISION_MIXED) {
+        A(b.pair_edge_n, b.cap_pairs);
+        A(b.pair_edge, (size_t)kPairEdges * b.cap_pairs);
+    } else {
+        A(b.part_diag, ((size_t)b.cap_kv1 / 64 + 1) * 2 * 27);
+        A(b.part_pair, ((size_t)b.cap_kv2 / 64 + 1) * 2 * 36);
+    }
     A(b.low_offset, N1);
     A(b.low_ent, b.cap_pairs);
     A(b.diag, 36 * (size_t)c.max_nodes);
@@ -380,7 +396,7 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
         delete s;
         return MVDR_E_HIP;
     }
-    s->self = Batch{s->b_dev, 1, s->cap_tiles, c.max_nodes, s->cap_chunks, b.cap_pairs};
+    s->self = Batch{s->b_dev, 1, s->cap_tiles, c.max_nodes, s->cap_chunks};
     *out = s;
     return MVDR_OK;
 }
@@ -502,8 +518,16 @@ mvdr_status mvdr_solver_solve(mvdr_solver* s, void* stream) {
     const uint32_t gn = s->cfg.gn_iterations;
     if (s->cfg.use_graph) {
         // The pointer table and every buffer are device-resident, so one
-        // captured graph serves every frame of this solver.
+        // captured graph serves every frame of this solver whose node count
+        // rounds to the same grid size; another is captured when it changes.
+        if (s->graph_exec && s->graph_nodes != live_nodes(s->frame_host.N)) {
+            (void)hipGraphExecDestroy(s->graph_exec);
+            (void)hipGraphDestroy(s->graph);
+            s->graph_exec = nullptr;
+            s->graph = nullptr;
+        }
         if (!s->graph_exec) {
+            s->graph_nodes = live_nodes(s->frame_host.N);
             hipStream_t cap = nullptr;
             HIP_TRY(s, hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
             hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
@@ -516,6 +540,7 @@ mvdr_status mvdr_solver_solve(mvdr_solver* s, void* stream) {
             if (e != hipSuccess) return fail(s, MVDR_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
         }
         HIP_TRY(s, hipGraphLaunch(s->graph_exec, st));
+        s->pair_terms = s->cfg.precision != MVDR_PRECISION_MIXED;
     } else {
         enqueue_solve(s, st, true, 0, gn);
         HIP_TRY(s, hipGetLastError());
@@ -582,6 +607,14 @@ mvdr_status mvdr_solver_debug_get(mvdr_solver* s, int tap, void* dst, size_t byt
     if (!s) return MVDR_E_INVALID;
     if (!s->has_input) return fail(s, MVDR_E_STATE, "no input");
     HIP_TRY(s, hipEventSynchronize(s->done));
+    if ((tap == MVDR_TAP_NODEPAIR_RANGE || tap == MVDR_TAP_NODEPAIR_TERM_INDEX) && s->indexed && !s->pair_terms) {
+        // the row Gram assembly needs no NodePair2Term lists: build them for inspection
+        s->self.nodes = live_nodes(s->frame_host.N);
+        launch_pair_term_lists(s->self, nullptr);
+        HIP_TRY(s, hipGetLastError());
+        HIP_TRY(s, hipStreamSynchronize(nullptr));
+        s->pair_terms = true;
+    }
     Counts c;
     HIP_TRY(s, hipMemcpy(&c, s->b.counts, sizeof(Counts), hipMemcpyDeviceToHost));
     const Buffers& b = s->b;
@@ -716,14 +749,12 @@ mvdr_status mvdr_batch_create(mvdr_solver* const* solvers, uint32_t count, mvdr_
     bt->members.assign(solvers, solvers + count);
     bt->cfg = solvers[0]->cfg;
     std::vector<Buffers> host(count);
-    Batch B{nullptr, count, 0, 0, 0, 0};
+    Batch B{nullptr, count, 0, 64, 0};
     for (uint32_t k = 0; k < count; ++k) {
         const mvdr_solver* m = solvers[k];
         host[k] = m->b;
         B.cap_tiles = std::max(B.cap_tiles, m->self.cap_tiles);
-        B.cap_nodes = std::max(B.cap_nodes, m->self.cap_nodes);
         B.cap_chunks = std::max(B.cap_chunks, m->self.cap_chunks);
-        B.cap_pairs = std::max(B.cap_pairs, m->self.cap_pairs);
     }
     hipError_t e = hipMalloc(&bt->dev, sizeof(Buffers) * count);
     if (e == hipSuccess) e = hipMemcpy(bt->dev, host.data(), sizeof(Buffers) * count, hipMemcpyHostToDevice);
@@ -767,9 +798,19 @@ mvdr_status mvdr_batch_solve(mvdr_batch* bt, void* stream) {
         }
     hipStream_t st = static_cast<hipStream_t>(stream);
     const uint32_t gn = bt->cfg.gn_iterations;
+    uint32_t nodes = 64;
+    for (const mvdr_solver* m : bt->members) nodes = std::max(nodes, live_nodes(m->frame_host.N));
+    bt->batch.nodes = nodes;
     hipError_t e = hipSuccess;
     if (bt->cfg.use_graph) {
+        if (bt->graph_exec && bt->graph_nodes != nodes) {
+            (void)hipGraphExecDestroy(bt->graph_exec);
+            (void)hipGraphDestroy(bt->graph);
+            bt->graph_exec = nullptr;
+            bt->graph = nullptr;
+        }
         if (!bt->graph_exec) {
+            bt->graph_nodes = nodes;
             hipStream_t cap = nullptr;
             e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
             if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
@@ -789,6 +830,7 @@ mvdr_status mvdr_batch_solve(mvdr_batch* bt, void* stream) {
     for (mvdr_solver* m : bt->members) {
         if (e == hipSuccess) e = hipEventRecord(m->done, st);
         m->indexed = true;
+        m->pair_terms = bt->cfg.precision != MVDR_PRECISION_MIXED;
         m->iter = gn;
     }
     if (e != hipSuccess) {
