@@ -314,6 +314,7 @@ struct Buffers {
     uint32_t* n2t_offset;    // N + 1
     uint32_t* n2t_index;     // kv1
     uint16_t* n2t_node;      // kv1: the node of each Node2Term entry
+    uint32_t* n2t_col;       // kv1: weight columns of the entry's knn slots, 4 x u8 (row Gram assembly)
     // node pairs (i, j), i < j, in (i, j) order
     uint32_t* row_up;        // N + 1 : pairs (i, j > i) per row i
     uint32_t* row_low;       // N + 1 : pairs (k < i, i) per row i

@@ -592,8 +592,11 @@ __device__ inline uint32_t lower_bound_lds(const uint32_t* a, uint32_t n, uint32
 }
 
 // kRowCount: pairs above / below the diagonal and pair-term entries of row i.
-// kRowKeys: the row's pair keys and lower entries, and a cleared graph-edge
-// count per pair (the row Gram assembly needs no pair-term lists).
+// kRowKeys: the row's pair keys and lower entries, a cleared graph-edge count
+// per pair, and for every Node2Term entry the weight column of each knn slot
+// holding an upper partner j (2 + rank of j among the row's upper partners;
+// 0xFF otherwise) -- what the row Gram assembly needs instead of pair-term
+// lists.
 // kRowTerms: pair keys, lower entries, and the pair-term lists (NodePair2Term)
 // with their ranges: for pair (i, j) every Node2Term entry of row i once per
 // knn slot holding j, in Node2Term order -- the order of the reference's stable
@@ -692,6 +695,23 @@ __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers
             b.row_kv2[i] = s2;
         }
         return;
+    }
+    if (kMode == kRowKeys) {
+        for (uint32_t p = beg + tid; p < end; p += 64 * kRowWaves) {
+            ushort4 k;
+            const int m = term_nodes(b, f, c.off, b.n2t_index[p], k);
+            uint32_t packed = 0xFFFFFFFFu;
+            if (m == 4 && up + 2 < 0xFF) {
+                packed = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t v = slot(k, q);
+                    const uint32_t col = (v != 0xFFFF && v > i) ? 2 + lower_bound_lds(ul, up, v) : 0xFFu;
+                    packed |= col << (8 * q);
+                }
+            }
+            b.n2t_col[p] = packed;
+        }
     }
     const uint32_t up_base = b.pair_offset[i], low_base = b.low_offset[i];
     const uint32_t len = end - beg, per = (len + kRowWaves - 1) / kRowWaves;

@@ -157,6 +157,23 @@ __global__ void __launch_bounds__(256) eval_terms_kernel(const Buffers* __restri
 // ------------------------------------------------------- per-term Jacobians
 enum : int { kDense = 0, kSmooth = 1, kForeground = 2, kFeature = 3, kCross = 4 };
 
+// The term-type bounds c.off[0..3] held in registers: kernels that write
+// global memory would otherwise reload them (with a full memory wait) at
+// every use.
+struct TermBounds {
+    uint32_t o0, o1, o2, o3;
+    __device__ explicit TermBounds(const Counts& c) : o0(c.off[0]), o1(c.off[1]), o2(c.off[2]), o3(c.off[3]) {}
+};
+
+__device__ inline int type_of(const TermBounds& c, uint32_t t, uint32_t& local) {
+    if (t < c.o0) { local = t; return kDense; }
+    if (t < c.o1) { local = t - c.o0; return kSmooth; }
+    if (t < c.o2) { local = t - c.o1; return kForeground; }
+    if (t < c.o3) { local = t - c.o2; return kFeature; }
+    local = t - c.o3;
+    return kCross;
+}
+
 __device__ inline int type_of(const Counts& c, uint32_t t, uint32_t& local) {
     if (t < c.off[0]) { local = t; return kDense; }
     if (t < c.off[1]) { local = t - c.off[0]; return kSmooth; }
@@ -654,54 +671,100 @@ __global__ void __launch_bounds__(256) assemble_fixup_kernel(const Buffers* __re
 // instruction, two 16-row tiles for the 27 values, one 16-column tile per 16
 // weight columns.  Graph-edge (smooth) terms have different rows per node: they
 // enter the diagonal and J^T r here and their pair blocks in the fix-up.
-// One wave per row; rows with more than 62 upper partners take several passes
-// of 64 weight columns.  No NodePair2Term lists are read.
+// One wave per row and 64 Node2Term entries per step of the row: the lanes
+// evaluate one entry each and stage G_t and the entry's weight row in LDS; the
+// wave then runs the product.  Each pass covers 16 weight columns (the
+// diagonal, J^T r and 14 partners); rows with more upper partners take further
+// passes.  The partner columns of every knn slot come from the index
+// (n2t_col); no NodePair2Term lists are read.
 constexpr uint32_t kGramVals = 27;
-constexpr uint32_t kGramStride = 28;    // LDS floats per term
-constexpr uint32_t kGramCols = 64;      // weight columns per pass (4 tiles of 16)
-constexpr uint32_t kGramPartners = 512; // a row's upper partners (pair_rows caps a row at 512 partners)
-constexpr uint16_t kNoCol = 0xFFFF;
+constexpr uint32_t kGramCols = 16;
+constexpr uint16_t kNoNode = 0xFFFF;
 
 typedef float gram_f4 __attribute__((ext_vector_type(4)));
 
-// G_t of term t for row node i; knn partners j > i and their weights w_i w_j.
-__device__ inline void gram_entry(const Buffers& b, const Counts& c, const Penalty& pen, uint32_t t, uint32_t i,
-                                  const float4 g, float G[kGramVals], float& wi, ushort4& part, float4& wq) {
+// A term record as loaded: 64 B (dense, foreground, feature, cross) or 32 B
+// (graph edge, upper half zero).  Loading it apart from its use lets a row
+// fetch its next 64 entries while the current ones are on the matrix cores.
+struct RawRec {
+    uint4 q[4];
+};
+
+__device__ inline RawRec load_record(const Buffers& b, const TermBounds& c, uint32_t t) {
+    RawRec r;
+    r.q[0] = r.q[1] = r.q[2] = r.q[3] = make_uint4(0, 0, 0, 0);
+    if (t == 0xFFFFFFFFu) return r;
+    uint32_t k;
+    const int type = type_of(c, t, k);
+    const uint4* p = type == kSmooth ? reinterpret_cast<const uint4*>(b.smooth + k)
+                   : type == kDense  ? reinterpret_cast<const uint4*>(b.dense + k)
+                   : type == kForeground ? reinterpret_cast<const uint4*>(b.fg + k)
+                   : type == kFeature ? reinterpret_cast<const uint4*>(b.feat + k)
+                                      : reinterpret_cast<const uint4*>(b.cross + k);
+    r.q[0] = p[0];
+    r.q[1] = p[1];
+    if (type != kSmooth) {
+        r.q[2] = p[2];
+        r.q[3] = p[3];
+    }
+    return r;
+}
+
+// lambda^2 of every term type (registers, not a by-value struct read through
+// a selected address).
+struct Lambda2 {
+    float smooth, density, foreground, feature, cross;
+    __device__ explicit Lambda2(const Penalty& p)
+        : smooth(p.smooth * p.smooth), density(p.density * p.density), foreground(p.foreground * p.foreground),
+          feature(p.feature * p.feature), cross(p.cross * p.cross) {}
+};
+
+// G_t of term t (record raw) for row node i, the knn weight w_i and, per knn
+// slot q holding an upper partner n_q > i, the weight w_i w_{n_q} (0 for the
+// other slots).
+__device__ inline void gram_entry(const Buffers& b, const TermBounds& c, const Lambda2& l2, bool density, uint32_t t,
+                                  const RawRec& raw, uint32_t i, const float4 g, float G[kGramVals], float& wi,
+                                  float wq[4], uint32_t nq[4]) {
     uint32_t k;
     const int type = type_of(c, t, k);
     float J[3][6], res[3], lam2[3];
     int rows = 3;
-    ushort4 knn = make_ushort4(kNoCol, kNoCol, kNoCol, kNoCol);
+    ushort4 knn = make_ushort4(kNoNode, kNoNode, kNoNode, kNoNode);
     float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
-    wi = 1.f;
     if (type == kSmooth) {
-        const SmoothRec sm = b.smooth[k];
+        SmoothRec sm;
+        __builtin_memcpy(&sm, &raw, sizeof(SmoothRec));
         smooth_rows(sm, i == (sm.ij & 0xFFFF), J, res);
-        lam2[0] = lam2[1] = lam2[2] = pen.smooth * pen.smooth;
+        lam2[0] = lam2[1] = lam2[2] = l2.smooth;
     } else if (type == kDense || type == kForeground) {
-        const ScalarRec& rec = type == kDense ? b.dense[k] : b.fg[k];
+        ScalarRec rec;
+        __builtin_memcpy(&rec, &raw, sizeof(ScalarRec));
         knn = rec.knn;
         w = rec.w;
 #pragma unroll
         for (int q = 0; q < 6; ++q) J[0][q] = rec.J[q];
         res[0] = rec.r;
-        lam2[0] = type == kDense ? 1.0f : pen.foreground * pen.foreground;
+        lam2[0] = type == kDense ? 1.0f : l2.foreground;
         rows = 1;
-        if (type == kDense && b.density && pen.density != 0.f) {
+        if (type == kDense && density) {
             const ScalarRec& d = b.density[k];
 #pragma unroll
             for (int q = 0; q < 6; ++q) J[1][q] = d.J[q];
             res[1] = d.r;
-            lam2[1] = pen.density * pen.density;
+            lam2[1] = l2.density;
             rows = 2;
         }
     } else {
-        const P2PRec& rec = type == kFeature ? b.feat[k] : b.cross[k];
+        P2PRec rec;
+        __builtin_memcpy(&rec, &raw, sizeof(P2PRec));
         knn = rec.knn;
         w = rec.w;
         p2p_rows(rec, J, res);
-        const float l = type == kFeature ? pen.feature : pen.cross;
-        lam2[0] = lam2[1] = lam2[2] = l * l;
+        // bit select: a select of two fields would become a load through a
+        // selected address (and the struct a scratch array)
+        const uint32_t mask = type == kFeature ? 0xFFFFFFFFu : 0u;
+        const float l = __uint_as_float((__float_as_uint(l2.feature) & mask) | (__float_as_uint(l2.cross) & ~mask));
+        lam2[0] = lam2[1] = lam2[2] = l;
     }
 #pragma unroll
     for (int q = 0; q < (int)kGramVals; ++q) G[q] = 0.f;
@@ -711,50 +774,23 @@ __device__ inline void gram_entry(const Buffers& b, const Counts& c, const Penal
             to_node_frame(J[r], g);
             int o = 0;
 #pragma unroll
-            for (int a = 0; a < 6; ++a)
+            for (int a = 0; a < 6; ++a) {
+                const float la = lam2[r] * J[r][a];
 #pragma unroll
-                for (int k2 = a; k2 < 6; ++k2) G[o++] += lam2[r] * J[r][a] * J[r][k2];
-#pragma unroll
-            for (int a = 0; a < 6; ++a) G[21 + a] -= lam2[r] * J[r][a] * res[r];
+                for (int k2 = a; k2 < 6; ++k2) G[o++] += la * J[r][k2];
+                G[21 + a] -= la * res[r];
+            }
         }
     }
-    part = make_ushort4(kNoCol, kNoCol, kNoCol, kNoCol);
-    wq = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (type != kSmooth) {
-        wi = slot_weight(knn, w, i);
-        const uint32_t n[4] = {knn.x, knn.y, knn.z, knn.w};
-        float v[4];
-        uint16_t pp[4];
+    const bool knn_term = type != kSmooth;
+    wi = knn_term ? slot_weight(knn, w, i) : 1.f;
+    const uint32_t n[4] = {knn.x, knn.y, knn.z, knn.w};
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const bool up = n[q] > i && n[q] != kNoCol;
-            pp[q] = up ? (uint16_t)n[q] : kNoCol;
-            v[q] = up ? wi * slot_weight(knn, w, n[q]) : 0.f;
-        }
-        part = make_ushort4(pp[0], pp[1], pp[2], pp[3]);
-        wq = make_float4(v[0], v[1], v[2], v[3]);
+    for (int q = 0; q < 4; ++q) {
+        const bool up = knn_term && n[q] > i && n[q] != kNoNode;
+        nq[q] = up ? n[q] : kNoNode;
+        wq[q] = up ? wi * slot_weight(knn, w, n[q]) : 0.f;
     }
-}
-
-__device__ inline uint32_t lower_bound_u32(const uint32_t* a, uint32_t n, uint32_t v) {
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (a[mid] < v) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// Weight of column n (global: 0 = diagonal, 1 = J^T r, 2 + u = upper partner u)
-// for a term with packed partner columns cols (4 x u16) and weights.
-__device__ inline float gram_weight(uint32_t n, uint2 cols, float4 wq, float wi, float wi2) {
-    float v = n == 0 ? wi2 : (n == 1 ? wi : 0.f);
-    v += (cols.x & 0xFFFF) == n ? wq.x : 0.f;
-    v += (cols.x >> 16) == n ? wq.y : 0.f;
-    v += (cols.y & 0xFFFF) == n ? wq.z : 0.f;
-    v += (cols.y >> 16) == n ? wq.w : 0.f;
-    return v;
 }
 
 __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict__ bs, Penalty pen) {
@@ -762,100 +798,122 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
-    __shared__ float Gl[64 * kGramStride];
-    __shared__ uint2 Cl[64];      // partner columns of each term
-    __shared__ float4 Wl[64];     // their weights
-    __shared__ float2 Dl[64];     // w_i^2, w_i
-    __shared__ uint32_t plist[kGramPartners];
+    // term-major rows of 16: G_t values 0..15, values 16..26 (+ 5 zeros), weights
+    __shared__ float Glo[64 * 16];
+    __shared__ float Ghi[64 * 16];
+    __shared__ float Wl[64 * 16];
     const uint32_t lane = threadIdx.x;
     const uint32_t m = lane & 15, kq = lane >> 4;
+    const Lambda2 l2(pen);
+    const bool density = b.density && pen.density != 0.f;
+    const TermBounds tb(c);
     for (uint32_t v = blockIdx.x; v < xcd_span(gridDim.x, f.N); v += gridDim.x) {
         const uint32_t i = xcd_remap(v, gridDim.x, f.N);
         if (i == 0xFFFFFFFFu) continue;
         const uint32_t beg = b.n2t_offset[i], end = b.n2t_offset[i + 1];
-        const uint32_t ubeg = b.pair_offset[i], U = min(b.pair_offset[i + 1] - ubeg, kGramPartners);
+        const uint32_t ubeg = b.pair_offset[i], U = b.pair_offset[i + 1] - ubeg;
         const float4 g = f.nodes[i];
-        for (uint32_t u = lane; u < U; u += 64) plist[u] = b.pair_key[ubeg + u] & 0xFFFF;
-        wave_lds_sync();
         const uint32_t cols = 2 + U;
+        const bool indexed = cols < 0xFF;  // partner columns held by n2t_col
         for (uint32_t c0 = 0; c0 < cols; c0 += kGramCols) {
-            const uint32_t tiles = min(4u, (cols - c0 + 15) / 16);
-            gram_f4 acc[4][2];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t][0] = acc[t][1] = (gram_f4){0.f, 0.f, 0.f, 0.f};
+            gram_f4 lo = {0.f, 0.f, 0.f, 0.f}, hi = {0.f, 0.f, 0.f, 0.f};
+            // two-stage fetch: the records of the next step and the entries of
+            // the step after it are in flight while this step computes
+            uint32_t t_cur = beg + lane < end ? b.n2t_index[beg + lane] : 0xFFFFFFFFu;
+            uint32_t pc_cur = indexed && beg + lane < end ? b.n2t_col[beg + lane] : 0xFFFFFFFFu;
+            RawRec rec_cur = load_record(b, tb, t_cur);
+            uint32_t t_nxt = beg + 64 + lane < end ? b.n2t_index[beg + 64 + lane] : 0xFFFFFFFFu;
+            uint32_t pc_nxt = indexed && beg + 64 + lane < end ? b.n2t_col[beg + 64 + lane] : 0xFFFFFFFFu;
             for (uint32_t e0 = beg; e0 < end; e0 += 64) {
-                // phase 1: lane = Node2Term entry; G_t and weight columns into LDS
-                const uint32_t e = e0 + lane;
                 float G[kGramVals];
-                float wi = 0.f;
-                ushort4 part = make_ushort4(kNoCol, kNoCol, kNoCol, kNoCol);
-                float4 wq = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (e < end) {
-                    gram_entry(b, c, pen, b.n2t_index[e], i, g, G, wi, part, wq);
+                float wi = 0.f, wq[4] = {0.f, 0.f, 0.f, 0.f};
+                uint32_t col[4] = {0xFFu, 0xFFu, 0xFFu, 0xFFu};
+                if (t_cur != 0xFFFFFFFFu) {
+                    uint32_t nq[4];
+                    gram_entry(b, tb, l2, density, t_cur, rec_cur, i, g, G, wi, wq, nq);
+                    if (indexed) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) col[q] = (pc_cur >> (8 * q)) & 0xFFu;
+                    } else {
+                        // more partners than the index's u8 columns: search the row's pair keys
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            if (nq[q] == kNoNode) continue;
+                            const uint32_t key = (i << 16) | nq[q];
+                            uint32_t l2 = 0, h2 = U;
+                            while (l2 < h2) {
+                                const uint32_t mid = (l2 + h2) >> 1;
+                                if (b.pair_key[ubeg + mid] < key) l2 = mid + 1;
+                                else h2 = mid;
+                            }
+                            col[q] = 2 + l2;
+                        }
+                    }
                 } else {
 #pragma unroll
                     for (int q = 0; q < (int)kGramVals; ++q) G[q] = 0.f;
                 }
-                uint32_t pc[4] = {part.x, part.y, part.z, part.w};
+                const RawRec rec_nxt = load_record(b, tb, t_nxt);
+                const uint32_t enn = e0 + 128 + lane;
+                const uint32_t t_nn = enn < end ? b.n2t_index[enn] : 0xFFFFFFFFu;
+                const uint32_t pc_nn = indexed && enn < end ? b.n2t_col[enn] : 0xFFFFFFFFu;
+                float* gl = Glo + 16 * lane;
+                float* gh = Ghi + 16 * lane;
+                float* wl = Wl + 16 * lane;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    if (pc[q] != kNoCol) {
-                        const uint32_t r = lower_bound_u32(plist, U, pc[q]);
-                        pc[q] = (r < U && plist[r] == pc[q]) ? 2 + r : kNoCol;
-                    }
+                    *reinterpret_cast<float4*>(gl + 4 * q) = make_float4(G[4 * q], G[4 * q + 1], G[4 * q + 2], G[4 * q + 3]);
+                    *reinterpret_cast<float4*>(wl + 4 * q) = make_float4(0.f, 0.f, 0.f, 0.f);
                 }
-                float* gl = Gl + lane * kGramStride;
+                *reinterpret_cast<float4*>(gh + 0) = make_float4(G[16], G[17], G[18], G[19]);
+                *reinterpret_cast<float4*>(gh + 4) = make_float4(G[20], G[21], G[22], G[23]);
+                *reinterpret_cast<float4*>(gh + 8) = make_float4(G[24], G[25], G[26], 0.f);
+                *reinterpret_cast<float4*>(gh + 12) = make_float4(0.f, 0.f, 0.f, 0.f);
+                // the entry's weight row: diagonal w_i^2, J^T r w_i, partners w_i w_j
+                // summed over the slots holding j
+                if (c0 == 0) {
+                    wl[0] = wi * wi;
+                    wl[1] = wi;
+                }
 #pragma unroll
-                for (int q = 0; q < (int)kGramVals; ++q) gl[q] = G[q];
-                gl[kGramVals] = 0.f;
-                Cl[lane] = make_uint2(pc[0] | (pc[1] << 16), pc[2] | (pc[3] << 16));
-                Wl[lane] = wq;
-                Dl[lane] = make_float2(wi * wi, wi);
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t r = col[q] - c0;
+                    if (wq[q] != 0.f && r < kGramCols) wl[r] += wq[q];  // the lane's own row
+                }
                 wave_lds_sync();
-                // phase 2: acc[t] += G^T W over 4 terms per step
+                // acc += G^T W over 4 entries per step
                 const uint32_t steps = (min(end - e0, 64u) + 3) / 4;
                 for (uint32_t st = 0; st < steps; ++st) {
-                    const uint32_t k = 4 * st + kq;
-                    const float a0 = Gl[k * kGramStride + m];
-                    const float a1 = m < kGramVals - 16 ? Gl[k * kGramStride + 16 + m] : 0.f;
-                    const uint2 cl = Cl[k];
-                    const float4 wk = Wl[k];
-                    const float2 dk = Dl[k];
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        if ((uint32_t)t < tiles) {
-                            const float bv = gram_weight(c0 + 16 * t + m, cl, wk, dk.y, dk.x);
-                            acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bv, acc[t][0], 0, 0, 0);
-                            acc[t][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bv, acc[t][1], 0, 0, 0);
-                        }
-                    }
+                    const uint32_t k = 16 * (4 * st + kq) + m;
+                    const float a0 = Glo[k], a1 = Ghi[k], bv = Wl[k];
+                    lo = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bv, lo, 0, 0, 0);
+                    hi = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bv, hi, 0, 0, 0);
                 }
-                wave_lds_sync();  // LDS is rewritten by the next group
+                wave_lds_sync();  // LDS is rewritten by the next step
+                t_cur = t_nxt;
+                pc_cur = pc_nxt;
+                rec_cur = rec_nxt;
+                t_nxt = t_nn;
+                pc_nxt = pc_nn;
             }
-            // lane (m, kq) holds rows 4 kq + r of both value tiles for column m of each tile
+            // lane (m, kq) holds values 4 kq + r (lo) and 16 + 4 kq + r (hi) of column c0 + m
+            const uint32_t n = c0 + m;
+            if (n == 0) {
+                float* D = b.diag + 36 * (size_t)i;  // 32 values; the fix-up reads the first 21
+                *reinterpret_cast<float4*>(D + 4 * kq) = make_float4(lo[0], lo[1], lo[2], lo[3]);
+                *reinterpret_cast<float4*>(D + 16 + 4 * kq) = make_float4(hi[0], hi[1], hi[2], hi[3]);
+            } else if (n == 1) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                if ((uint32_t)t >= tiles) continue;
-                const uint32_t n = c0 + 16 * t + m;
-                const gram_f4 lo = acc[t][0], hi = acc[t][1];
-                if (n == 0) {
-                    float* D = b.diag + 36 * (size_t)i;  // 32 values; the fix-up reads the first 21
-                    *reinterpret_cast<float4*>(D + 4 * kq) = make_float4(lo[0], lo[1], lo[2], lo[3]);
-                    *reinterpret_cast<float4*>(D + 16 + 4 * kq) = make_float4(hi[0], hi[1], hi[2], hi[3]);
-                } else if (n == 1) {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const uint32_t q = 16 + 4 * kq + r;
-                        if (q >= 21 && q < kGramVals) b.jtr[6 * (size_t)i + q - 21] = hi[r];
-                    }
-                } else if (n < cols) {
-                    float* S = b.offdiag + 36 * (size_t)(ubeg + n - 2);  // 24 values; the fix-up reads 21
-                    *reinterpret_cast<float4*>(S + 4 * kq) = make_float4(lo[0], lo[1], lo[2], lo[3]);
-                    if (kq < 2) *reinterpret_cast<float4*>(S + 16 + 4 * kq) = make_float4(hi[0], hi[1], hi[2], hi[3]);
+                for (int r = 0; r < 4; ++r) {
+                    const uint32_t q = 16 + 4 * kq + r;
+                    if (q >= 21 && q < kGramVals) b.jtr[6 * (size_t)i + q - 21] = hi[r];
                 }
+            } else if (n < cols) {
+                float* S = b.offdiag + 36 * (size_t)(ubeg + n - 2);  // 24 values; the fix-up reads 21
+                *reinterpret_cast<float4*>(S + 4 * kq) = make_float4(lo[0], lo[1], lo[2], lo[3]);
+                if (kq < 2) *reinterpret_cast<float4*>(S + 16 + 4 * kq) = make_float4(hi[0], hi[1], hi[2], hi[3]);
             }
         }
-        wave_lds_sync();  // plist is rewritten by the next row
     }
 }
 

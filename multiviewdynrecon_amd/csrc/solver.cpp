@@ -349,8 +349,9 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     A(b.pair_terms, b.cap_kv2);
     A(b.kv2_pair, b.cap_kv2);
     b.part_diag = b.part_pair = nullptr;
-    b.pair_edge_n = b.pair_edge = nullptr;
+    b.pair_edge_n = b.pair_edge = b.n2t_col = nullptr;
     if (c.precision == MVDR_PRECISION_MIXED) {
+        A(b.n2t_col, b.cap_kv1);
         A(b.pair_edge_n, b.cap_pairs);
         A(b.pair_edge, (size_t)kPairEdges * b.cap_pairs);
     } else {
