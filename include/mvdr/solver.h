@@ -171,7 +171,10 @@ typedef enum {
     MVDR_TAP_TWIST = 15,             /* f32[6N]  last PCG solution                       */
     MVDR_TAP_NODE_SE3 = 16,          /* mvdr_dq[N] current warp field                    */
     MVDR_TAP_PENALTY = 17,           /* f32[5] lambda: smooth, density, foreground, feature, cross */
-    MVDR_TAP_DENSITY_TERMS = 18      /* 16 x u32/f32 per dense term (density channel)     */
+    MVDR_TAP_DENSITY_TERMS = 18,     /* 16 x u32/f32 per dense term (density channel)     */
+    MVDR_TAP_SPMV_INPUT = 19,        /* f32[6N] vector d of the last J^T J apply (reference twist coordinates) */
+    MVDR_TAP_SPMV_OUTPUT = 20        /* f32[6N] its product J^T J d (reference coordinates); a STAGE_SPMV bench
+                                        run leaves d = M^-1 b of the last GN iteration here */
 } mvdr_tap;
 
 const char* mvdr_status_string(mvdr_status status);

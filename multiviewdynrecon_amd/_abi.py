@@ -90,6 +90,7 @@ TAP = {
     "nodepair_term_index": 4, "dense_terms": 5, "smooth_terms": 6, "feature_terms": 7, "cross_terms": 8,
     "foreground_terms": 9, "dense_pixels": 10, "jtj_diag": 11, "jtj_diag_inv": 12, "jtr": 13,
     "jtj_offdiag": 14, "twist": 15, "node_se3": 16, "penalty": 17, "density_terms": 18,
+    "spmv_input": 19, "spmv_output": 20,
 }
 
 STATUS = {0: "MVDR_OK", 1: "MVDR_E_INVALID", 2: "MVDR_E_SHAPE", 3: "MVDR_E_CAPACITY", 4: "MVDR_E_HIP",

@@ -551,6 +551,7 @@ __global__ void __launch_bounds__(1024) node_scan_kernel(const Buffers* __restri
             if (total > b.cap_kv1) atomicOr(&c.error, kErrKv1);
             c.kv1 = total;
             c.max_seg = mx;
+            c.kv2 = c.pairs = c.max_row = 0;  // set by the pair scans when the pair index is built
         } else if (job == 1) {
             if (total > b.cap_pairs) atomicOr(&c.error, kErrPairs);
             c.pairs = total;
@@ -855,9 +856,11 @@ __global__ void __launch_bounds__(256) pair_link_kernel(const Buffers* __restric
 }
 
 // ------------------------------------------------------------- host launch
-// gram: index for the row Gram assembly (pair keys and graph edges per pair);
-// otherwise the NodePair2Term lists of the reference's assembly.
-void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, bool gram, hipStream_t stream) {
+// mode kIndexLists: the NodePair2Term lists of the reference's assembly;
+// kIndexGram: pair keys, knn-slot columns and graph edges per pair for the row
+// Gram assembly; kIndexNodes: Node2Term alone (the matrix-free solve).
+void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, int mode, hipStream_t stream) {
+    const bool gram = mode == kIndexGram;
     const uint32_t F = B.frames;
     hipLaunchKernelGGL(compact_count_kernel<DensePred>, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream, B.dev,
                        DensePred{}, 1);
@@ -877,6 +880,7 @@ void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, boo
     hipLaunchKernelGGL(n2t_colscan_kernel, dim3((B.nodes + 63) / 64, F), dim3(64 * kScanWaves), 0, stream, B.dev);
     hipLaunchKernelGGL(node_scan_kernel, dim3(1, F), dim3(1024), 0, stream, B.dev, 0);
     hipLaunchKernelGGL(n2t_place_kernel, dim3(B.cap_chunks, F), dim3(64), tile * sizeof(uint32_t), stream, B.dev, tile);
+    if (mode == kIndexNodes) return;
     const uint32_t row_grid = (B.nodes + kXcds - 1) / kXcds * kXcds;
     hipLaunchKernelGGL(pair_rows_kernel<kRowCount>, dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
     hipLaunchKernelGGL(node_scan_kernel, dim3(3, F), dim3(1024), 0, stream, B.dev, 1);

@@ -14,94 +14,9 @@
 // and double in the mixed mode, where M^-1 is also kept in FP64.
 #include "common.h"
 #include "device_util.h"
+#include "pcg_util.h"
 
 namespace mvdr {
-
-constexpr int kRowLanes = 16;         // lanes per block row in the SpMV
-constexpr int kRowsPerBlock = 256 / kRowLanes;
-constexpr uint32_t kVecBlocks = 64;   // grid of the per-node vector kernels
-
-// partial-sum slots (kPartialStride doubles each)
-constexpr uint32_t kSlotDelta = 0;     // delta_k   : slot k, k <= kMaxIters
-constexpr uint32_t kSlotDQ = 72;       // dot(d, q)_k: slot 72 + k
-constexpr uint32_t kSlotRefNum = 144;  // refinement numerator / denominator
-constexpr uint32_t kSlotRefDen = 145;
-constexpr uint32_t kMaxIters = 64;
-
-__device__ inline double* slot_ptr(const Buffers& b, uint32_t slot) { return b.partial + (size_t)slot * kPartialStride; }
-
-// Sum of `count` partials, computed identically by every block (fixed order).
-template <typename Acc>
-__device__ inline Acc reduce_partials(const double* p, uint32_t count) {
-    __shared__ Acc sh[256];
-    Acc v = 0;
-    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) v += (Acc)p[i];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (uint32_t w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    const Acc total = sh[0];
-    __syncthreads();
-    return total;
-}
-
-// Block sum of per-thread values, stored as this block's partial of `slot`.
-template <typename Acc>
-__device__ inline void write_partial(const Buffers& b, uint32_t slot, Acc value) {
-    __shared__ Acc part[4];
-    value = wave_sum(value);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = value;
-    __syncthreads();
-    if (threadIdx.x == 0) slot_ptr(b, slot)[blockIdx.x] = (double)(part[0] + part[1] + part[2] + part[3]);
-}
-
-// 6x6 row-major block times a 6-vector (or its transpose), accumulated in Acc.
-template <typename Acc>
-__device__ inline void block_mv(const float* __restrict__ B, bool transposed, const float v[6], Acc y[6]) {
-    float m[36];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) {
-        const float4 t = reinterpret_cast<const float4*>(B)[q];
-        m[4 * q] = t.x; m[4 * q + 1] = t.y; m[4 * q + 2] = t.z; m[4 * q + 3] = t.w;
-    }
-    if (!transposed) {
-#pragma unroll
-        for (int a = 0; a < 6; ++a)
-#pragma unroll
-            for (int k = 0; k < 6; ++k) y[a] += (Acc)m[6 * a + k] * (Acc)v[k];
-    } else {
-#pragma unroll
-        for (int a = 0; a < 6; ++a)
-#pragma unroll
-            for (int k = 0; k < 6; ++k) y[a] += (Acc)m[6 * k + a] * (Acc)v[k];
-    }
-}
-
-// M^-1 v for node n: FP64 inverse in the mixed mode, FP32 otherwise.
-template <typename Acc>
-__device__ inline void apply_minv(const Buffers& b, uint32_t n, const float v[6], float out[6]) {
-    if constexpr (sizeof(Acc) == 8) {
-        const double* M = b.diag_inv64 + 36 * (size_t)n;
-#pragma unroll
-        for (int a = 0; a < 6; ++a) {
-            double s = 0.0;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) s += M[6 * a + k] * (double)v[k];
-            out[a] = (float)s;
-        }
-    } else {
-        const float* M = b.diag_inv + 36 * (size_t)n;
-#pragma unroll
-        for (int a = 0; a < 6; ++a) {
-            float s = 0.f;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) s += M[6 * a + k] * v[k];
-            out[a] = s;
-        }
-    }
-}
 
 // Input vector entry of node j: prev_j, or s_j + beta * prev_j.
 template <typename Acc>
@@ -817,6 +732,29 @@ void launch_cg_solve(const Batch& B, uint32_t iters, uint32_t refine, uint32_t k
             hipLaunchKernelGGL(cg_refine_step_kernel, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev, sg);
     }
     hipLaunchKernelGGL(apply_update_kernel<true>, dim3(64, B.frames), dim3(256), 0, stream, B.dev, k, refine ? sg : 0u);
+}
+
+uint32_t row_group_grid(uint32_t nodes) { return spmv_grid(nodes); }
+
+void launch_pcg_init(const Batch& B, bool fp64, hipStream_t stream) {
+    if (fp64) hipLaunchKernelGGL(pcg_init_kernel<double>, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev);
+    else hipLaunchKernelGGL(pcg_init_kernel<float>, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev);
+}
+
+void launch_pcg_update(const Batch& B, bool fp64, uint32_t k, uint32_t spmv_blocks, uint32_t dsel, hipStream_t stream) {
+    if (fp64)
+        hipLaunchKernelGGL(pcg_update_kernel<double>, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev, k,
+                           spmv_blocks, dsel);
+    else
+        hipLaunchKernelGGL(pcg_update_kernel<float>, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev, k,
+                           spmv_blocks, dsel);
+}
+
+void launch_refine_step(const Batch& B, bool fp64, uint32_t spmv_blocks, hipStream_t stream) {
+    if (fp64)
+        hipLaunchKernelGGL(refine_step_kernel<double>, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev, spmv_blocks);
+    else
+        hipLaunchKernelGGL(refine_step_kernel<float>, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev, spmv_blocks);
 }
 
 // One fused CG iteration (stage timing).

@@ -3,6 +3,7 @@
 // blocks per node pair, block-sparse PCG and the warp-field update.
 #include "common.h"
 #include "device_util.h"
+#include "terms.h"
 
 namespace mvdr {
 
@@ -152,108 +153,6 @@ __global__ void __launch_bounds__(256) eval_terms_kernel(const Buffers* __restri
             recs[k].warped = make_float4((float)w.x, (float)w.y, (float)w.z, 1.0f);
         }
     }
-}
-
-// ------------------------------------------------------- per-term Jacobians
-enum : int { kDense = 0, kSmooth = 1, kForeground = 2, kFeature = 3, kCross = 4 };
-
-// The term-type bounds c.off[0..3] held in registers: kernels that write
-// global memory would otherwise reload them (with a full memory wait) at
-// every use.
-struct TermBounds {
-    uint32_t o0, o1, o2, o3;
-    __device__ explicit TermBounds(const Counts& c) : o0(c.off[0]), o1(c.off[1]), o2(c.off[2]), o3(c.off[3]) {}
-};
-
-__device__ inline int type_of(const TermBounds& c, uint32_t t, uint32_t& local) {
-    if (t < c.o0) { local = t; return kDense; }
-    if (t < c.o1) { local = t - c.o0; return kSmooth; }
-    if (t < c.o2) { local = t - c.o1; return kForeground; }
-    if (t < c.o3) { local = t - c.o2; return kFeature; }
-    local = t - c.o3;
-    return kCross;
-}
-
-__device__ inline int type_of(const Counts& c, uint32_t t, uint32_t& local) {
-    if (t < c.off[0]) { local = t; return kDense; }
-    if (t < c.off[1]) { local = t - c.off[0]; return kSmooth; }
-    if (t < c.off[2]) { local = t - c.off[1]; return kForeground; }
-    if (t < c.off[3]) { local = t - c.off[2]; return kFeature; }
-    local = t - c.off[3];
-    return kCross;
-}
-
-// Knn slot weight of node n (PreconditionerBlocked.cu:25-30): the slot index is
-// the sum 1*(n==y) + 2*(n==z) + 3*(n==w).
-__device__ inline float slot_weight(ushort4 k, float4 w, uint32_t n) {
-    const int s = 1 * (n == k.y) + 2 * (n == k.z) + 3 * (n == k.w);
-    return s == 0 ? w.x : (s == 1 ? w.y : (s == 2 ? w.z : w.w));
-}
-
-// Huber-weighted point-to-point Jacobian rows and residuals
-// (computePointToPointICPTermJacobian, geometry_icp_jacobian.cuh:91-117).
-__device__ inline void p2p_rows(const P2PRec& t, float J[3][6], float res[3]) {
-    const float4 w = t.warped;
-    const float r0 = w.x - t.target.x, r1 = w.y - t.target.y, r2 = w.z - t.target.z;
-    const float h0 = huber(r0), h1 = huber(r1), h2 = huber(r2);
-    J[0][0] = 0.f;        J[0][1] = h0 * w.z;  J[0][2] = -h0 * w.y; J[0][3] = h0;  J[0][4] = 0.f; J[0][5] = 0.f;
-    J[1][0] = -h1 * w.z;  J[1][1] = 0.f;       J[1][2] = h1 * w.x;  J[1][3] = 0.f; J[1][4] = h1;  J[1][5] = 0.f;
-    J[2][0] = h2 * w.y;   J[2][1] = -h2 * w.x; J[2][2] = 0.f;       J[2][3] = 0.f; J[2][4] = 0.f; J[2][5] = h2;
-    res[0] = h0 * r0; res[1] = h1 * r1; res[2] = h2 * r2;
-}
-
-// computeSmoothTermJacobian (geometry_icp_jacobian.cuh:323-343) for node i or j.
-__device__ inline void smooth_rows(const SmoothRec& s, bool node_i, float J[3][6], float res[3]) {
-    const float sg = node_i ? 1.f : -1.f;
-    const float a[3] = {node_i ? s.Ti_xj[0] : s.Ti_xj[0] - s.r[0], node_i ? s.Ti_xj[1] : s.Ti_xj[1] - s.r[1],
-                        node_i ? s.Ti_xj[2] : s.Ti_xj[2] - s.r[2]};
-    const float on = s.valid ? 1.f : 0.f;
-    J[0][0] = 0.f;             J[0][1] = on * sg * a[2];  J[0][2] = -on * sg * a[1];
-    J[1][0] = -on * sg * a[2]; J[1][1] = 0.f;             J[1][2] = on * sg * a[0];
-    J[2][0] = on * sg * a[1];  J[2][1] = -on * sg * a[0]; J[2][2] = 0.f;
-    J[0][3] = on * sg; J[0][4] = 0.f;     J[0][5] = 0.f;
-    J[1][3] = 0.f;     J[1][4] = on * sg; J[1][5] = 0.f;
-    J[2][3] = 0.f;     J[2][4] = 0.f;     J[2][5] = on * sg;
-    res[0] = on * s.r[0];
-    res[1] = on * s.r[1];
-    res[2] = on * s.r[2];
-}
-
-// ------------------------------------------------ node-centred coordinates
-// The reference twist (w, v) moves a point p by w x p + v: its rotation is
-// about the world origin, so every node block has a near-null direction
-// ("rotate about the node, translate back") and FP32 storage of J^T J costs
-// ~1e-3 in the solution.  The solver assembles in the node-centred twist
-// (w, u), u = v + w x g_n, i.e. x_ref = T_n x with T_n = [[I, 0], [[g_n]x, I]].
-// Block-Jacobi PCG is invariant under this block-diagonal congruence
-// (A' = T^T A T, M' = T^T M T, b' = T^T b give x' = T^-1 x in exact arithmetic),
-// so the iterates are the reference's; only their FP32 rounding improves.
-// A row J = (J_w, J_v) becomes J' = (J_w + J_v x g, J_v).
-__device__ inline void to_node_frame(float J[6], const float4 g) {
-    const float a = J[4] * g.z - J[5] * g.y;
-    const float b = J[5] * g.x - J[3] * g.z;
-    const float c = J[3] * g.y - J[4] * g.x;
-    J[0] += a;
-    J[1] += b;
-    J[2] += c;
-}
-
-// The reference's damping 1e-2 I (PreconditionerBlocked.cu:272-275) seen in
-// node-centred coordinates: 1e-2 T^T T = 1e-2 [[I + G^T G, G^T], [G, I]], G = [g]x.
-__device__ inline void add_node_frame_damping(float D[36], const float4 g) {
-    const float l = 1e-2f;
-    const float gg = g.x * g.x + g.y * g.y + g.z * g.z;
-    const float gv[3] = {g.x, g.y, g.z};
-    const float G[3][3] = {{0.f, -g.z, g.y}, {g.z, 0.f, -g.x}, {-g.y, g.x, 0.f}};
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            D[6 * r + c] += l * ((r == c ? 1.f + gg : 0.f) - gv[r] * gv[c]);
-            D[6 * r + 3 + c] += l * G[c][r];        // G^T
-            D[6 * (3 + r) + c] += l * G[r][c];      // G
-            D[6 * (3 + r) + 3 + c] += (r == c) ? l : 0.f;
-        }
 }
 
 // -------------------------------------------------- 6x6 SPD inverse (Cholesky)
@@ -793,7 +692,8 @@ __device__ inline void gram_entry(const Buffers& b, const TermBounds& c, const L
     }
 }
 
-__global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict__ bs, Penalty pen) {
+// diag_only: the diagonal blocks and J^T r alone (the matrix-free solve).
+__global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict__ bs, Penalty pen, int diag_only) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
@@ -811,7 +711,7 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
         const uint32_t i = xcd_remap(v, gridDim.x, f.N);
         if (i == 0xFFFFFFFFu) continue;
         const uint32_t beg = b.n2t_offset[i], end = b.n2t_offset[i + 1];
-        const uint32_t ubeg = b.pair_offset[i], U = b.pair_offset[i + 1] - ubeg;
+        const uint32_t ubeg = diag_only ? 0 : b.pair_offset[i], U = diag_only ? 0 : b.pair_offset[i + 1] - ubeg;
         const float4 g = f.nodes[i];
         const uint32_t cols = 2 + U;
         const bool indexed = cols < 0xFF;  // partner columns held by n2t_col
@@ -820,10 +720,11 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
             // two-stage fetch: the records of the next step and the entries of
             // the step after it are in flight while this step computes
             uint32_t t_cur = beg + lane < end ? b.n2t_index[beg + lane] : 0xFFFFFFFFu;
-            uint32_t pc_cur = indexed && beg + lane < end ? b.n2t_col[beg + lane] : 0xFFFFFFFFu;
+            const bool cols_in = indexed && !diag_only;
+            uint32_t pc_cur = cols_in && beg + lane < end ? b.n2t_col[beg + lane] : 0xFFFFFFFFu;
             RawRec rec_cur = load_record(b, tb, t_cur);
             uint32_t t_nxt = beg + 64 + lane < end ? b.n2t_index[beg + 64 + lane] : 0xFFFFFFFFu;
-            uint32_t pc_nxt = indexed && beg + 64 + lane < end ? b.n2t_col[beg + 64 + lane] : 0xFFFFFFFFu;
+            uint32_t pc_nxt = cols_in && beg + 64 + lane < end ? b.n2t_col[beg + 64 + lane] : 0xFFFFFFFFu;
             for (uint32_t e0 = beg; e0 < end; e0 += 64) {
                 float G[kGramVals];
                 float wi = 0.f, wq[4] = {0.f, 0.f, 0.f, 0.f};
@@ -831,7 +732,9 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
                 if (t_cur != 0xFFFFFFFFu) {
                     uint32_t nq[4];
                     gram_entry(b, tb, l2, density, t_cur, rec_cur, i, g, G, wi, wq, nq);
-                    if (indexed) {
+                    if (diag_only) {
+                        // no partner columns
+                    } else if (indexed) {
 #pragma unroll
                         for (int q = 0; q < 4; ++q) col[q] = (pc_cur >> (8 * q)) & 0xFFu;
                     } else {
@@ -856,7 +759,7 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
                 const RawRec rec_nxt = load_record(b, tb, t_nxt);
                 const uint32_t enn = e0 + 128 + lane;
                 const uint32_t t_nn = enn < end ? b.n2t_index[enn] : 0xFFFFFFFFu;
-                const uint32_t pc_nn = indexed && enn < end ? b.n2t_col[enn] : 0xFFFFFFFFu;
+                const uint32_t pc_nn = cols_in && enn < end ? b.n2t_col[enn] : 0xFFFFFFFFu;
                 float* gl = Glo + 16 * lane;
                 float* gh = Ghi + 16 * lane;
                 float* wl = Wl + 16 * lane;
@@ -883,7 +786,23 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
                 wave_lds_sync();
                 // acc += G^T W over 4 entries per step
                 const uint32_t steps = (min(end - e0, 64u) + 3) / 4;
-                for (uint32_t st = 0; st < steps; ++st) {
+                uint32_t st = 0;
+                for (; st + 4 <= steps; st += 4) {  // four steps' operands in flight at once
+                    float a0[4], a1[4], bv[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const uint32_t k = 16 * (4 * (st + u) + kq) + m;
+                        a0[u] = Glo[k];
+                        a1[u] = Ghi[k];
+                        bv[u] = Wl[k];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        lo = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], bv[u], lo, 0, 0, 0);
+                        hi = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], bv[u], hi, 0, 0, 0);
+                    }
+                }
+                for (; st < steps; ++st) {
                     const uint32_t k = 16 * (4 * st + kq) + m;
                     const float a0 = Glo[k], a1 = Ghi[k], bv = Wl[k];
                     lo = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bv, lo, 0, 0, 0);
@@ -922,12 +841,12 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
 // (blockDiagonalInverseKernel, BlockDiagonalPreconditionerInverse.cu:6-41);
 // pair blocks get the change of centre B_ij = S_ij M_ij and the pair blocks of
 // the graph edges joining them, in edge order.
-__global__ void __launch_bounds__(256) gram_fixup_kernel(const Buffers* __restrict__ bs, Penalty pen) {
+__global__ void __launch_bounds__(256) gram_fixup_kernel(const Buffers* __restrict__ bs, Penalty pen, int diag_only) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
-    const uint32_t N = f.N, P = c.pairs;
+    const uint32_t N = f.N, P = diag_only ? 0 : c.pairs;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N + P; x += stride) {
         if (x < N) {
@@ -1089,12 +1008,14 @@ void launch_eval_terms(const Batch& B, uint32_t which, bool fp64_geometry, hipSt
 // gram: the row Gram assembly with FP64 factors (mixed precision mode);
 // otherwise the reference's structure -- segmented sums over Node2Term and
 // NodePair2Term -- with FP32 inverses.
-void launch_assemble(const Batch& B, const Penalty& pen, bool gram, hipStream_t stream) {
+// diag_only (with gram): diagonal blocks and J^T r only, for the matrix-free solve.
+void launch_assemble(const Batch& B, const Penalty& pen, bool gram, bool diag_only, hipStream_t stream) {
     const uint32_t fix_blocks = std::min<uint32_t>((8 * B.nodes + 255) / 256, 1024u);  // grid-stride over N + P
     if (gram) {
         const uint32_t rows = (B.nodes + kXcds - 1) / kXcds * kXcds;
-        hipLaunchKernelGGL(gram_rows_kernel, dim3(rows, B.frames), dim3(64), 0, stream, B.dev, pen);
-        hipLaunchKernelGGL(gram_fixup_kernel, dim3(fix_blocks, B.frames), dim3(256), 0, stream, B.dev, pen);
+        const int d = diag_only ? 1 : 0;
+        hipLaunchKernelGGL(gram_rows_kernel, dim3(rows, B.frames), dim3(64), 0, stream, B.dev, pen, d);
+        hipLaunchKernelGGL(gram_fixup_kernel, dim3(fix_blocks, B.frames), dim3(256), 0, stream, B.dev, pen, d);
     } else {
         hipLaunchKernelGGL(assemble_chunks_kernel, dim3(2048, B.frames), dim3(256), 0, stream, B.dev, pen);
         hipLaunchKernelGGL(assemble_fixup_kernel<float>, dim3(fix_blocks, B.frames), dim3(256), 0, stream, B.dev);

@@ -12,11 +12,14 @@
 #include "common.h"
 
 namespace mvdr {
-void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, bool gram, hipStream_t stream);
+void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, int mode, hipStream_t stream);
 void launch_pair_term_lists(const Batch& B, hipStream_t stream);
 uint32_t index_chunks(uint64_t max_terms);
 void launch_eval_terms(const Batch& B, uint32_t which, bool fp64_geometry, hipStream_t stream);
-void launch_assemble(const Batch& B, const Penalty& pen, bool gram, hipStream_t stream);
+void launch_assemble(const Batch& B, const Penalty& pen, bool gram, bool diag_only, hipStream_t stream);
+void launch_pcg_matrix_free(const Batch& B, uint32_t iters, uint32_t refine, bool fp64, const Penalty& pen,
+                            hipStream_t stream);
+void launch_mf_apply_once(const Batch& B, bool fp64, const Penalty& pen, hipStream_t stream);
 void launch_spmv_once(const Batch& B, bool fp64, hipStream_t stream);
 void launch_pcg_update_once(const Batch& B, bool fp64, hipStream_t stream);
 uint32_t launch_cost(const Batch& B, const Penalty& pen, bool density_in_jtj, hipStream_t stream);
@@ -138,14 +141,23 @@ const char* error_text(uint32_t bits) {
 void enqueue_batch(const Batch& B, const mvdr_solver_config& c, hipStream_t stream, bool index, uint32_t first_iter,
                    uint32_t iters) {
     const bool fp64 = c.precision == MVDR_PRECISION_MIXED;
+    const bool mf = c.jtj_mode == MVDR_JTJ_MATRIX_FREE;
     if (index) {
         launch_copy_se3(B, stream);
-        launch_index_build(B, c.use_foreground != 0, c.use_cross_view != 0, fp64, stream);
+        launch_index_build(B, c.use_foreground != 0, c.use_cross_view != 0,
+                           mf ? kIndexNodes : (fp64 ? kIndexGram : kIndexLists), stream);
     }
     for (uint32_t k = first_iter; k < first_iter + iters; ++k) {
         const Penalty pen = kernel_penalty(c, k);
         launch_eval_terms(B, k & 1, fp64, stream);
-        launch_assemble(B, pen, fp64, stream);
+        if (mf) {
+            // diagonal blocks and J^T r from the row Gram assembly; J^T J applied matrix-free
+            launch_assemble(B, pen, true, true, stream);
+            launch_pcg_matrix_free(B, c.pcg_iterations, c.refine_steps, fp64, pen, stream);
+            launch_apply_update(B, k, stream);
+            continue;
+        }
+        launch_assemble(B, pen, fp64, false, stream);
         if (fp64) {
             launch_cg_solve(B, c.pcg_iterations, c.refine_steps, k, stream);
         } else {
@@ -160,7 +172,7 @@ uint32_t live_nodes(uint32_t n) { return std::max<uint32_t>(64, (n + 63) / 64 * 
 
 void enqueue_solve(mvdr_solver* s, hipStream_t stream, bool index, uint32_t first_iter, uint32_t iters) {
     s->self.nodes = live_nodes(s->frame_host.N);
-    if (index) s->pair_terms = s->cfg.precision != MVDR_PRECISION_MIXED;
+    if (index) s->pair_terms = s->cfg.precision != MVDR_PRECISION_MIXED && s->cfg.jtj_mode != MVDR_JTJ_MATRIX_FREE;
     enqueue_batch(s->self, s->cfg, stream, index, first_iter, iters);
 }
 
@@ -203,6 +215,22 @@ mvdr_status to_reference_frame(mvdr_solver* s, int tap, float* data, uint32_t pa
     std::vector<float4> g(N);
     HIP_TRY(s, hipMemcpy(g.data(), s->frame_host.nodes, sizeof(float4) * N, hipMemcpyDeviceToHost));
     double Ti[36], Tinv[36], Tt[36];
+    if (tap == MVDR_TAP_SPMV_INPUT || tap == MVDR_TAP_SPMV_OUTPUT) {
+        // d_ref = T d', q_ref = T^-T q'
+        for (uint32_t n = 0; n < N; ++n) {
+            node_transform(g[n], tap == MVDR_TAP_SPMV_OUTPUT, Ti);
+            float* v = data + 6 * (size_t)n;
+            double out[6];
+            for (int a = 0; a < 6; ++a) {
+                double acc = 0;
+                for (int p = 0; p < 6; ++p)
+                    acc += (tap == MVDR_TAP_SPMV_INPUT ? Ti[6 * a + p] : Ti[6 * p + a]) * v[p];
+                out[a] = acc;
+            }
+            for (int a = 0; a < 6; ++a) v[a] = (float)out[a];
+        }
+        return MVDR_OK;
+    }
     if (tap == MVDR_TAP_JTJ_DIAG || tap == MVDR_TAP_JTJ_DIAG_INV || tap == MVDR_TAP_JTR) {
         for (uint32_t n = 0; n < N; ++n) {
             node_transform(g[n], true, Tinv);
@@ -292,7 +320,6 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     if (c.max_views < 1 || c.max_views > MVDR_MAX_VIEWS || c.max_width < 1 || c.max_height < 1 || c.max_width > 65535 ||
         c.max_height > 65535 || c.max_nodes < 1 || c.max_nodes > 65535 || c.precision > 1 || c.jtj_mode > 1)
         return MVDR_E_SHAPE;
-    if (c.jtj_mode == MVDR_JTJ_MATRIX_FREE) return MVDR_E_INVALID;  // not available in this build
     auto* s = new mvdr_solver();
     s->cfg = c;
     Buffers& b = s->b;
@@ -349,8 +376,12 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     A(b.pair_terms, b.cap_kv2);
     A(b.kv2_pair, b.cap_kv2);
     b.part_diag = b.part_pair = nullptr;
+    const bool mf = c.jtj_mode == MVDR_JTJ_MATRIX_FREE;
     b.pair_edge_n = b.pair_edge = b.n2t_col = nullptr;
-    if (c.precision == MVDR_PRECISION_MIXED) {
+    b.mf_u = nullptr;
+    if (mf) {
+        A(b.mf_u, knn_terms + b.cap_edges);
+    } else if (c.precision == MVDR_PRECISION_MIXED) {
         A(b.n2t_col, b.cap_kv1);
         A(b.pair_edge_n, b.cap_pairs);
         A(b.pair_edge, (size_t)kPairEdges * b.cap_pairs);
@@ -363,15 +394,17 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     A(b.diag, 36 * (size_t)c.max_nodes);
     A(b.diag_inv, 36 * (size_t)c.max_nodes);
     b.diag_inv64 = nullptr;
-    if (c.precision == MVDR_PRECISION_MIXED) A(b.diag_inv64, 36 * (size_t)c.max_nodes);
+    // FP64 factors: the mixed mode, and the row Gram assembly of the matrix-free solve
+    const bool factors64 = c.precision == MVDR_PRECISION_MIXED || mf;
+    if (factors64) A(b.diag_inv64, 36 * (size_t)c.max_nodes);
     A(b.jtr, 6 * (size_t)c.max_nodes);
     A(b.offdiag, 36 * (size_t)b.cap_pairs);
     for (float** v : {&b.x, &b.r, &b.s, &b.q, &b.d0, &b.d1}) A(*v, 6 * (size_t)c.max_nodes);
     b.linv64 = nullptr;
     b.offhat = nullptr;
     b.cg_scal = nullptr;
-    if (c.precision == MVDR_PRECISION_MIXED) {
-        A(b.linv64, 36 * (size_t)c.max_nodes);
+    if (factors64) A(b.linv64, 36 * (size_t)c.max_nodes);
+    if (c.precision == MVDR_PRECISION_MIXED && !mf) {
         A(b.offhat, 36 * (size_t)b.cap_pairs);
         for (float** v : {&b.cg_r[0], &b.cg_r[1], &b.cg_w[0], &b.cg_w[1], &b.cg_s[0], &b.cg_s[1], &b.cg_p, &b.cg_x,
                           &b.cg_b})
@@ -541,7 +574,7 @@ mvdr_status mvdr_solver_solve(mvdr_solver* s, void* stream) {
             if (e != hipSuccess) return fail(s, MVDR_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
         }
         HIP_TRY(s, hipGraphLaunch(s->graph_exec, st));
-        s->pair_terms = s->cfg.precision != MVDR_PRECISION_MIXED;
+        s->pair_terms = s->cfg.precision != MVDR_PRECISION_MIXED && s->cfg.jtj_mode != MVDR_JTJ_MATRIX_FREE;
     } else {
         enqueue_solve(s, st, true, 0, gn);
         HIP_TRY(s, hipGetLastError());
@@ -608,7 +641,8 @@ mvdr_status mvdr_solver_debug_get(mvdr_solver* s, int tap, void* dst, size_t byt
     if (!s) return MVDR_E_INVALID;
     if (!s->has_input) return fail(s, MVDR_E_STATE, "no input");
     HIP_TRY(s, hipEventSynchronize(s->done));
-    if ((tap == MVDR_TAP_NODEPAIR_RANGE || tap == MVDR_TAP_NODEPAIR_TERM_INDEX) && s->indexed && !s->pair_terms) {
+    if ((tap == MVDR_TAP_NODEPAIR_RANGE || tap == MVDR_TAP_NODEPAIR_TERM_INDEX) && s->indexed && !s->pair_terms &&
+        s->cfg.jtj_mode != MVDR_JTJ_MATRIX_FREE) {
         // the row Gram assembly needs no NodePair2Term lists: build them for inspection
         s->self.nodes = live_nodes(s->frame_host.N);
         launch_pair_term_lists(s->self, nullptr);
@@ -658,13 +692,15 @@ mvdr_status mvdr_solver_debug_get(mvdr_solver* s, int tap, void* dst, size_t byt
             return MVDR_OK;
         }
         case MVDR_TAP_DENSITY_TERMS: src = b.density; size = b.density ? sizeof(ScalarRec) * (size_t)c.dense : 0; break;
+        case MVDR_TAP_SPMV_INPUT: src = b.d1; size = 24ull * N; break;
+        case MVDR_TAP_SPMV_OUTPUT: src = b.q; size = 24ull * N; break;
         default: return fail(s, MVDR_E_INVALID, "unknown tap");
     }
     if (needed) *needed = size;
     if (size && (!dst || bytes < size)) return fail(s, MVDR_E_CAPACITY, "buffer too small");
     if (size) HIP_TRY(s, hipMemcpy(dst, src, size, hipMemcpyDeviceToHost));
     if (size && (tap == MVDR_TAP_JTJ_DIAG || tap == MVDR_TAP_JTJ_DIAG_INV || tap == MVDR_TAP_JTR ||
-                 tap == MVDR_TAP_JTJ_OFFDIAG)) {
+                 tap == MVDR_TAP_JTJ_OFFDIAG || tap == MVDR_TAP_SPMV_INPUT || tap == MVDR_TAP_SPMV_OUTPUT)) {
         const mvdr_status rc = to_reference_frame(s, tap, static_cast<float*>(dst), c.pairs);
         if (rc != MVDR_OK) return rc;
     }
@@ -678,12 +714,13 @@ mvdr_status mvdr_solver_debug_get(mvdr_solver* s, int tap, void* dst, size_t byt
 static mvdr_status bench_stage_impl(const Batch& B, const mvdr_solver_config& c, uint32_t k, int stage, uint32_t reps,
                                     hipStream_t st, float* avg_ms, std::string& err) {
     const bool fp64 = c.precision == MVDR_PRECISION_MIXED;
+    const bool mf = c.jtj_mode == MVDR_JTJ_MATRIX_FREE;
     if (stage < MVDR_STAGE_EVAL_TERMS || stage > MVDR_STAGE_CG_ITERATION) {
         err = "unknown stage";
         return MVDR_E_INVALID;
     }
-    if (stage == MVDR_STAGE_CG_ITERATION && !fp64) {
-        err = "the fused CG iteration runs in the mixed precision mode";
+    if (stage == MVDR_STAGE_CG_ITERATION && (!fp64 || mf)) {
+        err = "the fused CG iteration runs in the mixed precision mode with materialized J^T J";
         return MVDR_E_INVALID;
     }
     const Penalty pen = kernel_penalty(c, k);
@@ -694,9 +731,12 @@ static mvdr_status bench_stage_impl(const Batch& B, const mvdr_solver_config& c,
     for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) {
         switch (stage) {
             case MVDR_STAGE_EVAL_TERMS: launch_eval_terms(B, k & 1, fp64, st); break;
-            case MVDR_STAGE_ASSEMBLE: launch_assemble(B, pen, fp64, st); break;
+            case MVDR_STAGE_ASSEMBLE: launch_assemble(B, pen, fp64 || mf, mf, st); break;
             case MVDR_STAGE_PCG_UPDATE: launch_pcg_update_once(B, fp64, st); break;
-            case MVDR_STAGE_SPMV: launch_spmv_once(B, fp64, st); break;
+            case MVDR_STAGE_SPMV:
+                if (mf) launch_mf_apply_once(B, fp64, pen, st);
+                else launch_spmv_once(B, fp64, st);
+                break;
             case MVDR_STAGE_INDEX: enqueue_batch(B, c, st, true, 0, 0); break;
             default: launch_cg_iter_once(B, st); break;
         }
@@ -831,7 +871,7 @@ mvdr_status mvdr_batch_solve(mvdr_batch* bt, void* stream) {
     for (mvdr_solver* m : bt->members) {
         if (e == hipSuccess) e = hipEventRecord(m->done, st);
         m->indexed = true;
-        m->pair_terms = bt->cfg.precision != MVDR_PRECISION_MIXED;
+        m->pair_terms = bt->cfg.precision != MVDR_PRECISION_MIXED && bt->cfg.jtj_mode != MVDR_JTJ_MATRIX_FREE;
         m->iter = gn;
     }
     if (e != hipSuccess) {

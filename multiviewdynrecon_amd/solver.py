@@ -26,7 +26,7 @@ TAP_FORMAT = {
     "foreground_terms": (np.uint32, 16), "dense_pixels": (np.uint16, 4), "jtj_diag": (np.float32, 36),
     "jtj_diag_inv": (np.float32, 36), "jtr": (np.float32, 6), "jtj_offdiag": (np.float32, 36),
     "twist": (np.float32, 6), "node_se3": (np.float32, 8), "penalty": (np.float32, 1),
-    "density_terms": (np.uint32, 16),
+    "density_terms": (np.uint32, 16), "spmv_input": (np.float32, 6), "spmv_output": (np.float32, 6),
 }
 
 
