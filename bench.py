@@ -36,7 +36,8 @@ HBM_PEAK_GBS = 8000.0
 WORKLOAD_NAMES = {
     0: "sphere_1view_320x240_1k_nodes",
     1: "body_3view_640x480_4k_nodes",
-    3: "body_3view_640x480_8k_nodes",
+    3: "body_3view_640x480_8k_nodes_matrix_free",
+    4: "uneven_16_scenes_1k-8k_nodes_1-3_views",
 }
 
 
@@ -45,7 +46,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", type=int, default=1, choices=[0, 1, 3])
+    ap.add_argument("--workload", type=int, default=1, choices=[0, 1, 3, 4],
+                    help="1: the headline frame (default); 3: ~8k nodes, 15 PCG, matrix-free J^T J; "
+                         "4: a batch of 16 uneven scenes (aggregate frames/s, p50 / p99 latency)")
     ap.add_argument("--precision", default="mixed", choices=["fp32", "mixed"])
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--cpu-frames", type=int, default=8, help="frames in the CPU-baseline sample")
@@ -58,7 +61,7 @@ def parse():
     return ap.parse_args()
 
 
-def algorithmic_bytes(sz, mixed):
+def algorithmic_bytes(sz, mixed, matrix_free=False):
     """Minimum compulsory bytes per launch of each stage (format independent)."""
     N, P = sz["num_nodes"], sz["num_node_pairs"]
     Td, Tfg, E = sz["num_dense_terms"], sz["num_foreground_terms"], sz["num_smooth_terms"]
@@ -68,6 +71,13 @@ def algorithmic_bytes(sz, mixed):
     # + warped point + residual (smooth), knn + weights + target + warped (p2p)
     r_terms = 52 * (Td + Tfg) + 29 * E + 56 * (Tf + Tc)
     inv = 144 * N + (288 * N if mixed else 0)
+    if matrix_free:
+        return {
+            # diagonal blocks and J^T r only
+            "assemble": r_terms + 4 * kv1 + 8 * (N + 1) + 16 * N + 144 * N + inv + 24 * N,
+            # J^T (J d): term records, Node2Term lists, d in, q out
+            "spmv": r_terms + 4 * kv1 + 4 * (N + 1) + 24 * N + 24 * N,
+        }
     return {
         # Node2Term lists + offsets, pair keys + offsets, node positions in;
         # diagonal blocks, inverses, J^T r and off-diagonal blocks out
@@ -175,6 +185,74 @@ def measured_traffic(kernel, workload, frames):
     return None
 
 
+def bench_uneven(args):
+    """16 independent scenes, N ~ U[1000, 8000] nodes and 1-3 views each, solved
+    as one batch (mixed precision, one hipGraph per batch): aggregate frames/s
+    over >= 100 batches, and the per-scene latency (a scene completes with its
+    batch) at p50 / p99; each scene solved alone is reported beside it."""
+    import torch
+
+    from multiviewdynrecon_amd.config import workload_config
+    from multiviewdynrecon_amd.frame import DeviceFrame
+    from multiviewdynrecon_amd.scene import make_scene
+    from multiviewdynrecon_amd.solver import NonRigidBatch, NonRigidSolver
+
+    rng = np.random.default_rng(0x4D564452 + 100)
+    count = 16
+    nodes = rng.integers(1000, 8001, count)
+    views = rng.integers(1, 4, count)
+    scenes = [make_scene(4, seed=0x4D564452 + 100 + s, target_nodes=int(nodes[s]), target_surfels=int(37.5 * nodes[s]),
+                         num_views=int(views[s]), cross_pairs=2000 if views[s] > 1 else 0) for s in range(count)]
+    cfg = workload_config(4, use_graph=args.graph)
+    frames = [DeviceFrame(sc) for sc in scenes]
+    solvers = []
+    for fr in frames:
+        s = NonRigidSolver(cfg)
+        s.SetSolverInput(fr)
+        solvers.append(s)
+    batch = NonRigidBatch(solvers)
+    print(f"[bench] {count} scenes resident: nodes {sorted(int(n) for n in nodes)}, views {views.tolist()}",
+          file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        batch.SolveNonRigidAlignment()
+    torch.cuda.synchronize()
+    batches = max(args.steps, 100)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(batches)]
+    t0 = time.perf_counter()
+    for e0, e1 in ev:
+        e0.record()
+        batch.SolveNonRigidAlignment()
+        e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([a.elapsed_time(b) for a, b in ev])
+    fps = count * batches / wall
+    solo = []
+    for s in solvers:
+        for _ in range(2):
+            s.SolveNonRigidAlignment()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(5):
+            s.SolveNonRigidAlignment()
+        torch.cuda.synchronize()
+        solo.append((time.perf_counter() - t) * 1e3 / 5)
+    print(f"[bench] {batches} batches of {count}: {wall * 1e3 / batches:.3f} ms/batch, {fps:.1f} frames/s; "
+          f"latency p50 {np.percentile(lat, 50):.3f} p99 {np.percentile(lat, 99):.3f} ms", file=sys.stderr, flush=True)
+    out = {
+        "metric": "aggregate frames/s, 16 uneven scenes per batch (1k-8k nodes, 1-3 views)",
+        "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1, "steps": batches, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / batches, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (f64 reductions / 6x6 factors)", "data": "synthetic (seeded scene generator)",
+        "config": {"workload": WORKLOAD_NAMES[4], "scenes": count, "nodes": [int(n) for n in nodes],
+                   "views": views.tolist(), "gn_iterations": cfg.gn_iterations, "pcg_iterations": cfg.pcg_iterations,
+                   "precision": "mixed", "hip_graph": bool(args.graph)},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
+        "solo_latency_ms": {"p50": round(float(np.percentile(solo, 50)), 4), "p99": round(float(np.percentile(solo, 99)), 4)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -188,6 +266,8 @@ def main():
     from multiviewdynrecon_amd.scene import make_scene
     from multiviewdynrecon_amd.solver import NonRigidBatch, NonRigidSolver
 
+    if args.workload == 4:
+        return bench_uneven(args)
     overrides = dict(use_graph=args.graph)
     mixed = args.precision == "mixed"
     if mixed:
@@ -246,7 +326,12 @@ def main():
     print(f"[bench] single-frame latency {latency_ms:.3f} ms", file=sys.stderr, flush=True)
 
     # per-stage kernel timing of the batched launches (hipEvents on the stream)
-    if mixed:
+    mf = cfg.jtj_mode == 1
+    if mf:
+        per_solve = {"eval_terms": cfg.gn_iterations, "assemble": cfg.gn_iterations,
+                     "spmv": cfg.gn_iterations * (cfg.pcg_iterations + 2 * cfg.refine_steps),
+                     "pcg_update": cfg.gn_iterations * cfg.pcg_iterations, "index": 1}
+    elif mixed:
         per_solve = {"eval_terms": cfg.gn_iterations, "assemble": cfg.gn_iterations,
                      "cg_iteration": cfg.gn_iterations * cfg.pcg_iterations, "index": 1}
     else:
@@ -261,12 +346,12 @@ def main():
     sizes = all_sizes[0]
     ab = {}
     for sz in all_sizes:
-        for k, v in algorithmic_bytes(sz, mixed).items():
+        for k, v in algorithmic_bytes(sz, mixed, mf).items():
             ab[k] = ab.get(k, 0) + v
     share = {k: stages[k] * per_solve[k] for k in stages}
     print("[bench] stage ms/launch " + json.dumps({k: round(v, 4) for k, v in stages.items()}), file=sys.stderr)
     print("[bench] stage ms/step   " + json.dumps({k: round(v, 4) for k, v in share.items()}), file=sys.stderr)
-    apply_stage = "cg_iteration" if mixed else "spmv"
+    apply_stage = "cg_iteration" if mixed and not mf else "spmv"
     dominant = max(("assemble", apply_stage), key=lambda k: share[k])
     achieved = ab[dominant] / (stages[dominant] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
@@ -295,7 +380,8 @@ def main():
               file=sys.stderr, flush=True)
 
     out = {
-        "metric": "non-rigid solve ms/frame (3 views, ~4k nodes) at <=1e-4 rel err vs FP64",
+        "metric": ("non-rigid solve ms/frame (3 views, ~4k nodes) at <=1e-4 rel err vs FP64" if args.workload == 1 else
+                   f"non-rigid solve ms/frame ({WORKLOAD_NAMES[args.workload]})"),
         "value": round(ms_per_frame, 4),
         "unit": "ms/frame",
         "n_gpus": 1,

@@ -49,6 +49,8 @@ def workload_config(workload, **overrides):
     base = {0: dict(gn_iterations=3, pcg_iterations=10, max_width=320, max_height=240, max_views=1),
             1: dict(gn_iterations=5, pcg_iterations=10),
             2: dict(gn_iterations=5, pcg_iterations=10, precision=1, refine_steps=1),
-            3: dict(gn_iterations=5, pcg_iterations=15, jtj_mode=1)}[workload]
+            3: dict(gn_iterations=5, pcg_iterations=15, jtj_mode=1),
+            4: dict(gn_iterations=5, pcg_iterations=10, precision=1, refine_steps=1, max_nodes=12288,
+                    max_graph_edges=8 * 12288)}[workload]
     base.update(overrides)
     return default_config(**base)

@@ -427,6 +427,13 @@ int mvdr_scene_default_params(int workload, mvdr_scene_params* p) {
             p->cam_distance_min = 1.0f; p->cam_distance_max = 1.5f;
             p->features_per_view = 300; p->cross_pairs = 2000;
             return 0;
+        case 4:  // one scene of the uneven batch; the caller sets seed, nodes, surfels and views per scene
+            p->seed = 0x4D564452ull + 100;
+            p->shape = 1; p->num_views = 3; p->width = 640; p->height = 480; p->fx = p->fy = 570.f;
+            p->target_surfels = 150000; p->target_nodes = 4000;
+            p->cam_distance_min = 1.0f; p->cam_distance_max = 1.5f;
+            p->features_per_view = 300; p->cross_pairs = 2000;
+            return 0;
         default:
             return -1;
     }
