@@ -15,6 +15,7 @@
 #include "common.h"
 #include "device_util.h"
 #include "pcg_util.h"
+#include "terms.h"
 
 namespace mvdr {
 
@@ -249,12 +250,20 @@ __global__ void __launch_bounds__(256) refine_step_kernel(const Buffers* __restr
 // the previous launch's per-block partials (in a fixed order, in every block),
 // forms s_i = w_i + beta_i s_{i-1} and r_{i+1} = r_i - alpha_i s_i for its rows
 // and, redundantly, for each column its rows touch, and applies A^.  Vectors
-// are FP32 in HBM; scalars, sums and the factors L^-1 are FP64.
+// are FP32 in HBM, updated with fused FP32 multiply-adds, and the row products
+// of A^ accumulate in FP32; the inner products, alpha / beta and the factors
+// L^-1 are FP64.
 constexpr uint32_t kSlotCgGamma = 0;   // gamma_i partials: slot i
 constexpr uint32_t kSlotCgDelta = 72;  // delta_i partials: slot 72 + i
 
-// A^ blocks of the pairs and b^ of the nodes.
-__global__ void __launch_bounds__(256) cg_transform_kernel(const Buffers* __restrict__ bs) {
+// Pair blocks and the transformed system.  The row Gram assembly leaves
+// S_ij = sum w_i w_j lambda^2 u_i^T u_i (21 values) per pair; here
+// B_ij = S_ij M_ij (the change of centre g_i -> g_j: rows e_c x (g_j - g_i) add
+// the translation part to the rotation part) plus the pair blocks of the graph
+// edges joining i and j, in edge order, is stored (J^T J off-diagonal, node
+// frame), and A^_ij = L_i^-1 B_ij L_j^-T (FP64 factors) with it; the nodes get
+// b^ = L^-1 b.
+__global__ void __launch_bounds__(256) cg_transform_kernel(const Buffers* __restrict__ bs, Penalty pen) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
@@ -263,10 +272,89 @@ __global__ void __launch_bounds__(256) cg_transform_kernel(const Buffers* __rest
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < P + N; x += stride) {
         if (x < P) {
-            const uint32_t key = b.pair_key[x];
-            const double* Li = b.linv64 + 36 * (size_t)(key >> 16);
-            const double* Lj = b.linv64 + 36 * (size_t)(key & 0xFFFF);
-            const float* B = b.offdiag + 36 * (size_t)x;
+            const uint32_t p = x;
+            const uint32_t key = b.pair_key[p];
+            const uint32_t i = key >> 16, j = key & 0xFFFF;
+            float* Bp = b.offdiag + 36 * (size_t)p;
+            float v[21];
+#pragma unroll
+            for (int q = 0; q < 21; ++q) v[q] = Bp[q];
+            float S[36];
+            int o = 0;
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+#pragma unroll
+                for (int k2 = a; k2 < 6; ++k2) {
+                    S[6 * a + k2] = v[o];
+                    S[6 * k2 + a] = v[o];
+                    ++o;
+                }
+            const float4 gi = f.nodes[i], gj = f.nodes[j];
+            const float d[3] = {gj.x - gi.x, gj.y - gi.y, gj.z - gi.z};
+            // rows e_c x d of the change of centre (translation -> rotation part)
+            const float K[3][3] = {{0.f, -d[2], d[1]}, {d[2], 0.f, -d[0]}, {-d[1], d[0], 0.f}};
+            float B[36];
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+#pragma unroll
+                for (int k2 = 0; k2 < 6; ++k2) {
+                    float s = S[6 * a + k2];
+                    if (k2 < 3) s += S[6 * a + 3] * K[0][k2] + S[6 * a + 4] * K[1][k2] + S[6 * a + 5] * K[2][k2];
+                    B[6 * a + k2] = s;
+                }
+            auto add_edge = [&](uint32_t e) {
+                float Ji[3][6], Jj[3][6], li[3];
+                const int rows = pair_term_rows(b, c, pen, c.off[0] + e, i, j, Ji, Jj, li);
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    if (r < rows) {
+                        to_node_frame(Ji[r], gi);
+                        to_node_frame(Jj[r], gj);
+#pragma unroll
+                        for (int a = 0; a < 6; ++a)
+#pragma unroll
+                            for (int k2 = 0; k2 < 6; ++k2) B[6 * a + k2] += li[r] * Ji[r][a] * Jj[r][k2];
+                    }
+                }
+            };
+            const uint32_t ne = b.pair_edge_n[p];
+            if (ne <= kPairEdges) {
+                uint32_t ed[kPairEdges];
+#pragma unroll
+                for (int q = 0; q < (int)kPairEdges; ++q)
+                    ed[q] = (uint32_t)q < ne ? b.pair_edge[kPairEdges * (size_t)p + q] : 0xFFFFFFFFu;
+                // edge order (the edges were appended in arbitrary order)
+#pragma unroll
+                for (int q = 1; q < (int)kPairEdges; ++q)
+#pragma unroll
+                    for (int r = q; r > 0; --r)
+                        if (ed[r] < ed[r - 1]) {
+                            const uint32_t tmp = ed[r];
+                            ed[r] = ed[r - 1];
+                            ed[r - 1] = tmp;
+                        }
+                for (uint32_t q = 0; q < ne; ++q) add_edge(ed[q]);
+            } else {
+                // repeated graph edges: row i's Node2Term list holds each edge
+                // term of i once, in edge order
+                const uint32_t last = b.n2t_offset[i + 1];
+                uint32_t lo = b.n2t_offset[i], hi = last;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (b.n2t_index[mid] < c.off[0]) lo = mid + 1;
+                    else hi = mid;
+                }
+                for (uint32_t e = lo; e < last; ++e) {
+                    const uint32_t t = b.n2t_index[e];
+                    if (t >= c.off[1]) break;
+                    const ushort2 ge = f.graph[t - c.off[0]];
+                    if (ge.x != ge.y && (ge.x == j || ge.y == j)) add_edge(t - c.off[0]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 36; ++q) Bp[q] = B[q];
+            const double* Li = b.linv64 + 36 * (size_t)i;
+            const double* Lj = b.linv64 + 36 * (size_t)j;
             double T[36];
 #pragma unroll
             for (int a = 0; a < 6; ++a)
@@ -304,10 +392,10 @@ __global__ void __launch_bounds__(256) cg_transform_kernel(const Buffers* __rest
 
 // y = sum over the off-diagonal blocks of row i of A^ v_j, with v_j = V(j)
 // supplied by the caller; split over the row's kRowLanes lanes and reduced.
-template <typename Load>
-__device__ inline void hat_row_apply(const Buffers& b, uint32_t i, uint32_t sub, const Load& load, double y[6]) {
+template <typename Load, typename Acc = double>
+__device__ inline void hat_row_apply(const Buffers& b, uint32_t i, uint32_t sub, const Load& load, Acc y[6]) {
 #pragma unroll
-    for (int a = 0; a < 6; ++a) y[a] = 0.0;
+    for (int a = 0; a < 6; ++a) y[a] = 0;
     const uint32_t lo_beg = b.low_offset[i], lo_n = b.low_offset[i + 1] - lo_beg;
     const uint32_t up_beg = b.pair_offset[i], up_n = b.pair_offset[i + 1] - up_beg;
     for (uint32_t e = sub; e < lo_n + up_n; e += kRowLanes) {
@@ -327,7 +415,7 @@ __device__ inline void hat_row_apply(const Buffers& b, uint32_t i, uint32_t sub,
         }
         float v[6];
         load(j, v);
-        block_mv<double>(B, tr, v, y);
+        block_mv<Acc>(B, tr, v, y);
     }
 #pragma unroll
     for (int a = 0; a < 6; ++a)
@@ -409,53 +497,47 @@ __global__ void __launch_bounds__(256) cg_iter_kernel(const Buffers* __restrict_
         float* Wo = odd ? b.cg_w[0] : b.cg_w[1];
         float* So = odd ? b.cg_s[1] : b.cg_s[0];
         // r_{i+1} of node j, rounded as it is stored
+        // FP32 vector updates (one rounding each, fused): every row forms its
+        // columns' r_{i+1} exactly as their owners store it
+        const float af = (float)alpha, bf = (float)beta;
         auto next_r = [&](uint32_t j, float v[6]) {
 #pragma unroll
             for (int a = 0; a < 6; ++a) {
                 const size_t k = 6 * (size_t)j + a;
-                const float s_i = (float)((double)W[k] + beta * (double)S[k]);
-                v[a] = (float)((double)R[k] - alpha * (double)s_i);
+                const float s_i = __builtin_fmaf(bf, S[k], W[k]);
+                v[a] = __builtin_fmaf(-af, s_i, R[k]);
             }
         };
         const uint32_t groups = gridDim.x * kRowsPerBlock;
         for (uint32_t base = blockIdx.x * kRowsPerBlock; base < f.N; base += groups) {
             const uint32_t n = base + threadIdx.x / kRowLanes;
-            double y[6];
+            // component a = sub of the row's own vectors (lanes 0..5), loaded
+            // before the row apply so that their latency overlaps it
+            const bool own = n < f.N && sub < 6;
+            const size_t ka = 6 * (size_t)(n < f.N ? n : 0) + (sub < 6 ? sub : 0);
+            float rk = 0.f, wk = 0.f, sk = 0.f, pk = 0.f, xk = 0.f;
+            if (own) {
+                rk = R[ka];
+                wk = W[ka];
+                sk = S[ka];
+                pk = b.cg_p[ka];
+                xk = b.cg_x[ka];
+            }
+            float y[6];
             hat_row_apply(b, n < f.N ? n : 0, sub, next_r, y);
-            if (n < f.N && sub == 0) {
-                // every input first (no aliasing stalls), then the updates
-                float rk[6], wk[6], sk[6], pk[6], xk[6];
-                const size_t k0 = 6 * (size_t)n;
-#pragma unroll
-                for (int a = 0; a < 6; ++a) {
-                    rk[a] = R[k0 + a];
-                    wk[a] = W[k0 + a];
-                    sk[a] = S[k0 + a];
-                    pk[a] = b.cg_p[k0 + a];
-                    xk[a] = b.cg_x[k0 + a];
-                }
-#pragma unroll
-                for (int a = 0; a < 6; ++a) {
-                    const float s_i = (float)((double)wk[a] + beta * (double)sk[a]);
-                    const float r_next = (float)((double)rk[a] - alpha * (double)s_i);
-                    const float p_i = (float)((double)rk[a] + beta * (double)pk[a]);
-                    const double w_next = (double)(float)(y[a] + (double)r_next);
-                    sk[a] = s_i;
-                    rk[a] = r_next;
-                    wk[a] = (float)w_next;
-                    pk[a] = p_i;
-                    xk[a] = (float)((double)xk[a] + alpha * (double)p_i);
-                    g += (double)r_next * (double)r_next;
-                    d += w_next * (double)r_next;
-                }
-#pragma unroll
-                for (int a = 0; a < 6; ++a) {
-                    So[k0 + a] = sk[a];
-                    Ro[k0 + a] = rk[a];
-                    Wo[k0 + a] = wk[a];
-                    b.cg_p[k0 + a] = pk[a];
-                    b.cg_x[k0 + a] = xk[a];
-                }
+            if (own) {
+                const float ya = sub == 0 ? y[0] : sub == 1 ? y[1] : sub == 2 ? y[2] : sub == 3 ? y[3] : sub == 4 ? y[4] : y[5];
+                const float s_i = __builtin_fmaf(bf, sk, wk);
+                const float r_next = __builtin_fmaf(-af, s_i, rk);
+                const float p_i = __builtin_fmaf(bf, pk, rk);
+                const double w_next = (double)(ya + r_next);
+                So[ka] = s_i;
+                Ro[ka] = r_next;
+                Wo[ka] = (float)w_next;
+                b.cg_p[ka] = p_i;
+                b.cg_x[ka] = __builtin_fmaf(af, p_i, xk);
+                g += (double)r_next * (double)r_next;
+                d += w_next * (double)r_next;
             }
         }
     }
@@ -717,11 +799,12 @@ void launch_apply_update(const Batch& B, uint32_t k, hipStream_t stream) {
 
 // Fused CG of the mixed precision mode: transform, init, one launch per
 // iteration, refinement, warp update of GN iteration k.
-void launch_cg_solve(const Batch& B, uint32_t iters, uint32_t refine, uint32_t k, hipStream_t stream) {
+void launch_cg_solve(const Batch& B, uint32_t iters, uint32_t refine, uint32_t k, const Penalty& pen,
+                     hipStream_t stream) {
     const uint32_t sg = spmv_grid(B.nodes);
     iters = std::min(iters, kMaxIters);
     const uint32_t tgrid = std::min<uint32_t>((8 * B.nodes + 255) / 256, 2048u);  // grid-stride over P + N
-    hipLaunchKernelGGL(cg_transform_kernel, dim3(tgrid, B.frames), dim3(256), 0, stream, B.dev);
+    hipLaunchKernelGGL(cg_transform_kernel, dim3(tgrid, B.frames), dim3(256), 0, stream, B.dev, pen);
     hipLaunchKernelGGL(cg_init_kernel, dim3(sg, B.frames), dim3(256), 0, stream, B.dev);
     for (uint32_t i = 0; i < iters; ++i)
         hipLaunchKernelGGL(cg_iter_kernel, dim3(sg, B.frames), dim3(256), 0, stream, B.dev, i);

@@ -191,107 +191,6 @@ __device__ inline void spd_inverse6(const float* A, float* out, double* out64, d
         }
 }
 
-// --------------------------------------------------------- per-term rows
-// Jacobian rows of term t with respect to node n (knn slot weight of n
-// applied), residuals and lambda^2.  Returns the number of rows (1..3).
-__device__ inline int term_rows(const Buffers& b, const Counts& c, const Penalty& pen, uint32_t t, uint32_t n,
-                                float J[3][6], float res[3], float lam2[3]) {
-    uint32_t k;
-    const int type = type_of(c, t, k);
-    if (type == kDense || type == kForeground) {
-        const ScalarRec& rec = type == kDense ? b.dense[k] : b.fg[k];
-        const float w = slot_weight(rec.knn, rec.w, n);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) J[0][q] = w * rec.J[q];
-        res[0] = rec.r;
-        lam2[0] = type == kDense ? 1.0f : pen.foreground * pen.foreground;
-        if (type == kDense && b.density && pen.density != 0.f) {
-            const ScalarRec& d = b.density[k];
-#pragma unroll
-            for (int q = 0; q < 6; ++q) J[1][q] = w * d.J[q];
-            res[1] = d.r;
-            lam2[1] = pen.density * pen.density;
-            return 2;
-        }
-        return 1;
-    }
-    if (type == kSmooth) {
-        const SmoothRec s = b.smooth[k];
-        smooth_rows(s, n == (s.ij & 0xFFFF), J, res);
-        lam2[0] = lam2[1] = lam2[2] = pen.smooth * pen.smooth;
-        return 3;
-    }
-    const P2PRec& rec = type == kFeature ? b.feat[k] : b.cross[k];
-    p2p_rows(rec, J, res);
-    const float w = slot_weight(rec.knn, rec.w, n);
-    const float l = type == kFeature ? pen.feature : pen.cross;
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-#pragma unroll
-        for (int q = 0; q < 6; ++q) J[ch][q] *= w;
-        lam2[ch] = l * l;
-    }
-    return 3;
-}
-
-// Rows of term t for both nodes of the pair (i, j): one record read, one
-// type dispatch.  lam2 holds the i-side lambda^2 (equal on both sides).
-__device__ inline int pair_term_rows(const Buffers& b, const Counts& c, const Penalty& pen, uint32_t t, uint32_t i,
-                                     uint32_t j, float Ji[3][6], float Jj[3][6], float lam2[3]) {
-    uint32_t k;
-    const int type = type_of(c, t, k);
-    float res[3];
-    if (type == kSmooth) {
-        const SmoothRec sm = b.smooth[k];
-        const uint32_t first = sm.ij & 0xFFFF;
-        smooth_rows(sm, i == first, Ji, res);
-        smooth_rows(sm, j == first, Jj, res);
-        lam2[0] = lam2[1] = lam2[2] = pen.smooth * pen.smooth;
-        return 3;
-    }
-    if (type == kDense || type == kForeground) {
-        const ScalarRec& rec = type == kDense ? b.dense[k] : b.fg[k];
-        const ushort4 knn = rec.knn;
-        const float4 w = rec.w;
-        const float wi = slot_weight(knn, w, i), wj = slot_weight(knn, w, j);
-        float J0[6];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) J0[q] = rec.J[q];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            Ji[0][q] = wi * J0[q];
-            Jj[0][q] = wj * J0[q];
-        }
-        lam2[0] = type == kDense ? 1.0f : pen.foreground * pen.foreground;
-        if (type == kDense && b.density && pen.density != 0.f) {
-            const ScalarRec& d = b.density[k];
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                Ji[1][q] = wi * d.J[q];
-                Jj[1][q] = wj * d.J[q];
-            }
-            lam2[1] = pen.density * pen.density;
-            return 2;
-        }
-        return 1;
-    }
-    const P2PRec& rec = type == kFeature ? b.feat[k] : b.cross[k];
-    float J[3][6];
-    p2p_rows(rec, J, res);
-    const float wi = slot_weight(rec.knn, rec.w, i), wj = slot_weight(rec.knn, rec.w, j);
-    const float l = type == kFeature ? pen.feature : pen.cross;
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            Ji[ch][q] = wi * J[ch][q];
-            Jj[ch][q] = wj * J[ch][q];
-        }
-        lam2[ch] = l * l;
-    }
-    return 3;
-}
-
 // ------------------------------------------------- J^T J, J^T r assembly
 // Segmented reductions over the flat index arrays, one wave per 64 entries:
 //   Node2Term entries (node-major)     -> D_n = sum lambda^2 J^T J and
@@ -838,121 +737,34 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
 
 // Fix-up of the row Gram assembly: node rows get the damping and their
 // block-Jacobi inverse with the FP64 Cholesky factor
-// (blockDiagonalInverseKernel, BlockDiagonalPreconditionerInverse.cu:6-41);
-// pair blocks get the change of centre B_ij = S_ij M_ij and the pair blocks of
-// the graph edges joining them, in edge order.
-__global__ void __launch_bounds__(256) gram_fixup_kernel(const Buffers* __restrict__ bs, Penalty pen, int diag_only) {
+// (blockDiagonalInverseKernel, BlockDiagonalPreconditionerInverse.cu:6-41).
+// The pair blocks are finished by the transform that starts the CG
+// (cg_transform_kernel), which needs both nodes' factors.
+__global__ void __launch_bounds__(256) gram_fixup_kernel(const Buffers* __restrict__ bs) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
-    const uint32_t N = f.N, P = diag_only ? 0 : c.pairs;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N + P; x += stride) {
-        if (x < N) {
-            const uint32_t n = x;
-            const float* Dn = b.diag + 36 * (size_t)n;
-            float v[21];
+    for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < f.N; n += stride) {
+        const float* Dn = b.diag + 36 * (size_t)n;
+        float v[21];
 #pragma unroll
-            for (int q = 0; q < 21; ++q) v[q] = Dn[q];
-            float D[36];
-            int o = 0;
+        for (int q = 0; q < 21; ++q) v[q] = Dn[q];
+        float D[36];
+        int o = 0;
 #pragma unroll
-            for (int a = 0; a < 6; ++a)
+        for (int a = 0; a < 6; ++a)
 #pragma unroll
-                for (int k2 = a; k2 < 6; ++k2) {
-                    D[6 * a + k2] = v[o];
-                    D[6 * k2 + a] = v[o];
-                    ++o;
-                }
-            add_node_frame_damping(D, f.nodes[n]);
-#pragma unroll
-            for (int q = 0; q < 36; ++q) b.diag[36 * (size_t)n + q] = D[q];
-            spd_inverse6<double>(D, b.diag_inv + 36 * (size_t)n, b.diag_inv64 + 36 * (size_t)n,
-                                 b.linv64 + 36 * (size_t)n);
-        } else {
-            const uint32_t p = x - N;
-            const uint32_t key = b.pair_key[p];
-            const uint32_t i = key >> 16, j = key & 0xFFFF;
-            float* Bp = b.offdiag + 36 * (size_t)p;
-            float v[21];
-#pragma unroll
-            for (int q = 0; q < 21; ++q) v[q] = Bp[q];
-            float S[36];
-            int o = 0;
-#pragma unroll
-            for (int a = 0; a < 6; ++a)
-#pragma unroll
-                for (int k2 = a; k2 < 6; ++k2) {
-                    S[6 * a + k2] = v[o];
-                    S[6 * k2 + a] = v[o];
-                    ++o;
-                }
-            const float4 gi = f.nodes[i], gj = f.nodes[j];
-            const float d[3] = {gj.x - gi.x, gj.y - gi.y, gj.z - gi.z};
-            // rows e_c x d of the change of centre (translation -> rotation part)
-            const float K[3][3] = {{0.f, -d[2], d[1]}, {d[2], 0.f, -d[0]}, {-d[1], d[0], 0.f}};
-            float B[36];
-#pragma unroll
-            for (int a = 0; a < 6; ++a)
-#pragma unroll
-                for (int k2 = 0; k2 < 6; ++k2) {
-                    float s = S[6 * a + k2];
-                    if (k2 < 3) s += S[6 * a + 3] * K[0][k2] + S[6 * a + 4] * K[1][k2] + S[6 * a + 5] * K[2][k2];
-                    B[6 * a + k2] = s;
-                }
-            auto add_edge = [&](uint32_t e) {
-                float Ji[3][6], Jj[3][6], li[3];
-                const int rows = pair_term_rows(b, c, pen, c.off[0] + e, i, j, Ji, Jj, li);
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    if (r < rows) {
-                        to_node_frame(Ji[r], gi);
-                        to_node_frame(Jj[r], gj);
-#pragma unroll
-                        for (int a = 0; a < 6; ++a)
-#pragma unroll
-                            for (int k2 = 0; k2 < 6; ++k2) B[6 * a + k2] += li[r] * Ji[r][a] * Jj[r][k2];
-                    }
-                }
-            };
-            const uint32_t ne = b.pair_edge_n[p];
-            if (ne <= kPairEdges) {
-                uint32_t ed[kPairEdges];
-#pragma unroll
-                for (int q = 0; q < (int)kPairEdges; ++q)
-                    ed[q] = (uint32_t)q < ne ? b.pair_edge[kPairEdges * (size_t)p + q] : 0xFFFFFFFFu;
-                // edge order (the edges were appended in arbitrary order)
-#pragma unroll
-                for (int q = 1; q < (int)kPairEdges; ++q)
-#pragma unroll
-                    for (int r = q; r > 0; --r)
-                        if (ed[r] < ed[r - 1]) {
-                            const uint32_t tmp = ed[r];
-                            ed[r] = ed[r - 1];
-                            ed[r - 1] = tmp;
-                        }
-                for (uint32_t q = 0; q < ne; ++q) add_edge(ed[q]);
-            } else {
-                // repeated graph edges: row i's Node2Term list holds each edge
-                // term of i once, in edge order
-                const uint32_t last = b.n2t_offset[i + 1];
-                uint32_t lo = b.n2t_offset[i], hi = last;
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (b.n2t_index[mid] < c.off[0]) lo = mid + 1;
-                    else hi = mid;
-                }
-                for (uint32_t e = lo; e < last; ++e) {
-                    const uint32_t t = b.n2t_index[e];
-                    if (t >= c.off[1]) break;
-                    const ushort2 ge = f.graph[t - c.off[0]];
-                    if (ge.x != ge.y && (ge.x == j || ge.y == j)) add_edge(t - c.off[0]);
-                }
+            for (int k2 = a; k2 < 6; ++k2) {
+                D[6 * a + k2] = v[o];
+                D[6 * k2 + a] = v[o];
+                ++o;
             }
+        add_node_frame_damping(D, f.nodes[n]);
 #pragma unroll
-            for (int q = 0; q < 36; ++q) Bp[q] = B[q];
-        }
+        for (int q = 0; q < 36; ++q) b.diag[36 * (size_t)n + q] = D[q];
+        spd_inverse6<double>(D, b.diag_inv + 36 * (size_t)n, b.diag_inv64 + 36 * (size_t)n, b.linv64 + 36 * (size_t)n);
     }
 }
 
@@ -1013,9 +825,8 @@ void launch_assemble(const Batch& B, const Penalty& pen, bool gram, bool diag_on
     const uint32_t fix_blocks = std::min<uint32_t>((8 * B.nodes + 255) / 256, 1024u);  // grid-stride over N + P
     if (gram) {
         const uint32_t rows = (B.nodes + kXcds - 1) / kXcds * kXcds;
-        const int d = diag_only ? 1 : 0;
-        hipLaunchKernelGGL(gram_rows_kernel, dim3(rows, B.frames), dim3(64), 0, stream, B.dev, pen, d);
-        hipLaunchKernelGGL(gram_fixup_kernel, dim3(fix_blocks, B.frames), dim3(256), 0, stream, B.dev, pen, d);
+        hipLaunchKernelGGL(gram_rows_kernel, dim3(rows, B.frames), dim3(64), 0, stream, B.dev, pen, diag_only ? 1 : 0);
+        hipLaunchKernelGGL(gram_fixup_kernel, dim3((B.nodes + 255) / 256, B.frames), dim3(256), 0, stream, B.dev);
     } else {
         hipLaunchKernelGGL(assemble_chunks_kernel, dim3(2048, B.frames), dim3(256), 0, stream, B.dev, pen);
         hipLaunchKernelGGL(assemble_fixup_kernel<float>, dim3(fix_blocks, B.frames), dim3(256), 0, stream, B.dev);

@@ -25,7 +25,8 @@ void launch_pcg_update_once(const Batch& B, bool fp64, hipStream_t stream);
 uint32_t launch_cost(const Batch& B, const Penalty& pen, bool density_in_jtj, hipStream_t stream);
 void launch_pcg(const Batch& B, uint32_t iters, uint32_t refine, bool fp64, hipStream_t stream);
 void launch_apply_update(const Batch& B, uint32_t k, hipStream_t stream);
-void launch_cg_solve(const Batch& B, uint32_t iters, uint32_t refine, uint32_t k, hipStream_t stream);
+void launch_cg_solve(const Batch& B, uint32_t iters, uint32_t refine, uint32_t k, const Penalty& pen,
+                     hipStream_t stream);
 void launch_cg_iter_once(const Batch& B, hipStream_t stream);
 void launch_copy_se3(const Batch& B, hipStream_t stream);
 }  // namespace mvdr
@@ -159,7 +160,7 @@ void enqueue_batch(const Batch& B, const mvdr_solver_config& c, hipStream_t stre
         }
         launch_assemble(B, pen, fp64, false, stream);
         if (fp64) {
-            launch_cg_solve(B, c.pcg_iterations, c.refine_steps, k, stream);
+            launch_cg_solve(B, c.pcg_iterations, c.refine_steps, k, pen, stream);
         } else {
             launch_pcg(B, c.pcg_iterations, 0, false, stream);
             launch_apply_update(B, k, stream);

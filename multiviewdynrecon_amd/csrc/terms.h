@@ -108,4 +108,105 @@ __device__ inline void add_node_frame_damping(float D[36], const float4 g) {
         }
 }
 
+// --------------------------------------------------------- per-term rows
+// Jacobian rows of term t with respect to node n (knn slot weight of n
+// applied), residuals and lambda^2.  Returns the number of rows (1..3).
+__device__ inline int term_rows(const Buffers& b, const Counts& c, const Penalty& pen, uint32_t t, uint32_t n,
+                                float J[3][6], float res[3], float lam2[3]) {
+    uint32_t k;
+    const int type = type_of(c, t, k);
+    if (type == kDense || type == kForeground) {
+        const ScalarRec& rec = type == kDense ? b.dense[k] : b.fg[k];
+        const float w = slot_weight(rec.knn, rec.w, n);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) J[0][q] = w * rec.J[q];
+        res[0] = rec.r;
+        lam2[0] = type == kDense ? 1.0f : pen.foreground * pen.foreground;
+        if (type == kDense && b.density && pen.density != 0.f) {
+            const ScalarRec& d = b.density[k];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) J[1][q] = w * d.J[q];
+            res[1] = d.r;
+            lam2[1] = pen.density * pen.density;
+            return 2;
+        }
+        return 1;
+    }
+    if (type == kSmooth) {
+        const SmoothRec s = b.smooth[k];
+        smooth_rows(s, n == (s.ij & 0xFFFF), J, res);
+        lam2[0] = lam2[1] = lam2[2] = pen.smooth * pen.smooth;
+        return 3;
+    }
+    const P2PRec& rec = type == kFeature ? b.feat[k] : b.cross[k];
+    p2p_rows(rec, J, res);
+    const float w = slot_weight(rec.knn, rec.w, n);
+    const float l = type == kFeature ? pen.feature : pen.cross;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) J[ch][q] *= w;
+        lam2[ch] = l * l;
+    }
+    return 3;
+}
+
+// Rows of term t for both nodes of the pair (i, j): one record read, one
+// type dispatch.  lam2 holds the i-side lambda^2 (equal on both sides).
+__device__ inline int pair_term_rows(const Buffers& b, const Counts& c, const Penalty& pen, uint32_t t, uint32_t i,
+                                     uint32_t j, float Ji[3][6], float Jj[3][6], float lam2[3]) {
+    uint32_t k;
+    const int type = type_of(c, t, k);
+    float res[3];
+    if (type == kSmooth) {
+        const SmoothRec sm = b.smooth[k];
+        const uint32_t first = sm.ij & 0xFFFF;
+        smooth_rows(sm, i == first, Ji, res);
+        smooth_rows(sm, j == first, Jj, res);
+        lam2[0] = lam2[1] = lam2[2] = pen.smooth * pen.smooth;
+        return 3;
+    }
+    if (type == kDense || type == kForeground) {
+        const ScalarRec& rec = type == kDense ? b.dense[k] : b.fg[k];
+        const ushort4 knn = rec.knn;
+        const float4 w = rec.w;
+        const float wi = slot_weight(knn, w, i), wj = slot_weight(knn, w, j);
+        float J0[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) J0[q] = rec.J[q];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            Ji[0][q] = wi * J0[q];
+            Jj[0][q] = wj * J0[q];
+        }
+        lam2[0] = type == kDense ? 1.0f : pen.foreground * pen.foreground;
+        if (type == kDense && b.density && pen.density != 0.f) {
+            const ScalarRec& d = b.density[k];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                Ji[1][q] = wi * d.J[q];
+                Jj[1][q] = wj * d.J[q];
+            }
+            lam2[1] = pen.density * pen.density;
+            return 2;
+        }
+        return 1;
+    }
+    const P2PRec& rec = type == kFeature ? b.feat[k] : b.cross[k];
+    float J[3][6];
+    p2p_rows(rec, J, res);
+    const float wi = slot_weight(rec.knn, rec.w, i), wj = slot_weight(rec.knn, rec.w, j);
+    const float l = type == kFeature ? pen.feature : pen.cross;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            Ji[ch][q] = wi * J[ch][q];
+            Jj[ch][q] = wj * J[ch][q];
+        }
+        lam2[ch] = l * l;
+    }
+    return 3;
+}
+
 }  // namespace mvdr
