@@ -327,6 +327,7 @@ struct Buffers {
     uint32_t* kv2_pair;      // kv2: the pair of each entry
     uint32_t* pair_edge_n;   // cap_pairs: graph edges joining the pair (row Gram assembly)
     uint32_t* pair_edge;     // kPairEdges x cap_pairs: their edge indices
+    uint16_t* row_part;      // N x kRowPartners: each row's partners ascending, upper then lower
     float4* mf_u;            // terms: u_t of the matrix-free apply
     // assembly partials of runs crossing 64-entry chunk boundaries
     float* part_diag;        // (kv1 / 64 + 1) x 2 x 27
@@ -377,6 +378,9 @@ struct Batch {
 
 // Index variants (launch_index_build).
 enum : int { kIndexLists = 0, kIndexGram = 1, kIndexNodes = 2 };
+
+// Partner nodes a row may have (upper and lower pairs together).
+constexpr uint32_t kRowPartners = 512;
 
 // Graph edges per node pair the row Gram assembly can hold.
 constexpr uint32_t kPairEdges = 4;

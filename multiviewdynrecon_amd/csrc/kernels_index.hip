@@ -570,7 +570,7 @@ __global__ void __launch_bounds__(1024) node_scan_kernel(const Buffers* __restri
 // One wave per row i collects the partner nodes of its Node2Term list in an LDS
 // hash set, then emits them ascending (wave minimum per step).  The blocks are
 // single waves, so LDS traffic between lanes needs no workgroup barrier.
-constexpr uint32_t kRowHash = 512;  // partner set capacity per row
+constexpr uint32_t kRowHash = kRowPartners;  // partner set capacity per row
 
 // Partners of node i in term t with the reference's pair multiplicity: every
 // knn slot q with knn[q] != i, or the other end of a non-loop graph edge.
@@ -593,18 +593,18 @@ __device__ inline uint32_t lower_bound_lds(const uint32_t* a, uint32_t n, uint32
 }
 
 // kRowCount: pairs above / below the diagonal and pair-term entries of row i.
-// kRowKeys: the row's pair keys and lower entries, a cleared graph-edge count
-// per pair, and for every Node2Term entry the weight column of each knn slot
-// holding an upper partner j (2 + rank of j among the row's upper partners;
-// 0xFF otherwise) -- what the row Gram assembly needs instead of pair-term
-// lists.
+// kRowGram: the same counts, the row's partner lists (row_part, emitted at
+// their offsets by pair_emit_kernel once the counts are scanned), and for every
+// Node2Term entry the weight column of each knn slot holding an upper partner
+// j (2 + rank of j among the row's upper partners; 0xFF otherwise) -- what the
+// row Gram assembly needs instead of pair-term lists.
 // kRowTerms: pair keys, lower entries, and the pair-term lists (NodePair2Term)
 // with their ranges: for pair (i, j) every Node2Term entry of row i once per
 // knn slot holding j, in Node2Term order -- the order of the reference's stable
 // key sort.  kRowWaves waves per row; wave w places a contiguous w-th share of
 // the row's entries.
 constexpr uint32_t kRowWaves = 4;
-enum : int { kRowCount = 0, kRowKeys = 1, kRowTerms = 2 };
+enum : int { kRowCount = 0, kRowGram = 1, kRowTerms = 2 };
 
 template <int kMode>
 __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers* __restrict__ bs) {
@@ -687,7 +687,7 @@ __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers
     }
     __syncthreads();
     const uint32_t up = nup, low = nlow;
-    if (kMode == kRowCount) {
+    if (kMode == kRowCount || kMode == kRowGram) {
         if (tid == 0) {
             b.row_up[i] = up;
             b.row_low[i] = low;
@@ -695,9 +695,10 @@ __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers
             for (uint32_t k = 0; k < kRowWaves; ++k) s2 += part[k];
             b.row_kv2[i] = s2;
         }
-        return;
-    }
-    if (kMode == kRowKeys) {
+        if (kMode == kRowCount) return;
+        uint16_t* rp = b.row_part + (size_t)i * kRowHash;
+        for (uint32_t u = tid; u < up; u += 64 * kRowWaves) rp[u] = (uint16_t)ul[u];
+        for (uint32_t k = tid; k < low; k += 64 * kRowWaves) rp[up + k] = (uint16_t)ll[k];
         for (uint32_t p = beg + tid; p < end; p += 64 * kRowWaves) {
             ushort4 k;
             const int m = term_nodes(b, f, c.off, b.n2t_index[p], k);
@@ -713,6 +714,7 @@ __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers
             }
             b.n2t_col[p] = packed;
         }
+        return;
     }
     const uint32_t up_base = b.pair_offset[i], low_base = b.low_offset[i];
     const uint32_t len = end - beg, per = (len + kRowWaves - 1) / kRowWaves;
@@ -793,11 +795,30 @@ __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers
             }
         }
     }
-    for (uint32_t u = tid; u < up; u += 64 * kRowWaves) {
-        b.pair_key[up_base + u] = (i << 16) | ul[u];
-        if (kMode == kRowKeys) b.pair_edge_n[up_base + u] = 0;
-    }
+    for (uint32_t u = tid; u < up; u += 64 * kRowWaves) b.pair_key[up_base + u] = (i << 16) | ul[u];
     for (uint32_t k = tid; k < low; k += 64 * kRowWaves) b.low_ent[low_base + k] = make_uint2(ll[k], 0u);
+}
+
+// Pair keys and lower entries of every row at their scanned offsets, from the
+// partner lists kRowGram left in row_part; graph-edge counts cleared.  A
+// quarter wave per row.
+__global__ void __launch_bounds__(256) pair_emit_kernel(const Buffers* __restrict__ bs) {
+    const Buffers b = frame_buffers(bs);
+    const FrameDev& f = *b.frame;
+    const Counts& c = *b.counts;
+    if (c.error) return;
+    const uint32_t sub = threadIdx.x & 15;
+    const uint32_t groups = gridDim.x * 16;
+    for (uint32_t i = blockIdx.x * 16 + threadIdx.x / 16; i < f.N; i += groups) {
+        const uint32_t up = b.row_up[i], low = b.row_low[i];
+        const uint32_t up_base = b.pair_offset[i], low_base = b.low_offset[i];
+        const uint16_t* rp = b.row_part + (size_t)i * kRowHash;
+        for (uint32_t u = sub; u < up; u += 16) {
+            b.pair_key[up_base + u] = (i << 16) | rp[u];
+            b.pair_edge_n[up_base + u] = 0;
+        }
+        for (uint32_t k = sub; k < low; k += 16) b.low_ent[low_base + k] = make_uint2(rp[up + k], 0u);
+    }
 }
 
 // Graph edges of every pair (i, j): the row Gram assembly adds their pair
@@ -882,12 +903,15 @@ void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, int
     hipLaunchKernelGGL(n2t_place_kernel, dim3(B.cap_chunks, F), dim3(64), tile * sizeof(uint32_t), stream, B.dev, tile);
     if (mode == kIndexNodes) return;
     const uint32_t row_grid = (B.nodes + kXcds - 1) / kXcds * kXcds;
-    hipLaunchKernelGGL(pair_rows_kernel<kRowCount>, dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
-    hipLaunchKernelGGL(node_scan_kernel, dim3(3, F), dim3(1024), 0, stream, B.dev, 1);
-    if (gram)
-        hipLaunchKernelGGL(pair_rows_kernel<kRowKeys>, dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
-    else
+    if (gram) {
+        hipLaunchKernelGGL(pair_rows_kernel<kRowGram>, dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
+        hipLaunchKernelGGL(node_scan_kernel, dim3(3, F), dim3(1024), 0, stream, B.dev, 1);
+        hipLaunchKernelGGL(pair_emit_kernel, dim3((B.nodes + 15) / 16, F), dim3(256), 0, stream, B.dev);
+    } else {
+        hipLaunchKernelGGL(pair_rows_kernel<kRowCount>, dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
+        hipLaunchKernelGGL(node_scan_kernel, dim3(3, F), dim3(1024), 0, stream, B.dev, 1);
         hipLaunchKernelGGL(pair_rows_kernel<kRowTerms>, dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
+    }
     hipLaunchKernelGGL(pair_link_kernel, dim3((B.nodes + 255) / 256, F), dim3(256), 0, stream, B.dev);
     if (gram) hipLaunchKernelGGL(pair_edges_kernel, dim3((8 * B.nodes + 255) / 256, F), dim3(256), 0, stream, B.dev);
 }

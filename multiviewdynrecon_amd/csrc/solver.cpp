@@ -379,11 +379,13 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     b.part_diag = b.part_pair = nullptr;
     const bool mf = c.jtj_mode == MVDR_JTJ_MATRIX_FREE;
     b.pair_edge_n = b.pair_edge = b.n2t_col = nullptr;
+    b.row_part = nullptr;
     b.mf_u = nullptr;
     if (mf) {
         A(b.mf_u, knn_terms + b.cap_edges);
     } else if (c.precision == MVDR_PRECISION_MIXED) {
         A(b.n2t_col, b.cap_kv1);
+        A(b.row_part, (size_t)kRowPartners * c.max_nodes);
         A(b.pair_edge_n, b.cap_pairs);
         A(b.pair_edge, (size_t)kPairEdges * b.cap_pairs);
     } else {
