@@ -644,8 +644,13 @@ __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers
                     overflow = true;
                     break;
                 }
-                const uint32_t old = atomicCAS(&set[h], 0xFFFFFFFFu, v);
-                if (old == 0xFFFFFFFFu || old == v) break;
+                // most partners are already present: a plain read settles them
+                const uint32_t seen = set[h];
+                if (seen == v) break;
+                if (seen == 0xFFFFFFFFu) {
+                    const uint32_t old = atomicCAS(&set[h], 0xFFFFFFFFu, v);
+                    if (old == 0xFFFFFFFFu || old == v) break;
+                }
                 h = (h + 1) & (kRowHash - 1);
             }
         }
