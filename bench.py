@@ -132,6 +132,27 @@ def cpu_baseline_leg(scene, workload, frames):
     return cpu
 
 
+def cpu_baseline_threads(scene, workload, threads):
+    """The same CPU restatement with one frame per thread (frames are independent;
+    the oracle's calls release the GIL): aggregate ms per frame over 2 frames per
+    thread."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    Oracle = _oracle()
+    from multiviewdynrecon_amd.config import workload_config
+    from multiviewdynrecon_amd.frame import HostFrame
+
+    hf = HostFrame(scene)
+    cfg32 = workload_config(workload, precision=0, refine_steps=0)
+    oracles = [Oracle(cfg32, hf, "fp32") for _ in range(2 * threads)]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(lambda o: o.solve(), oracles))
+    wall = time.perf_counter() - t0
+    return {"value": float(wall * 1e3 / len(oracles)), "unit": "ms/frame", "cores": threads, "kind": "port",
+            "sample": f"{len(oracles)} full solves of the benchmark frame on {threads} threads, FP32, aggregate"}
+
+
 def twist_errors(scene, cfg, frame_dev):
     """Per-GN-iteration twist error of the GPU path against the FP64 oracle from
     the same linearisation point (the oracle is the checker only)."""
@@ -369,11 +390,14 @@ def main():
     del plain
     print(f"[bench] plain FP32 path {plain_ms:.3f} ms/frame", file=sys.stderr, flush=True)
 
-    cpu = None
+    cpu = cpu_mt = None
     errs = plain_errs = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline_leg(scene, args.workload, args.cpu_frames)
         print(f"[bench] cpu baseline {cpu['value']:.1f} ms/frame", file=sys.stderr, flush=True)
+        cpu_mt = cpu_baseline_threads(scene, args.workload, min(16, os.cpu_count() or 1))
+        print(f"[bench] cpu baseline, {cpu_mt['cores']} threads: {cpu_mt['value']:.1f} ms/frame", file=sys.stderr,
+              flush=True)
         errs = twist_errors(scene, cfg, frame)
         plain_errs = twist_errors(scene, plain_cfg, frame)
         print(f"[bench] twist rel err per GN iteration: {args.precision} {errs} plain {plain_errs}",
@@ -406,6 +430,7 @@ def main():
         "jtj_assembly_gbs": round(jtj_gbs, 2),
         "stage_ms": {k: round(v, 5) for k, v in stages.items()},
         "cpu_baseline": cpu,
+        "cpu_baseline_threads": cpu_mt,
         "twist_rel_err": max(errs) if errs else None,
         "plain_fp32": {"ms_per_frame": round(plain_ms, 4), "speedup": round(plain_ms / ms_per_frame, 3),
                        "twist_rel_err": max(plain_errs) if plain_errs else None},
