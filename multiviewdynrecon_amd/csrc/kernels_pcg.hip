@@ -425,7 +425,8 @@ __device__ inline void hat_row_apply(const Buffers& b, uint32_t i, uint32_t sub,
 
 // w_0 = A^ r_0 (r_0 = b^), x^ = p_{-1} = s_{-1} = 0, partials of gamma_0, delta_0.
 __global__ void __launch_bounds__(256) cg_init_kernel(const Buffers* __restrict__ bs) {
-    const Buffers b = frame_buffers(bs);
+    const WorkItem wi = frame_affine();
+    const Buffers b = frame_buffers_at(bs, wi.frame);
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     double g = 0.0, d = 0.0;
@@ -436,7 +437,7 @@ __global__ void __launch_bounds__(256) cg_init_kernel(const Buffers* __restrict_
             for (int a = 0; a < 6; ++a) v[a] = R[6 * (size_t)j + a];
         };
         const uint32_t groups = gridDim.x * kRowsPerBlock;
-        for (uint32_t base = blockIdx.x * kRowsPerBlock; base < f.N; base += groups) {
+        for (uint32_t base = wi.block * kRowsPerBlock; base < f.N; base += groups) {
             const uint32_t i = base + threadIdx.x / kRowLanes;
             double y[6];
             hat_row_apply(b, i < f.N ? i : 0, sub, load, y);
@@ -456,13 +457,14 @@ __global__ void __launch_bounds__(256) cg_init_kernel(const Buffers* __restrict_
             }
         }
     }
-    write_partial<double>(b, kSlotCgGamma + 0, g);
-    write_partial<double>(b, kSlotCgDelta + 0, d);
+    write_partial<double>(b, kSlotCgGamma + 0, g, wi.block);
+    write_partial<double>(b, kSlotCgDelta + 0, d, wi.block);
 }
 
 // alpha_i, beta_i from the partials of gamma_i, delta_i and the stored
 // alpha_{i-1}, gamma_{i-1}; block 0 stores alpha_i, gamma_i for launch i + 1.
-__device__ inline void cg_scalars(const Buffers& b, uint32_t i, uint32_t blocks, double& alpha, double& beta) {
+__device__ inline void cg_scalars(const Buffers& b, uint32_t i, uint32_t blocks, double& alpha, double& beta,
+                                  uint32_t block) {
     const double gamma = reduce_partials<double>(slot_ptr(b, kSlotCgGamma + i), blocks);
     const double delta = reduce_partials<double>(slot_ptr(b, kSlotCgDelta + i), blocks);
     beta = 0.0;
@@ -473,20 +475,25 @@ __device__ inline void cg_scalars(const Buffers& b, uint32_t i, uint32_t blocks,
         den = alpha_prev != 0.0 ? delta - beta * gamma / alpha_prev : 0.0;
     }
     alpha = den != 0.0 ? gamma / den : 0.0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (block == 0 && threadIdx.x == 0) {
         b.cg_scal[2 * i] = alpha;
         b.cg_scal[2 * i + 1] = gamma;
     }
 }
 
+// The row kernels of the fused CG (init, iterations, refinement) keep every
+// workgroup of a frame on one XCD (frame_affine), so a frame's A^ blocks and
+// vectors stay in that XCD's L2 from one launch to the next.
+
 // Launch i of the fused CG (see above).  Reads r_i, w_i, s_{i-1}; writes
 // r_{i+1}, w_{i+1}, s_i, p_i, x^_{i+1}.
 __global__ void __launch_bounds__(256) cg_iter_kernel(const Buffers* __restrict__ bs, uint32_t i) {
-    const Buffers b = frame_buffers(bs);
+    const WorkItem wi = frame_affine();
+    const Buffers b = frame_buffers_at(bs, wi.frame);
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     double alpha, beta;
-    cg_scalars(b, i, gridDim.x, alpha, beta);
+    cg_scalars(b, i, gridDim.x, alpha, beta, wi.block);
     double g = 0.0, d = 0.0;
     if (!b.counts->error) {
         const bool odd = (i & 1) != 0;
@@ -509,7 +516,7 @@ __global__ void __launch_bounds__(256) cg_iter_kernel(const Buffers* __restrict_
             }
         };
         const uint32_t groups = gridDim.x * kRowsPerBlock;
-        for (uint32_t base = blockIdx.x * kRowsPerBlock; base < f.N; base += groups) {
+        for (uint32_t base = wi.block * kRowsPerBlock; base < f.N; base += groups) {
             const uint32_t n = base + threadIdx.x / kRowLanes;
             // component a = sub of the row's own vectors (lanes 0..5), loaded
             // before the row apply so that their latency overlaps it
@@ -541,15 +548,16 @@ __global__ void __launch_bounds__(256) cg_iter_kernel(const Buffers* __restrict_
             }
         }
     }
-    write_partial<double>(b, kSlotCgGamma + i + 1, g);
-    write_partial<double>(b, kSlotCgDelta + i + 1, d);
+    write_partial<double>(b, kSlotCgGamma + i + 1, g, wi.block);
+    write_partial<double>(b, kSlotCgDelta + i + 1, d, wi.block);
 }
 
 // Residual refinement on the CG solution (preconditioned steepest descent on
 // the true residual, as refine_* above, in the A^ coordinates):
 // r^ = b^ - A^ x^, num = (r^, r^) ...
 __global__ void __launch_bounds__(256) cg_refine_residual_kernel(const Buffers* __restrict__ bs) {
-    const Buffers b = frame_buffers(bs);
+    const WorkItem wi = frame_affine();
+    const Buffers b = frame_buffers_at(bs, wi.frame);
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     double acc = 0.0;
@@ -560,7 +568,7 @@ __global__ void __launch_bounds__(256) cg_refine_residual_kernel(const Buffers* 
             for (int a = 0; a < 6; ++a) v[a] = X[6 * (size_t)j + a];
         };
         const uint32_t groups = gridDim.x * kRowsPerBlock;
-        for (uint32_t base = blockIdx.x * kRowsPerBlock; base < f.N; base += groups) {
+        for (uint32_t base = wi.block * kRowsPerBlock; base < f.N; base += groups) {
             const uint32_t n = base + threadIdx.x / kRowLanes;
             double y[6];
             hat_row_apply(b, n < f.N ? n : 0, sub, load, y);
@@ -575,12 +583,13 @@ __global__ void __launch_bounds__(256) cg_refine_residual_kernel(const Buffers* 
             }
         }
     }
-    write_partial<double>(b, kSlotRefNum, acc);
+    write_partial<double>(b, kSlotRefNum, acc, wi.block);
 }
 
 // ... den = (r^, A^ r^) ...
 __global__ void __launch_bounds__(256) cg_refine_curvature_kernel(const Buffers* __restrict__ bs) {
-    const Buffers b = frame_buffers(bs);
+    const WorkItem wi = frame_affine();
+    const Buffers b = frame_buffers_at(bs, wi.frame);
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kRowLanes;
     double acc = 0.0;
@@ -591,7 +600,7 @@ __global__ void __launch_bounds__(256) cg_refine_curvature_kernel(const Buffers*
             for (int a = 0; a < 6; ++a) v[a] = Q[6 * (size_t)j + a];
         };
         const uint32_t groups = gridDim.x * kRowsPerBlock;
-        for (uint32_t base = blockIdx.x * kRowsPerBlock; base < f.N; base += groups) {
+        for (uint32_t base = wi.block * kRowsPerBlock; base < f.N; base += groups) {
             const uint32_t n = base + threadIdx.x / kRowLanes;
             double y[6];
             hat_row_apply(b, n < f.N ? n : 0, sub, load, y);
@@ -604,7 +613,7 @@ __global__ void __launch_bounds__(256) cg_refine_curvature_kernel(const Buffers*
             }
         }
     }
-    write_partial<double>(b, kSlotRefDen, acc);
+    write_partial<double>(b, kSlotRefDen, acc, wi.block);
 }
 
 // ... x^ += (num / den) r^ (all refinement steps but the last; the last is

@@ -20,6 +20,27 @@ constexpr uint32_t kMaxIters = 64;
 
 __device__ inline double* slot_ptr(const Buffers& b, uint32_t slot) { return b.partial + (size_t)slot * kPartialStride; }
 
+// Frame-to-XCD affinity for a grid (blocks, frames) whose frame count is a
+// multiple of the XCD count: every workgroup of a frame runs on one XCD
+// (workgroup id mod 8), so the frame's blocks and vectors share that L2.
+struct WorkItem {
+    uint32_t frame, block;
+};
+__device__ inline WorkItem frame_affine() {
+    const uint32_t F = gridDim.y, blocks = gridDim.x;
+    if (F % kXcds != 0) return WorkItem{blockIdx.y, blockIdx.x};
+    const uint32_t L = blockIdx.x + blocks * blockIdx.y;
+    const uint32_t x = L % kXcds, r = L / kXcds;
+    return WorkItem{x + kXcds * (r / blocks), r % blocks};
+}
+__device__ inline Buffers frame_buffers_at(const Buffers* bs, uint32_t frame) {
+#if __HIP_DEVICE_COMPILE__
+    return *((const __attribute__((address_space(4))) Buffers*)bs + frame);
+#else
+    return bs[frame];
+#endif
+}
+
 // Sum of `count` partials, computed identically by every block (fixed order).
 template <typename Acc>
 __device__ inline Acc reduce_partials(const double* p, uint32_t count) {
@@ -39,12 +60,16 @@ __device__ inline Acc reduce_partials(const double* p, uint32_t count) {
 
 // Block sum of per-thread values, stored as this block's partial of `slot`.
 template <typename Acc>
-__device__ inline void write_partial(const Buffers& b, uint32_t slot, Acc value) {
+__device__ inline void write_partial(const Buffers& b, uint32_t slot, Acc value, uint32_t block) {
     __shared__ Acc part[4];
     value = wave_sum(value);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = value;
     __syncthreads();
-    if (threadIdx.x == 0) slot_ptr(b, slot)[blockIdx.x] = (double)(part[0] + part[1] + part[2] + part[3]);
+    if (threadIdx.x == 0) slot_ptr(b, slot)[block] = (double)(part[0] + part[1] + part[2] + part[3]);
+}
+template <typename Acc>
+__device__ inline void write_partial(const Buffers& b, uint32_t slot, Acc value) {
+    write_partial<Acc>(b, slot, value, blockIdx.x);
 }
 
 // 6x6 row-major block times a 6-vector (or its transpose), accumulated in Acc.

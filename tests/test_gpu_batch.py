@@ -30,17 +30,21 @@ def frames():
     return [DeviceFrame(s) for s in scenes]
 
 
-@pytest.mark.parametrize("precision", ["mixed", "fp32"])
-def test_batch_members_equal_single_solves(frames, precision):
+@pytest.mark.parametrize("precision,count", [("mixed", 4), ("fp32", 4), ("mixed", 8)])
+def test_batch_members_equal_single_solves(frames, precision, count):
+    """count = 8: a frame count divisible by the XCD count, where the CG row
+    kernels keep each frame's workgroups on one XCD."""
     cfg = shared_config(precision)
+    members_frames = [frames[k % len(frames)] for k in range(count)]
     single = []
     for fr in frames:
         s = NonRigidSolver(cfg)
         s.SetSolverInput(fr)
         s.SolveNonRigidAlignment()
         single.append((s.solved_node_se3_host(), s.debug_get("twist"), s.sizes()))
+    single = [single[k % len(frames)] for k in range(count)]
     members = []
-    for fr in frames:
+    for fr in members_frames:
         s = NonRigidSolver(cfg)
         s.SetSolverInput(fr)
         members.append(s)
