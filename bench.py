@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=8, help="frames in the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage-reps", type=int, default=50)
-    ap.add_argument("--batch", type=int, default=16, help="frames solved per step (one launch per stage)")
+    ap.add_argument("--batch", type=int, default=32, help="frames solved per step (one launch per stage)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first: the known-size kernel that "
                          "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE counters")

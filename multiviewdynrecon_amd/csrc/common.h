@@ -344,7 +344,7 @@ struct Buffers {
     // PCG
     float *x, *r, *s, *q, *d0, *d1;
     // fused CG on A^ = L^-1 A L^-T (mixed precision mode)
-    double* linv64;          // 36 N: L^-1 of each diagonal block, row-major lower
+    double* linv64;          // 21 N: L^-1 of each diagonal block, packed lower triangle (tri_index)
     float* offhat;           // 36 P: L_i^-1 B_ij L_j^-T
     float *cg_r[2], *cg_w[2], *cg_s[2], *cg_p, *cg_x, *cg_b;
     double* cg_scal;         // alpha_i, gamma_i per iteration
@@ -378,6 +378,10 @@ struct Batch {
 
 // Index variants (launch_index_build).
 enum : int { kIndexLists = 0, kIndexGram = 1, kIndexNodes = 2 };
+
+// Packed row-major lower triangle of a 6x6 matrix: entry (a, k), k <= a.
+__host__ __device__ constexpr uint32_t tri_index(uint32_t a, uint32_t k) { return a * (a + 1) / 2 + k; }
+constexpr uint32_t kTri = 21;
 
 // Partner nodes a row may have (upper and lower pairs together).
 constexpr uint32_t kRowPartners = 512;

@@ -353,8 +353,8 @@ __global__ void __launch_bounds__(256) cg_transform_kernel(const Buffers* __rest
             }
 #pragma unroll
             for (int q = 0; q < 36; ++q) Bp[q] = B[q];
-            const double* Li = b.linv64 + 36 * (size_t)i;
-            const double* Lj = b.linv64 + 36 * (size_t)j;
+            const double* Li = b.linv64 + kTri * (size_t)i;
+            const double* Lj = b.linv64 + kTri * (size_t)j;
             double T[36];
 #pragma unroll
             for (int a = 0; a < 6; ++a)
@@ -362,7 +362,7 @@ __global__ void __launch_bounds__(256) cg_transform_kernel(const Buffers* __rest
                 for (int cc = 0; cc < 6; ++cc) {
                     double t = 0.0;
 #pragma unroll
-                    for (int k = 0; k <= a; ++k) t += Li[6 * a + k] * (double)B[6 * k + cc];
+                    for (int k = 0; k <= a; ++k) t += Li[tri_index(a, k)] * (double)B[6 * k + cc];
                     T[6 * a + cc] = t;
                 }
             float* out = b.offhat + 36 * (size_t)x;
@@ -372,17 +372,17 @@ __global__ void __launch_bounds__(256) cg_transform_kernel(const Buffers* __rest
                 for (int bb = 0; bb < 6; ++bb) {
                     double t = 0.0;
 #pragma unroll
-                    for (int cc = 0; cc <= bb; ++cc) t += T[6 * a + cc] * Lj[6 * bb + cc];
+                    for (int cc = 0; cc <= bb; ++cc) t += T[6 * a + cc] * Lj[tri_index(bb, cc)];
                     out[6 * a + bb] = (float)t;
                 }
         } else {
             const uint32_t n = x - P;
-            const double* L = b.linv64 + 36 * (size_t)n;
+            const double* L = b.linv64 + kTri * (size_t)n;
 #pragma unroll
             for (int a = 0; a < 6; ++a) {
                 double t = 0.0;
 #pragma unroll
-                for (int k = 0; k <= a; ++k) t += L[6 * a + k] * (double)b.jtr[6 * (size_t)n + k];
+                for (int k = 0; k <= a; ++k) t += L[tri_index(a, k)] * (double)b.jtr[6 * (size_t)n + k];
                 b.cg_b[6 * (size_t)n + a] = (float)t;
                 b.cg_r[0][6 * (size_t)n + a] = (float)t;
             }
@@ -708,12 +708,12 @@ __global__ void __launch_bounds__(256) apply_update_kernel(const Buffers* __rest
                 const size_t k = 6 * (size_t)n + a;
                 xh[a] = (double)b.cg_x[k] + (refine_blocks ? step * (double)b.q[k] : 0.0);
             }
-            const double* L = b.linv64 + 36 * (size_t)n;
+            const double* L = b.linv64 + kTri * (size_t)n;
 #pragma unroll
             for (int a = 0; a < 6; ++a) {
                 double t = 0.0;
 #pragma unroll
-                for (int k = a; k < 6; ++k) t += L[6 * k + a] * xh[k];
+                for (int k = a; k < 6; ++k) t += L[tri_index(k, a)] * xh[k];
                 xi[a] = (float)t;
             }
         }

@@ -180,7 +180,8 @@ __device__ inline void spd_inverse6(const float* A, float* out, double* out64, d
         }
 #undef LI
     if (linv64)
-        for (int q = 0; q < 36; ++q) linv64[q] = (double)X[q];
+        for (int a = 0; a < 6; ++a)
+            for (int k = 0; k <= a; ++k) linv64[tri_index(a, k)] = (double)X[6 * a + k];
     // inv = X^T X
     for (int r = 0; r < 6; ++r)
         for (int col = 0; col < 6; ++col) {
@@ -423,7 +424,7 @@ __global__ void __launch_bounds__(256) assemble_fixup_kernel(const Buffers* __re
             for (int q = 0; q < 6; ++q) b.jtr[6 * (size_t)n + q] = jtr[q];
             spd_inverse6<InvT>(D, b.diag_inv + 36 * (size_t)n,
                                sizeof(InvT) == 8 ? b.diag_inv64 + 36 * (size_t)n : nullptr,
-                               sizeof(InvT) == 8 ? b.linv64 + 36 * (size_t)n : nullptr);
+                               sizeof(InvT) == 8 ? b.linv64 + kTri * (size_t)n : nullptr);
         } else {
             const uint32_t p = x - N;
             const uint2 r = b.pair_range[p];
@@ -583,11 +584,18 @@ __device__ inline void gram_entry(const Buffers& b, const TermBounds& c, const L
     const bool knn_term = type != kSmooth;
     wi = knn_term ? slot_weight(knn, w, i) : 1.f;
     const uint32_t n[4] = {knn.x, knn.y, knn.z, knn.w};
+    float ws[4] = {w.x, w.y, w.z, w.w};  // distinct knn: the slot rule gives knn[q] the weight w[q]
+    const bool distinct = knn.x != knn.y && knn.x != knn.z && knn.x != knn.w && knn.y != knn.z && knn.y != knn.w &&
+                          knn.z != knn.w;
+    if (!distinct) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ws[q] = slot_weight(knn, w, n[q]);
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const bool up = knn_term && n[q] > i && n[q] != kNoNode;
         nq[q] = up ? n[q] : kNoNode;
-        wq[q] = up ? wi * slot_weight(knn, w, n[q]) : 0.f;
+        wq[q] = up ? wi * ws[q] : 0.f;
     }
 }
 
@@ -764,7 +772,7 @@ __global__ void __launch_bounds__(256) gram_fixup_kernel(const Buffers* __restri
         add_node_frame_damping(D, f.nodes[n]);
 #pragma unroll
         for (int q = 0; q < 36; ++q) b.diag[36 * (size_t)n + q] = D[q];
-        spd_inverse6<double>(D, b.diag_inv + 36 * (size_t)n, b.diag_inv64 + 36 * (size_t)n, b.linv64 + 36 * (size_t)n);
+        spd_inverse6<double>(D, b.diag_inv + 36 * (size_t)n, b.diag_inv64 + 36 * (size_t)n, b.linv64 + kTri * (size_t)n);
     }
 }
 

@@ -406,7 +406,7 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     b.linv64 = nullptr;
     b.offhat = nullptr;
     b.cg_scal = nullptr;
-    if (factors64) A(b.linv64, 36 * (size_t)c.max_nodes);
+    if (factors64) A(b.linv64, kTri * (size_t)c.max_nodes);
     if (c.precision == MVDR_PRECISION_MIXED && !mf) {
         A(b.offhat, 36 * (size_t)b.cap_pairs);
         for (float** v : {&b.cg_r[0], &b.cg_r[1], &b.cg_w[0], &b.cg_w[1], &b.cg_s[0], &b.cg_s[1], &b.cg_p, &b.cg_x,
