@@ -605,11 +605,17 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
-    // term-major rows of 16: G_t values 0..15, values 16..26 (+ 5 zeros), weights
-    __shared__ float Glo[64 * 16];
-    __shared__ float Ghi[64 * 16];
-    __shared__ float Wl[64 * 16];
+    // term-major rows: G_t values 0..15, values 16..26 (+ 3 zeros), the weight
+    // row.  The row strides (18, 14, 18 words) keep the lanes' float2 stores of
+    // their own rows and the product's column reads spread over the banks; a
+    // read of values 30..31 lands in the next row (or the zeroed tail) and only
+    // feeds product rows that are never used.
+    constexpr uint32_t kLo = 18, kHi = 14, kW = 18;
+    __shared__ float Glo[64 * kLo];
+    __shared__ float Ghi[64 * kHi + 2];
+    __shared__ float Wl[64 * kW];
     const uint32_t lane = threadIdx.x;
+    if (lane < 2) Ghi[64 * kHi + lane] = 0.f;
     const uint32_t m = lane & 15, kq = lane >> 4;
     const Lambda2 l2(pen);
     const bool density = b.density && pen.density != 0.f;
@@ -667,18 +673,18 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
                 const uint32_t enn = e0 + 128 + lane;
                 const uint32_t t_nn = enn < end ? b.n2t_index[enn] : 0xFFFFFFFFu;
                 const uint32_t pc_nn = cols_in && enn < end ? b.n2t_col[enn] : 0xFFFFFFFFu;
-                float* gl = Glo + 16 * lane;
-                float* gh = Ghi + 16 * lane;
-                float* wl = Wl + 16 * lane;
+                float* gl = Glo + kLo * lane;
+                float* gh = Ghi + kHi * lane;
+                float* wl = Wl + kW * lane;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    *reinterpret_cast<float4*>(gl + 4 * q) = make_float4(G[4 * q], G[4 * q + 1], G[4 * q + 2], G[4 * q + 3]);
-                    *reinterpret_cast<float4*>(wl + 4 * q) = make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int q = 0; q < 8; ++q) {
+                    *reinterpret_cast<float2*>(gl + 2 * q) = make_float2(G[2 * q], G[2 * q + 1]);
+                    *reinterpret_cast<float2*>(wl + 2 * q) = make_float2(0.f, 0.f);
                 }
-                *reinterpret_cast<float4*>(gh + 0) = make_float4(G[16], G[17], G[18], G[19]);
-                *reinterpret_cast<float4*>(gh + 4) = make_float4(G[20], G[21], G[22], G[23]);
-                *reinterpret_cast<float4*>(gh + 8) = make_float4(G[24], G[25], G[26], 0.f);
-                *reinterpret_cast<float4*>(gh + 12) = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                for (int q = 0; q < 5; ++q) *reinterpret_cast<float2*>(gh + 2 * q) = make_float2(G[16 + 2 * q], G[17 + 2 * q]);
+                *reinterpret_cast<float2*>(gh + 10) = make_float2(G[26], 0.f);
+                *reinterpret_cast<float2*>(gh + 12) = make_float2(0.f, 0.f);
                 // the entry's weight row: diagonal w_i^2, J^T r w_i, partners w_i w_j
                 // summed over the slots holding j
                 if (c0 == 0) {
@@ -698,10 +704,10 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
                     float a0[4], a1[4], bv[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        const uint32_t k = 16 * (4 * (st + u) + kq) + m;
-                        a0[u] = Glo[k];
-                        a1[u] = Ghi[k];
-                        bv[u] = Wl[k];
+                        const uint32_t k = 4 * (st + u) + kq;
+                        a0[u] = Glo[kLo * k + m];
+                        a1[u] = Ghi[kHi * k + m];
+                        bv[u] = Wl[kW * k + m];
                     }
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
@@ -710,8 +716,8 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
                     }
                 }
                 for (; st < steps; ++st) {
-                    const uint32_t k = 16 * (4 * st + kq) + m;
-                    const float a0 = Glo[k], a1 = Ghi[k], bv = Wl[k];
+                    const uint32_t k = 4 * st + kq;
+                    const float a0 = Glo[kLo * k + m], a1 = Ghi[kHi * k + m], bv = Wl[kW * k + m];
                     lo = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bv, lo, 0, 0, 0);
                     hi = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bv, hi, 0, 0, 0);
                 }
