@@ -496,24 +496,18 @@ __global__ void __launch_bounds__(64) n2t_place_kernel(const Buffers* __restrict
             round_nodes[lane] = make_uint4(n[0], n[1], n[2], n[3]);
             wave_lds_sync();
             // stable rank among the round's entries of the same node, ordered by
-            // (term, slot); `after` marks the node's last entry of the round
-            uint32_t before[4] = {0, 0, 0, 0}, after[4] = {0, 0, 0, 0};
-            for (uint32_t l = 0; l < 64; ++l) {
+            // (term, slot); the counters then advance by the round's entries
+            // (LDS adds: their order does not matter)
+            uint32_t before[4] = {0, 0, 0, 0};
+            for (uint32_t l = 0; l < lane; ++l) {
                 const uint4 o = round_nodes[l];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const uint32_t eq = (o.x == n[q]) + (o.y == n[q]) + (o.z == n[q]) + (o.w == n[q]);
-                    before[q] += l < lane ? eq : 0u;
-                    after[q] += l > lane ? eq : 0u;
-                }
+                for (int q = 0; q < 4; ++q) before[q] += (o.x == n[q]) + (o.y == n[q]) + (o.z == n[q]) + (o.w == n[q]);
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q)
 #pragma unroll
-                for (int qq = 0; qq < 4; ++qq) {
-                    before[q] += (qq < q && n[qq] == n[q]) ? 1u : 0u;
-                    after[q] += (qq > q && n[qq] == n[q]) ? 1u : 0u;
-                }
+                for (int qq = 0; qq < 4; ++qq) before[q] += (qq < q && n[qq] == n[q]) ? 1u : 0u;
             uint32_t base[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -527,7 +521,7 @@ __global__ void __launch_bounds__(64) n2t_place_kernel(const Buffers* __restrict
             wave_lds_sync();  // every lane has read cur before the updates
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                if (n[q] != 0xFFFFFFFFu && after[q] == 0) cur[n[q] - n0] = base[q] + before[q] + 1;
+                if (n[q] != 0xFFFFFFFFu) atomicAdd(&cur[n[q] - n0], 1u);
             wave_lds_sync();
         }
     }

@@ -56,8 +56,8 @@ __global__ void __launch_bounds__(256) mf_terms_kernel(const Buffers* __restrict
     const Counts& c = *b.counts;
     Acc beta = 0;
     if (kSrc == kSrcDir && k > 0) {
-        const Acc num = reduce_partials<Acc>(slot_ptr(b, kSlotDelta + k), kVecBlocks);
-        const Acc den = reduce_partials<Acc>(slot_ptr(b, kSlotDelta + k - 1), kVecBlocks);
+        Acc num, den;
+        reduce_partials2<Acc>(slot_ptr(b, kSlotDelta + k), kVecBlocks, slot_ptr(b, kSlotDelta + k - 1), kVecBlocks, num, den);
         beta = den != (Acc)0 ? num / den : (Acc)0;
     }
     if (c.error) return;

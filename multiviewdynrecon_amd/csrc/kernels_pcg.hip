@@ -106,8 +106,8 @@ __global__ void __launch_bounds__(256) pcg_spmv_kernel(const Buffers* __restrict
     const uint32_t sub = threadIdx.x % kRowLanes;
     Acc beta = 0;
     if (k > 0) {
-        const Acc num = reduce_partials<Acc>(slot_ptr(b, kSlotDelta + k), kVecBlocks);
-        const Acc den = reduce_partials<Acc>(slot_ptr(b, kSlotDelta + k - 1), kVecBlocks);
+        Acc num, den;
+        reduce_partials2<Acc>(slot_ptr(b, kSlotDelta + k), kVecBlocks, slot_ptr(b, kSlotDelta + k - 1), kVecBlocks, num, den);
         beta = den != (Acc)0 ? num / den : (Acc)0;
     }
     Acc acc = 0;
@@ -140,8 +140,8 @@ __global__ void __launch_bounds__(256) pcg_update_kernel(const Buffers* __restri
     const Buffers b = frame_buffers(bs);
     const float* __restrict__ d = dsel ? b.d1 : b.d0;
     const FrameDev& f = *b.frame;
-    const Acc delta = reduce_partials<Acc>(slot_ptr(b, kSlotDelta + k), kVecBlocks);
-    const Acc dq = reduce_partials<Acc>(slot_ptr(b, kSlotDQ + k), spmv_blocks);
+    Acc delta, dq;
+    reduce_partials2<Acc>(slot_ptr(b, kSlotDelta + k), kVecBlocks, slot_ptr(b, kSlotDQ + k), spmv_blocks, delta, dq);
     const Acc alpha = dq != (Acc)0 ? delta / dq : (Acc)0;
     Acc acc = 0;
     if (!b.counts->error) {
@@ -226,8 +226,8 @@ template <typename Acc>
 __global__ void __launch_bounds__(256) refine_step_kernel(const Buffers* __restrict__ bs, uint32_t spmv_blocks) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
-    const Acc num = reduce_partials<Acc>(slot_ptr(b, kSlotRefNum), spmv_blocks);
-    const Acc den = reduce_partials<Acc>(slot_ptr(b, kSlotRefDen), spmv_blocks);
+    Acc num, den;
+    reduce_partials2<Acc>(slot_ptr(b, kSlotRefNum), spmv_blocks, slot_ptr(b, kSlotRefDen), spmv_blocks, num, den);
     if (b.counts->error) return;
     const Acc a = den > (Acc)0 ? num / den : (Acc)0;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -457,16 +457,15 @@ __global__ void __launch_bounds__(256) cg_init_kernel(const Buffers* __restrict_
             }
         }
     }
-    write_partial<double>(b, kSlotCgGamma + 0, g, wi.block);
-    write_partial<double>(b, kSlotCgDelta + 0, d, wi.block);
+    write_partials2<double>(b, kSlotCgGamma + 0, g, kSlotCgDelta + 0, d, wi.block);
 }
 
 // alpha_i, beta_i from the partials of gamma_i, delta_i and the stored
 // alpha_{i-1}, gamma_{i-1}; block 0 stores alpha_i, gamma_i for launch i + 1.
 __device__ inline void cg_scalars(const Buffers& b, uint32_t i, uint32_t blocks, double& alpha, double& beta,
                                   uint32_t block) {
-    const double gamma = reduce_partials<double>(slot_ptr(b, kSlotCgGamma + i), blocks);
-    const double delta = reduce_partials<double>(slot_ptr(b, kSlotCgDelta + i), blocks);
+    double gamma, delta;
+    reduce_partials2<double>(slot_ptr(b, kSlotCgGamma + i), blocks, slot_ptr(b, kSlotCgDelta + i), blocks, gamma, delta);
     beta = 0.0;
     double den = delta;
     if (i > 0) {
@@ -548,8 +547,7 @@ __global__ void __launch_bounds__(256) cg_iter_kernel(const Buffers* __restrict_
             }
         }
     }
-    write_partial<double>(b, kSlotCgGamma + i + 1, g, wi.block);
-    write_partial<double>(b, kSlotCgDelta + i + 1, d, wi.block);
+    write_partials2<double>(b, kSlotCgGamma + i + 1, g, kSlotCgDelta + i + 1, d, wi.block);
 }
 
 // Residual refinement on the CG solution (preconditioned steepest descent on
@@ -621,8 +619,8 @@ __global__ void __launch_bounds__(256) cg_refine_curvature_kernel(const Buffers*
 __global__ void __launch_bounds__(256) cg_refine_step_kernel(const Buffers* __restrict__ bs, uint32_t spmv_blocks) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
-    const double num = reduce_partials<double>(slot_ptr(b, kSlotRefNum), spmv_blocks);
-    const double den = reduce_partials<double>(slot_ptr(b, kSlotRefDen), spmv_blocks);
+    double num, den;
+    reduce_partials2<double>(slot_ptr(b, kSlotRefNum), spmv_blocks, slot_ptr(b, kSlotRefDen), spmv_blocks, num, den);
     if (b.counts->error) return;
     const double a = den > 0.0 ? num / den : 0.0;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -690,8 +688,8 @@ __global__ void __launch_bounds__(256) apply_update_kernel(const Buffers* __rest
     const uint32_t stride = gridDim.x * blockDim.x;
     double step = 0.0;
     if (kHat && refine_blocks) {
-        const double num = reduce_partials<double>(slot_ptr(b, kSlotRefNum), refine_blocks);
-        const double den = reduce_partials<double>(slot_ptr(b, kSlotRefDen), refine_blocks);
+        double num, den;
+        reduce_partials2<double>(slot_ptr(b, kSlotRefNum), refine_blocks, slot_ptr(b, kSlotRefDen), refine_blocks, num, den);
         step = den > 0.0 ? num / den : 0.0;
     }
     const bool bad = b.counts->error != 0;
@@ -808,9 +806,14 @@ void launch_apply_update(const Batch& B, uint32_t k, hipStream_t stream) {
 
 // Fused CG of the mixed precision mode: transform, init, one launch per
 // iteration, refinement, warp update of GN iteration k.
+// Workgroups per frame of the fused CG's row kernels: at most 128, each
+// looping over its rows, so a launch of 32 frames is ~4k workgroups and every
+// workgroup reduces 128 partials at its start.
+static uint32_t cg_grid(uint32_t nodes) { return std::min<uint32_t>(spmv_grid(nodes), 128u); }
+
 void launch_cg_solve(const Batch& B, uint32_t iters, uint32_t refine, uint32_t k, const Penalty& pen,
                      hipStream_t stream) {
-    const uint32_t sg = spmv_grid(B.nodes);
+    const uint32_t sg = cg_grid(B.nodes);
     iters = std::min(iters, kMaxIters);
     const uint32_t tgrid = std::min<uint32_t>((8 * B.nodes + 255) / 256, 2048u);  // grid-stride over P + N
     hipLaunchKernelGGL(cg_transform_kernel, dim3(tgrid, B.frames), dim3(256), 0, stream, B.dev, pen);
@@ -851,7 +854,7 @@ void launch_refine_step(const Batch& B, bool fp64, uint32_t spmv_blocks, hipStre
 
 // One fused CG iteration (stage timing).
 void launch_cg_iter_once(const Batch& B, hipStream_t stream) {
-    hipLaunchKernelGGL(cg_iter_kernel, dim3(spmv_grid(B.nodes), B.frames), dim3(256), 0, stream, B.dev, 1u);
+    hipLaunchKernelGGL(cg_iter_kernel, dim3(cg_grid(B.nodes), B.frames), dim3(256), 0, stream, B.dev, 1u);
 }
 
 void launch_copy_se3(const Batch& B, hipStream_t stream) {

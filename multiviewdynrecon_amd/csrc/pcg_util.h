@@ -41,24 +41,33 @@ __device__ inline Buffers frame_buffers_at(const Buffers* bs, uint32_t frame) {
 #endif
 }
 
-// Sum of `count` partials, computed identically by every block (fixed order).
+// Sums of the partials of two slots (n0 and n1 of them), computed identically
+// by every block (fixed order: strided per-thread sums, a wave butterfly, the
+// waves in order).  One workgroup barrier; at most one call per kernel (the
+// LDS is not reused).
 template <typename Acc>
-__device__ inline Acc reduce_partials(const double* p, uint32_t count) {
-    __shared__ Acc sh[256];
-    Acc v = 0;
-    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) v += (Acc)p[i];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (uint32_t w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
+__device__ inline void reduce_partials2(const double* p0, uint32_t n0, const double* p1, uint32_t n1, Acc& s0, Acc& s1) {
+    __shared__ Acc sh[2][4];
+    Acc v0 = 0, v1 = 0;
+    for (uint32_t i = threadIdx.x; i < n0; i += blockDim.x) v0 += (Acc)p0[i];
+    for (uint32_t i = threadIdx.x; i < n1; i += blockDim.x) v1 += (Acc)p1[i];
+    v0 = wave_sum(v0);
+    v1 = wave_sum(v1);
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = v0;
+        sh[1][threadIdx.x >> 6] = v1;
     }
-    const Acc total = sh[0];
     __syncthreads();
-    return total;
+    s0 = sh[0][0];
+    s1 = sh[1][0];
+    for (uint32_t w = 1; w < blockDim.x / 64; ++w) {
+        s0 += sh[0][w];
+        s1 += sh[1][w];
+    }
 }
 
-// Block sum of per-thread values, stored as this block's partial of `slot`.
+// Block sum of per-thread values, stored as this block's partial of `slot`
+// (once per kernel: the LDS values are not protected against a second call).
 template <typename Acc>
 __device__ inline void write_partial(const Buffers& b, uint32_t slot, Acc value, uint32_t block) {
     __shared__ Acc part[4];
@@ -70,6 +79,25 @@ __device__ inline void write_partial(const Buffers& b, uint32_t slot, Acc value,
 template <typename Acc>
 __device__ inline void write_partial(const Buffers& b, uint32_t slot, Acc value) {
     write_partial<Acc>(b, slot, value, blockIdx.x);
+}
+
+// Two block sums stored as this block's partials of two slots.  One call, not
+// two of write_partial: a second write_partial would overwrite its LDS values
+// while thread 0 may still be reading the first ones.
+template <typename Acc>
+__device__ inline void write_partials2(const Buffers& b, uint32_t slot0, Acc v0, uint32_t slot1, Acc v1, uint32_t block) {
+    __shared__ Acc part[2][4];
+    v0 = wave_sum(v0);
+    v1 = wave_sum(v1);
+    if ((threadIdx.x & 63) == 0) {
+        part[0][threadIdx.x >> 6] = v0;
+        part[1][threadIdx.x >> 6] = v1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        slot_ptr(b, slot0)[block] = (double)(part[0][0] + part[0][1] + part[0][2] + part[0][3]);
+        slot_ptr(b, slot1)[block] = (double)(part[1][0] + part[1][1] + part[1][2] + part[1][3]);
+    }
 }
 
 // 6x6 row-major block times a 6-vector (or its transpose), accumulated in Acc.
