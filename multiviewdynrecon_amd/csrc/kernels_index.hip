@@ -595,13 +595,14 @@ __device__ inline uint32_t lower_bound_lds(const uint32_t* a, uint32_t n, uint32
 // kRowTerms: pair keys, lower entries, and the pair-term lists (NodePair2Term)
 // with their ranges: for pair (i, j) every Node2Term entry of row i once per
 // knn slot holding j, in Node2Term order -- the order of the reference's stable
-// key sort.  kRowWaves waves per row; wave w places a contiguous w-th share of
-// the row's entries.
+// key sort.  kWaves waves per row (one in the counting modes, kRowWaves for
+// the pair-term lists, where wave w places a contiguous w-th share of the
+// row's entries).
 constexpr uint32_t kRowWaves = 4;
 enum : int { kRowCount = 0, kRowGram = 1, kRowTerms = 2 };
 
-template <int kMode>
-__global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers* __restrict__ bs) {
+template <int kMode, uint32_t kWaves>
+__global__ void __launch_bounds__(64 * kWaves) pair_rows_kernel(const Buffers* __restrict__ bs) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     Counts& c = *b.counts;
@@ -615,17 +616,17 @@ __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers
     __shared__ uint32_t set[kRowHash];
     __shared__ uint32_t ul[kRowHash];             // ascending partners j > i
     __shared__ uint32_t ll[kRowHash];             // ascending partners k < i
-    __shared__ uint32_t cnt[kRowWaves][kRowHash]; // entries per pair and wave, then next positions
+    __shared__ uint32_t cnt[kWaves][kRowHash]; // entries per pair and wave, then next positions
     __shared__ uint32_t start[kRowHash + 1];      // pair-term range starts
-    __shared__ uint32_t part[kRowWaves];
+    __shared__ uint32_t part[kWaves];
     __shared__ uint32_t nup, nlow, ovf;
-    for (uint32_t k = tid; k < kRowHash; k += 64 * kRowWaves) set[k] = 0xFFFFFFFFu;
+    for (uint32_t k = tid; k < kRowHash; k += 64 * kWaves) set[k] = 0xFFFFFFFFu;
     if (tid == 0) ovf = 0;
     __syncthreads();
     const uint32_t beg = b.n2t_offset[i], end = b.n2t_offset[i + 1];
     uint32_t upper_entries = 0;
     bool overflow = false;
-    for (uint32_t p = beg + tid; p < end; p += 64 * kRowWaves) {
+    for (uint32_t p = beg + tid; p < end; p += 64 * kWaves) {
         const ushort4 o = term_partners(b, f, c.off, b.n2t_index[p], i);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -691,14 +692,14 @@ __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers
             b.row_up[i] = up;
             b.row_low[i] = low;
             uint32_t s2 = 0;
-            for (uint32_t k = 0; k < kRowWaves; ++k) s2 += part[k];
+            for (uint32_t k = 0; k < kWaves; ++k) s2 += part[k];
             b.row_kv2[i] = s2;
         }
         if (kMode == kRowCount) return;
         uint16_t* rp = b.row_part + (size_t)i * kRowHash;
-        for (uint32_t u = tid; u < up; u += 64 * kRowWaves) rp[u] = (uint16_t)ul[u];
-        for (uint32_t k = tid; k < low; k += 64 * kRowWaves) rp[up + k] = (uint16_t)ll[k];
-        for (uint32_t p = beg + tid; p < end; p += 64 * kRowWaves) {
+        for (uint32_t u = tid; u < up; u += 64 * kWaves) rp[u] = (uint16_t)ul[u];
+        for (uint32_t k = tid; k < low; k += 64 * kWaves) rp[up + k] = (uint16_t)ll[k];
+        for (uint32_t p = beg + tid; p < end; p += 64 * kWaves) {
             ushort4 k;
             const int m = term_nodes(b, f, c.off, b.n2t_index[p], k);
             uint32_t packed = 0xFFFFFFFFu;
@@ -716,7 +717,7 @@ __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers
         return;
     }
     const uint32_t up_base = b.pair_offset[i], low_base = b.low_offset[i];
-    const uint32_t len = end - beg, per = (len + kRowWaves - 1) / kRowWaves;
+    const uint32_t len = end - beg, per = (len + kWaves - 1) / kWaves;
     const uint32_t wb = beg + min(len, w * per), we = beg + min(len, (w + 1) * per);
     uint32_t* run = cnt[w];
     if (kMode == kRowTerms && up > 0) {
@@ -737,7 +738,7 @@ __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers
             uint32_t acc = b.kv2_offset[i];
             for (uint32_t u = 0; u < up; ++u) {
                 start[u] = acc;
-                for (uint32_t k = 0; k < kRowWaves; ++k) {
+                for (uint32_t k = 0; k < kWaves; ++k) {
                     const uint32_t n = cnt[k][u];
                     cnt[k][u] = acc;
                     acc += n;
@@ -746,7 +747,7 @@ __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers
             start[up] = acc;
         }
         __syncthreads();
-        for (uint32_t u = tid; u < up; u += 64 * kRowWaves)
+        for (uint32_t u = tid; u < up; u += 64 * kWaves)
             b.pair_range[up_base + u] = make_uint2(start[u], start[u + 1]);
         // pass 2: entries in Node2Term order, slot order within an entry
         for (uint32_t base = wb; base < we; base += 64) {
@@ -794,8 +795,8 @@ __global__ void __launch_bounds__(64 * kRowWaves) pair_rows_kernel(const Buffers
             }
         }
     }
-    for (uint32_t u = tid; u < up; u += 64 * kRowWaves) b.pair_key[up_base + u] = (i << 16) | ul[u];
-    for (uint32_t k = tid; k < low; k += 64 * kRowWaves) b.low_ent[low_base + k] = make_uint2(ll[k], 0u);
+    for (uint32_t u = tid; u < up; u += 64 * kWaves) b.pair_key[up_base + u] = (i << 16) | ul[u];
+    for (uint32_t k = tid; k < low; k += 64 * kWaves) b.low_ent[low_base + k] = make_uint2(ll[k], 0u);
 }
 
 // Pair keys and lower entries of every row at their scanned offsets, from the
@@ -903,13 +904,13 @@ void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, int
     if (mode == kIndexNodes) return;
     const uint32_t row_grid = (B.nodes + kXcds - 1) / kXcds * kXcds;
     if (gram) {
-        hipLaunchKernelGGL(pair_rows_kernel<kRowGram>, dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
+        hipLaunchKernelGGL((pair_rows_kernel<kRowGram, 1>), dim3(row_grid, F), dim3(64), 0, stream, B.dev);
         hipLaunchKernelGGL(node_scan_kernel, dim3(3, F), dim3(1024), 0, stream, B.dev, 1);
         hipLaunchKernelGGL(pair_emit_kernel, dim3((B.nodes + 15) / 16, F), dim3(256), 0, stream, B.dev);
     } else {
-        hipLaunchKernelGGL(pair_rows_kernel<kRowCount>, dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
+        hipLaunchKernelGGL((pair_rows_kernel<kRowCount, 1>), dim3(row_grid, F), dim3(64), 0, stream, B.dev);
         hipLaunchKernelGGL(node_scan_kernel, dim3(3, F), dim3(1024), 0, stream, B.dev, 1);
-        hipLaunchKernelGGL(pair_rows_kernel<kRowTerms>, dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
+        hipLaunchKernelGGL((pair_rows_kernel<kRowTerms, kRowWaves>), dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
     }
     hipLaunchKernelGGL(pair_link_kernel, dim3((B.nodes + 255) / 256, F), dim3(256), 0, stream, B.dev);
     if (gram) hipLaunchKernelGGL(pair_edges_kernel, dim3((8 * B.nodes + 255) / 256, F), dim3(256), 0, stream, B.dev);
@@ -920,7 +921,7 @@ void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, int
 // as they are.
 void launch_pair_term_lists(const Batch& B, hipStream_t stream) {
     const uint32_t row_grid = (B.nodes + kXcds - 1) / kXcds * kXcds;
-    hipLaunchKernelGGL(pair_rows_kernel<kRowTerms>, dim3(row_grid, B.frames), dim3(64 * kRowWaves), 0, stream, B.dev);
+    hipLaunchKernelGGL((pair_rows_kernel<kRowTerms, kRowWaves>), dim3(row_grid, B.frames), dim3(64 * kRowWaves), 0, stream, B.dev);
     hipLaunchKernelGGL(pair_link_kernel, dim3((B.nodes + 255) / 256, B.frames), dim3(256), 0, stream, B.dev);
 }
 

@@ -518,57 +518,33 @@ struct Lambda2 {
           feature(p.feature * p.feature), cross(p.cross * p.cross) {}
 };
 
-// G_t of term t (record raw) for row node i, the knn weight w_i and, per knn
-// slot q holding an upper partner n_q > i, the weight w_i w_{n_q} (0 for the
-// other slots).
-__device__ inline void gram_entry(const Buffers& b, const TermBounds& c, const Lambda2& l2, bool density, uint32_t t,
-                                  const RawRec& raw, uint32_t i, const float4 g, float G[kGramVals], float& wi,
-                                  float wq[4], uint32_t nq[4]) {
-    uint32_t k;
-    const int type = type_of(c, t, k);
-    float J[3][6], res[3], lam2[3];
-    int rows = 3;
-    ushort4 knn = make_ushort4(kNoNode, kNoNode, kNoNode, kNoNode);
-    float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (type == kSmooth) {
-        SmoothRec sm;
-        __builtin_memcpy(&sm, &raw, sizeof(SmoothRec));
-        smooth_rows(sm, i == (sm.ij & 0xFFFF), J, res);
-        lam2[0] = lam2[1] = lam2[2] = l2.smooth;
-    } else if (type == kDense || type == kForeground) {
-        ScalarRec rec;
-        __builtin_memcpy(&rec, &raw, sizeof(ScalarRec));
-        knn = rec.knn;
-        w = rec.w;
+// LDS rows of one staged entry: G_t values 0..15 (lo), values 16..26 and
+// zeros (hi), the weight row (16 columns and a padding column).  The row
+// strides (18, 14, 18 words) keep the lanes' float2 stores of their own rows
+// and the product's column reads spread over the banks.
+constexpr uint32_t kLo = 18, kHi = 14, kW = 18;
+static_assert(kW > kGramCols, "the weight row needs a padding column");
+
+// G_t of `rows` Jacobian rows (world frame; moved to the node frame of g here)
+// with their lambda^2 and residuals, stored to the lane's LDS rows.  Row 0
+// always exists; rows 1, 2 are added when present.
+template <int kMaxRows>
+__device__ inline void store_gram(float J[3][6], const float res[3], const float lam2[3], int rows, const float4 g,
+                                  float* gl, float* gh) {
+    float G[kGramVals];
+    to_node_frame(J[0], g);
+    {
+        int o = 0;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) J[0][q] = rec.J[q];
-        res[0] = rec.r;
-        lam2[0] = type == kDense ? 1.0f : l2.foreground;
-        rows = 1;
-        if (type == kDense && density) {
-            const ScalarRec& d = b.density[k];
+        for (int a = 0; a < 6; ++a) {
+            const float la = lam2[0] * J[0][a];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) J[1][q] = d.J[q];
-            res[1] = d.r;
-            lam2[1] = l2.density;
-            rows = 2;
+            for (int k2 = a; k2 < 6; ++k2) G[o++] = la * J[0][k2];
+            G[21 + a] = -la * res[0];
         }
-    } else {
-        P2PRec rec;
-        __builtin_memcpy(&rec, &raw, sizeof(P2PRec));
-        knn = rec.knn;
-        w = rec.w;
-        p2p_rows(rec, J, res);
-        // bit select: a select of two fields would become a load through a
-        // selected address (and the struct a scratch array)
-        const uint32_t mask = type == kFeature ? 0xFFFFFFFFu : 0u;
-        const float l = __uint_as_float((__float_as_uint(l2.feature) & mask) | (__float_as_uint(l2.cross) & ~mask));
-        lam2[0] = lam2[1] = lam2[2] = l;
     }
 #pragma unroll
-    for (int q = 0; q < (int)kGramVals; ++q) G[q] = 0.f;
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
+    for (int r = 1; r < kMaxRows; ++r) {
         if (r < rows) {
             to_node_frame(J[r], g);
             int o = 0;
@@ -581,22 +557,145 @@ __device__ inline void gram_entry(const Buffers& b, const TermBounds& c, const L
             }
         }
     }
-    const bool knn_term = type != kSmooth;
-    wi = knn_term ? slot_weight(knn, w, i) : 1.f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) *reinterpret_cast<float2*>(gl + 2 * q) = make_float2(G[2 * q], G[2 * q + 1]);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) *reinterpret_cast<float2*>(gh + 2 * q) = make_float2(G[16 + 2 * q], G[17 + 2 * q]);
+    *reinterpret_cast<float2*>(gh + 10) = make_float2(G[26], 0.f);
+    *reinterpret_cast<float2*>(gh + 12) = make_float2(0.f, 0.f);
+}
+
+// Partner columns of row i held by the index or searched in the row's pair keys.
+struct GramColumns {
+    uint32_t c0;       // first weight column of this pass
+    uint32_t ubeg, U;  // the row's upper pairs
+    int mode;          // 0: no partner columns, 1: n2t_col, 2: pair-key search
+};
+
+// The weight row of a knn-weighted term for row node i: diagonal w_i^2, J^T r
+// w_i, and per upper partner j the weight w_i w_j summed over the knn slots
+// holding j (every such slot stores the same sum).  Slots outside this pass
+// store into the padding column.
+__device__ inline void store_knn_weights(const Buffers& b, const GramColumns& gc, ushort4 knn, float4 w, uint32_t i,
+                                         uint32_t pc, float* wl) {
+    const float wi = slot_weight(knn, w, i);
+    if (gc.c0 == 0) {
+        wl[0] = wi * wi;
+        wl[1] = wi;
+    }
+    if (gc.mode == 0) return;
     const uint32_t n[4] = {knn.x, knn.y, knn.z, knn.w};
-    float ws[4] = {w.x, w.y, w.z, w.w};  // distinct knn: the slot rule gives knn[q] the weight w[q]
     const bool distinct = knn.x != knn.y && knn.x != knn.z && knn.x != knn.w && knn.y != knn.z && knn.y != knn.w &&
                           knn.z != knn.w;
+    float wq[4] = {w.x, w.y, w.z, w.w};  // distinct knn: the slot rule gives knn[q] the weight w[q]
     if (!distinct) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) ws[q] = slot_weight(knn, w, n[q]);
+        for (int q = 0; q < 4; ++q) wq[q] = slot_weight(knn, w, n[q]);
+    }
+    uint32_t col[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const bool up = n[q] > i && n[q] != kNoNode;
+        wq[q] = up ? wi * wq[q] : 0.f;
+        col[q] = (pc >> (8 * q)) & 0xFFu;
+    }
+    if (gc.mode == 2) {
+        // more partners than the index's u8 columns: search the row's pair keys
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            col[q] = 0xFFu;
+            if (wq[q] == 0.f) continue;
+            const uint32_t key = (i << 16) | n[q];
+            uint32_t lo = 0, hi = gc.U;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (b.pair_key[gc.ubeg + mid] < key) lo = mid + 1;
+                else hi = mid;
+            }
+            col[q] = 2 + lo;
+        }
+    }
+    float sum[4] = {wq[0], wq[1], wq[2], wq[3]};
+    if (!distinct) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            sum[q] = 0.f;
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) sum[q] += col[qq] == col[q] ? wq[qq] : 0.f;
+        }
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const bool up = knn_term && n[q] > i && n[q] != kNoNode;
-        nq[q] = up ? n[q] : kNoNode;
-        wq[q] = up ? wi * ws[q] : 0.f;
+        const uint32_t r = col[q] - gc.c0;
+        wl[wq[q] != 0.f && r < kGramCols ? r : kGramCols] = sum[q];
     }
+}
+
+// One Node2Term entry of row i (term t, record raw) staged in LDS: its G_t
+// row and its weight row (zeros for an empty lane).
+__device__ inline void stage_entry(const Buffers& b, const TermBounds& c, const Lambda2& l2, bool density,
+                                   const GramColumns& gc, uint32_t t, const RawRec& raw, uint32_t pc, uint32_t i,
+                                   const float4 g, float* gl, float* gh, float* wl) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) *reinterpret_cast<float2*>(wl + 2 * q) = make_float2(0.f, 0.f);
+    if (t == 0xFFFFFFFFu) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) *reinterpret_cast<float2*>(gl + 2 * q) = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int q = 0; q < 7; ++q) *reinterpret_cast<float2*>(gh + 2 * q) = make_float2(0.f, 0.f);
+        return;
+    }
+    uint32_t k;
+    const int type = type_of(c, t, k);
+    float J[3][6], res[3], lam2[3];
+    if (type == kSmooth) {
+        // graph edge: rows of node i or j, unit weight, no partner columns here
+        SmoothRec sm;
+        __builtin_memcpy(&sm, &raw, sizeof(SmoothRec));
+        smooth_rows(sm, i == (sm.ij & 0xFFFF), J, res);
+        lam2[0] = lam2[1] = lam2[2] = l2.smooth;
+        store_gram<3>(J, res, lam2, 3, g, gl, gh);
+        if (gc.c0 == 0) {
+            wl[0] = 1.f;
+            wl[1] = 1.f;
+        }
+        return;
+    }
+    ushort4 knn;
+    float4 w;
+    if (type == kDense || type == kForeground) {
+        ScalarRec rec;
+        __builtin_memcpy(&rec, &raw, sizeof(ScalarRec));
+        knn = rec.knn;
+        w = rec.w;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) J[0][q] = rec.J[q];
+        res[0] = rec.r;
+        lam2[0] = type == kDense ? 1.0f : l2.foreground;
+        int rows = 1;
+        if (type == kDense && density) {
+            const ScalarRec& d = b.density[k];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) J[1][q] = d.J[q];
+            res[1] = d.r;
+            lam2[1] = l2.density;
+            rows = 2;
+        }
+        store_gram<2>(J, res, lam2, rows, g, gl, gh);
+    } else {
+        P2PRec rec;
+        __builtin_memcpy(&rec, &raw, sizeof(P2PRec));
+        knn = rec.knn;
+        w = rec.w;
+        p2p_rows(rec, J, res);
+        // bit select: a select of two fields would become a load through a
+        // selected address (and the struct a scratch array)
+        const uint32_t mask = type == kFeature ? 0xFFFFFFFFu : 0u;
+        const float l = __uint_as_float((__float_as_uint(l2.feature) & mask) | (__float_as_uint(l2.cross) & ~mask));
+        lam2[0] = lam2[1] = lam2[2] = l;
+        store_gram<3>(J, res, lam2, 3, g, gl, gh);
+    }
+    store_knn_weights(b, gc, knn, w, i, pc, wl);
 }
 
 // diag_only: the diagonal blocks and J^T r alone (the matrix-free solve).
@@ -605,18 +704,15 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
-    // term-major rows: G_t values 0..15, values 16..26 (+ 3 zeros), the weight
-    // row.  The row strides (18, 14, 18 words) keep the lanes' float2 stores of
-    // their own rows and the product's column reads spread over the banks; a
-    // read of values 30..31 lands in the next row (or the zeroed tail) and only
-    // feeds product rows that are never used.
-    constexpr uint32_t kLo = 18, kHi = 14, kW = 18;
     __shared__ float Glo[64 * kLo];
-    __shared__ float Ghi[64 * kHi + 2];
+    __shared__ float Ghi[64 * kHi + 2];  // a read of values 30, 31 of row 63 lands in the zeroed tail
     __shared__ float Wl[64 * kW];
     const uint32_t lane = threadIdx.x;
     if (lane < 2) Ghi[64 * kHi + lane] = 0.f;
     const uint32_t m = lane & 15, kq = lane >> 4;
+    float* gl = Glo + kLo * lane;
+    float* gh = Ghi + kHi * lane;
+    float* wl = Wl + kW * lane;
     const Lambda2 l2(pen);
     const bool density = b.density && pen.density != 0.f;
     const TermBounds tb(c);
@@ -627,79 +723,45 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
         const uint32_t ubeg = diag_only ? 0 : b.pair_offset[i], U = diag_only ? 0 : b.pair_offset[i + 1] - ubeg;
         const float4 g = f.nodes[i];
         const uint32_t cols = 2 + U;
-        const bool indexed = cols < 0xFF;  // partner columns held by n2t_col
+        // partner columns held by n2t_col while they fit its u8 columns
+        const int mode = diag_only ? 0 : (cols < 0xFF ? 1 : 2);
         for (uint32_t c0 = 0; c0 < cols; c0 += kGramCols) {
+            const GramColumns gc{c0, ubeg, U, mode};
             gram_f4 lo = {0.f, 0.f, 0.f, 0.f}, hi = {0.f, 0.f, 0.f, 0.f};
             // two-stage fetch: the records of the next step and the entries of
             // the step after it are in flight while this step computes
             uint32_t t_cur = beg + lane < end ? b.n2t_index[beg + lane] : 0xFFFFFFFFu;
-            const bool cols_in = indexed && !diag_only;
+            const bool cols_in = mode == 1;
             uint32_t pc_cur = cols_in && beg + lane < end ? b.n2t_col[beg + lane] : 0xFFFFFFFFu;
             RawRec rec_cur = load_record(b, tb, t_cur);
             uint32_t t_nxt = beg + 64 + lane < end ? b.n2t_index[beg + 64 + lane] : 0xFFFFFFFFu;
             uint32_t pc_nxt = cols_in && beg + 64 + lane < end ? b.n2t_col[beg + 64 + lane] : 0xFFFFFFFFu;
             for (uint32_t e0 = beg; e0 < end; e0 += 64) {
-                float G[kGramVals];
-                float wi = 0.f, wq[4] = {0.f, 0.f, 0.f, 0.f};
-                uint32_t col[4] = {0xFFu, 0xFFu, 0xFFu, 0xFFu};
-                if (t_cur != 0xFFFFFFFFu) {
-                    uint32_t nq[4];
-                    gram_entry(b, tb, l2, density, t_cur, rec_cur, i, g, G, wi, wq, nq);
-                    if (diag_only) {
-                        // no partner columns
-                    } else if (indexed) {
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) col[q] = (pc_cur >> (8 * q)) & 0xFFu;
-                    } else {
-                        // more partners than the index's u8 columns: search the row's pair keys
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            if (nq[q] == kNoNode) continue;
-                            const uint32_t key = (i << 16) | nq[q];
-                            uint32_t l2 = 0, h2 = U;
-                            while (l2 < h2) {
-                                const uint32_t mid = (l2 + h2) >> 1;
-                                if (b.pair_key[ubeg + mid] < key) l2 = mid + 1;
-                                else h2 = mid;
-                            }
-                            col[q] = 2 + l2;
-                        }
-                    }
-                } else {
-#pragma unroll
-                    for (int q = 0; q < (int)kGramVals; ++q) G[q] = 0.f;
-                }
+                stage_entry(b, tb, l2, density, gc, t_cur, rec_cur, pc_cur, i, g, gl, gh, wl);
                 const RawRec rec_nxt = load_record(b, tb, t_nxt);
                 const uint32_t enn = e0 + 128 + lane;
                 const uint32_t t_nn = enn < end ? b.n2t_index[enn] : 0xFFFFFFFFu;
                 const uint32_t pc_nn = cols_in && enn < end ? b.n2t_col[enn] : 0xFFFFFFFFu;
-                float* gl = Glo + kLo * lane;
-                float* gh = Ghi + kHi * lane;
-                float* wl = Wl + kW * lane;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    *reinterpret_cast<float2*>(gl + 2 * q) = make_float2(G[2 * q], G[2 * q + 1]);
-                    *reinterpret_cast<float2*>(wl + 2 * q) = make_float2(0.f, 0.f);
-                }
-#pragma unroll
-                for (int q = 0; q < 5; ++q) *reinterpret_cast<float2*>(gh + 2 * q) = make_float2(G[16 + 2 * q], G[17 + 2 * q]);
-                *reinterpret_cast<float2*>(gh + 10) = make_float2(G[26], 0.f);
-                *reinterpret_cast<float2*>(gh + 12) = make_float2(0.f, 0.f);
-                // the entry's weight row: diagonal w_i^2, J^T r w_i, partners w_i w_j
-                // summed over the slots holding j
-                if (c0 == 0) {
-                    wl[0] = wi * wi;
-                    wl[1] = wi;
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const uint32_t r = col[q] - c0;
-                    if (wq[q] != 0.f && r < kGramCols) wl[r] += wq[q];  // the lane's own row
-                }
                 wave_lds_sync();
                 // acc += G^T W over 4 entries per step
                 const uint32_t steps = (min(end - e0, 64u) + 3) / 4;
                 uint32_t st = 0;
+                if (steps == 16) {  // a full step: every operand read is issued before the products
+                    float a0[16], a1[16], bv[16];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) {
+                        const uint32_t k = 4 * u + kq;
+                        a0[u] = Glo[kLo * k + m];
+                        a1[u] = Ghi[kHi * k + m];
+                        bv[u] = Wl[kW * k + m];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) {
+                        lo = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], bv[u], lo, 0, 0, 0);
+                        hi = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], bv[u], hi, 0, 0, 0);
+                    }
+                    st = 16;
+                }
                 for (; st + 4 <= steps; st += 4) {  // four steps' operands in flight at once
                     float a0[4], a1[4], bv[4];
 #pragma unroll
@@ -840,7 +902,7 @@ void launch_assemble(const Batch& B, const Penalty& pen, bool gram, bool diag_on
     if (gram) {
         const uint32_t rows = (B.nodes + kXcds - 1) / kXcds * kXcds;
         hipLaunchKernelGGL(gram_rows_kernel, dim3(rows, B.frames), dim3(64), 0, stream, B.dev, pen, diag_only ? 1 : 0);
-        hipLaunchKernelGGL(gram_fixup_kernel, dim3((B.nodes + 255) / 256, B.frames), dim3(256), 0, stream, B.dev);
+        hipLaunchKernelGGL(gram_fixup_kernel, dim3((B.nodes + 63) / 64, B.frames), dim3(64), 0, stream, B.dev);
     } else {
         hipLaunchKernelGGL(assemble_chunks_kernel, dim3(2048, B.frames), dim3(256), 0, stream, B.dev, pen);
         hipLaunchKernelGGL(assemble_fixup_kernel<float>, dim3(fix_blocks, B.frames), dim3(256), 0, stream, B.dev);
