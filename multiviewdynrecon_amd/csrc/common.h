@@ -335,6 +335,7 @@ struct Buffers {
     // block-sparse rows
     uint32_t* low_offset;    // N + 1
     uint2* low_ent;          // cap_pairs: (k, pair) of row i's lower entries, k ascending
+    uint32_t* low_pos;       // cap_pairs: position in low_ent of each pair's lower entry (mixed mode)
     // linear system
     float* diag;             // 36 N
     float* diag_inv;         // 36 N
@@ -345,8 +346,15 @@ struct Buffers {
     float *x, *r, *s, *q, *d0, *d1;
     // fused CG on A^ = L^-1 A L^-T (mixed precision mode)
     double* linv64;          // 21 N: L^-1 of each diagonal block, packed lower triangle (tri_index)
-    float* offhat;           // 36 P: L_i^-1 B_ij L_j^-T
-    float *cg_r[2], *cg_w[2], *cg_s[2], *cg_p, *cg_x, *cg_b;
+    // L^-1 A L^-T by block rows: row i holds its lower blocks (low_ent order)
+    // then its upper blocks (pair order), every block row-major for row i;
+    // row i starts at slot low_offset[i] + pair_offset[i]
+    float* offrow;           // 36 x 2P
+    uint32_t* row_col;       // 2P: column node of each slot
+    uint32_t* row_ptr;       // N + 1: first slot of each block row
+    // fused CG: per node one record (r 0..5, w 6..11, s 12..17, 2 pad) of the
+    // set read by iteration i (cg_rws[i & 1]) and of the set it writes
+    float *cg_rws[2], *cg_p, *cg_x, *cg_b;
     double* cg_scal;         // alpha_i, gamma_i per iteration
     double* partial;         // kPartialSlots x kPartialStride per-block partial sums
     // warp field double buffer
@@ -382,6 +390,7 @@ enum : int { kIndexLists = 0, kIndexGram = 1, kIndexNodes = 2 };
 // Packed row-major lower triangle of a 6x6 matrix: entry (a, k), k <= a.
 __host__ __device__ constexpr uint32_t tri_index(uint32_t a, uint32_t k) { return a * (a + 1) / 2 + k; }
 constexpr uint32_t kTri = 21;
+constexpr uint32_t kRws = 20;  // floats per node record of the fused CG's r, w, s
 
 // Partner nodes a row may have (upper and lower pairs together).
 constexpr uint32_t kRowPartners = 512;

@@ -853,7 +853,7 @@ __global__ void __launch_bounds__(256) pair_edges_kernel(const Buffers* __restri
 }
 
 // Pair index of every lower entry (k, .) of row j: the position of (k, j) in
-// row k's ascending upper list.
+// row k's ascending upper list (and, in the mixed mode, the inverse map).
 __global__ void __launch_bounds__(256) pair_link_kernel(const Buffers* __restrict__ bs) {
     const Buffers b = frame_buffers(bs);
     const Counts& c = *b.counts;
@@ -870,8 +870,12 @@ __global__ void __launch_bounds__(256) pair_link_kernel(const Buffers* __restric
                 if (b.pair_key[mid] < key) lo = mid + 1;
                 else hi = mid;
             }
-            if (lo == b.pair_offset[k + 1] || b.pair_key[lo] != key) atomicOr(&b.counts->error, kErrPairs);
+            if (lo == b.pair_offset[k + 1] || b.pair_key[lo] != key) {
+                atomicOr(&b.counts->error, kErrPairs);
+                continue;
+            }
             b.low_ent[l].y = lo;
+            if (b.low_pos) b.low_pos[lo] = l;
         }
     }
 }

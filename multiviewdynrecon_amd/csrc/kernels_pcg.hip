@@ -256,13 +256,96 @@ __global__ void __launch_bounds__(256) refine_step_kernel(const Buffers* __restr
 constexpr uint32_t kSlotCgGamma = 0;   // gamma_i partials: slot i
 constexpr uint32_t kSlotCgDelta = 72;  // delta_i partials: slot 72 + i
 
-// Pair blocks and the transformed system.  The row Gram assembly leaves
+// The transformed system.  The row Gram assembly leaves
 // S_ij = sum w_i w_j lambda^2 u_i^T u_i (21 values) per pair; here
 // B_ij = S_ij M_ij (the change of centre g_i -> g_j: rows e_c x (g_j - g_i) add
 // the translation part to the rotation part) plus the pair blocks of the graph
-// edges joining i and j, in edge order, is stored (J^T J off-diagonal, node
-// frame), and A^_ij = L_i^-1 B_ij L_j^-T (FP64 factors) with it; the nodes get
-// b^ = L^-1 b.
+// edges joining i and j, in edge order, gives A^_ij = L_i^-1 B_ij L_j^-T (FP64
+// factors) in block row i and its transpose A^_ji in block row j; the nodes get
+// b^ = L^-1 b and their block-row starts.  B_ij itself (J^T J off-diagonal,
+// node frame) is not stored: pair_blocks_kernel forms it on demand.
+// B_ij of pair p = (i, j): the row Gram assembly's S_ij (offdiag, 21 values)
+// times the change of centre M_ij, plus the pair blocks of the graph edges
+// joining i and j, in edge order.
+__device__ inline void pair_block(const Buffers& b, const Counts& c, const FrameDev& f, const Penalty& pen, uint32_t p,
+                                  uint32_t i, uint32_t j, float B[36]) {
+    const float* Bp = b.offdiag + 36 * (size_t)p;
+    float v[21];
+#pragma unroll
+    for (int q = 0; q < 21; ++q) v[q] = Bp[q];
+    float S[36];
+    int o = 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+        for (int k2 = a; k2 < 6; ++k2) {
+            S[6 * a + k2] = v[o];
+            S[6 * k2 + a] = v[o];
+            ++o;
+        }
+    const float4 gi = f.nodes[i], gj = f.nodes[j];
+    const float d[3] = {gj.x - gi.x, gj.y - gi.y, gj.z - gi.z};
+    // rows e_c x d of the change of centre (translation -> rotation part)
+    const float K[3][3] = {{0.f, -d[2], d[1]}, {d[2], 0.f, -d[0]}, {-d[1], d[0], 0.f}};
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+        for (int k2 = 0; k2 < 6; ++k2) {
+            float s = S[6 * a + k2];
+            if (k2 < 3) s += S[6 * a + 3] * K[0][k2] + S[6 * a + 4] * K[1][k2] + S[6 * a + 5] * K[2][k2];
+            B[6 * a + k2] = s;
+        }
+    auto add_edge = [&](uint32_t e) {
+        float Ji[3][6], Jj[3][6], li[3];
+        const int rows = pair_term_rows(b, c, pen, c.off[0] + e, i, j, Ji, Jj, li);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            if (r < rows) {
+                to_node_frame(Ji[r], gi);
+                to_node_frame(Jj[r], gj);
+#pragma unroll
+                for (int a = 0; a < 6; ++a)
+#pragma unroll
+                    for (int k2 = 0; k2 < 6; ++k2) B[6 * a + k2] += li[r] * Ji[r][a] * Jj[r][k2];
+            }
+        }
+    };
+    const uint32_t ne = b.pair_edge_n[p];
+    if (ne <= kPairEdges) {
+        uint32_t ed[kPairEdges];
+#pragma unroll
+        for (int q = 0; q < (int)kPairEdges; ++q)
+            ed[q] = (uint32_t)q < ne ? b.pair_edge[kPairEdges * (size_t)p + q] : 0xFFFFFFFFu;
+        // edge order (the edges were appended in arbitrary order)
+#pragma unroll
+        for (int q = 1; q < (int)kPairEdges; ++q)
+#pragma unroll
+            for (int r = q; r > 0; --r)
+                if (ed[r] < ed[r - 1]) {
+                    const uint32_t tmp = ed[r];
+                    ed[r] = ed[r - 1];
+                    ed[r - 1] = tmp;
+                }
+        for (uint32_t q = 0; q < ne; ++q) add_edge(ed[q]);
+    } else {
+        // repeated graph edges: row i's Node2Term list holds each edge
+        // term of i once, in edge order
+        const uint32_t last = b.n2t_offset[i + 1];
+        uint32_t lo = b.n2t_offset[i], hi = last;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (b.n2t_index[mid] < c.off[0]) lo = mid + 1;
+            else hi = mid;
+        }
+        for (uint32_t e = lo; e < last; ++e) {
+            const uint32_t t = b.n2t_index[e];
+            if (t >= c.off[1]) break;
+            const ushort2 ge = f.graph[t - c.off[0]];
+            if (ge.x != ge.y && (ge.x == j || ge.y == j)) add_edge(t - c.off[0]);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) cg_transform_kernel(const Buffers* __restrict__ bs, Penalty pen) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
@@ -275,84 +358,8 @@ __global__ void __launch_bounds__(256) cg_transform_kernel(const Buffers* __rest
             const uint32_t p = x;
             const uint32_t key = b.pair_key[p];
             const uint32_t i = key >> 16, j = key & 0xFFFF;
-            float* Bp = b.offdiag + 36 * (size_t)p;
-            float v[21];
-#pragma unroll
-            for (int q = 0; q < 21; ++q) v[q] = Bp[q];
-            float S[36];
-            int o = 0;
-#pragma unroll
-            for (int a = 0; a < 6; ++a)
-#pragma unroll
-                for (int k2 = a; k2 < 6; ++k2) {
-                    S[6 * a + k2] = v[o];
-                    S[6 * k2 + a] = v[o];
-                    ++o;
-                }
-            const float4 gi = f.nodes[i], gj = f.nodes[j];
-            const float d[3] = {gj.x - gi.x, gj.y - gi.y, gj.z - gi.z};
-            // rows e_c x d of the change of centre (translation -> rotation part)
-            const float K[3][3] = {{0.f, -d[2], d[1]}, {d[2], 0.f, -d[0]}, {-d[1], d[0], 0.f}};
             float B[36];
-#pragma unroll
-            for (int a = 0; a < 6; ++a)
-#pragma unroll
-                for (int k2 = 0; k2 < 6; ++k2) {
-                    float s = S[6 * a + k2];
-                    if (k2 < 3) s += S[6 * a + 3] * K[0][k2] + S[6 * a + 4] * K[1][k2] + S[6 * a + 5] * K[2][k2];
-                    B[6 * a + k2] = s;
-                }
-            auto add_edge = [&](uint32_t e) {
-                float Ji[3][6], Jj[3][6], li[3];
-                const int rows = pair_term_rows(b, c, pen, c.off[0] + e, i, j, Ji, Jj, li);
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    if (r < rows) {
-                        to_node_frame(Ji[r], gi);
-                        to_node_frame(Jj[r], gj);
-#pragma unroll
-                        for (int a = 0; a < 6; ++a)
-#pragma unroll
-                            for (int k2 = 0; k2 < 6; ++k2) B[6 * a + k2] += li[r] * Ji[r][a] * Jj[r][k2];
-                    }
-                }
-            };
-            const uint32_t ne = b.pair_edge_n[p];
-            if (ne <= kPairEdges) {
-                uint32_t ed[kPairEdges];
-#pragma unroll
-                for (int q = 0; q < (int)kPairEdges; ++q)
-                    ed[q] = (uint32_t)q < ne ? b.pair_edge[kPairEdges * (size_t)p + q] : 0xFFFFFFFFu;
-                // edge order (the edges were appended in arbitrary order)
-#pragma unroll
-                for (int q = 1; q < (int)kPairEdges; ++q)
-#pragma unroll
-                    for (int r = q; r > 0; --r)
-                        if (ed[r] < ed[r - 1]) {
-                            const uint32_t tmp = ed[r];
-                            ed[r] = ed[r - 1];
-                            ed[r - 1] = tmp;
-                        }
-                for (uint32_t q = 0; q < ne; ++q) add_edge(ed[q]);
-            } else {
-                // repeated graph edges: row i's Node2Term list holds each edge
-                // term of i once, in edge order
-                const uint32_t last = b.n2t_offset[i + 1];
-                uint32_t lo = b.n2t_offset[i], hi = last;
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (b.n2t_index[mid] < c.off[0]) lo = mid + 1;
-                    else hi = mid;
-                }
-                for (uint32_t e = lo; e < last; ++e) {
-                    const uint32_t t = b.n2t_index[e];
-                    if (t >= c.off[1]) break;
-                    const ushort2 ge = f.graph[t - c.off[0]];
-                    if (ge.x != ge.y && (ge.x == j || ge.y == j)) add_edge(t - c.off[0]);
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 36; ++q) Bp[q] = B[q];
+            pair_block(b, c, f, pen, p, i, j, B);
             const double* Li = b.linv64 + kTri * (size_t)i;
             const double* Lj = b.linv64 + kTri * (size_t)j;
             double T[36];
@@ -365,7 +372,9 @@ __global__ void __launch_bounds__(256) cg_transform_kernel(const Buffers* __rest
                     for (int k = 0; k <= a; ++k) t += Li[tri_index(a, k)] * (double)B[6 * k + cc];
                     T[6 * a + cc] = t;
                 }
-            float* out = b.offhat + 36 * (size_t)x;
+            // A^_ij in row i's upper slot, its transpose in row j's lower slot
+            const uint32_t su = b.low_offset[i + 1] + p, sl = b.pair_offset[j] + b.low_pos[p];
+            float Ah[36];
 #pragma unroll
             for (int a = 0; a < 6; ++a)
 #pragma unroll
@@ -373,10 +382,28 @@ __global__ void __launch_bounds__(256) cg_transform_kernel(const Buffers* __rest
                     double t = 0.0;
 #pragma unroll
                     for (int cc = 0; cc <= bb; ++cc) t += T[6 * a + cc] * Lj[tri_index(bb, cc)];
-                    out[6 * a + bb] = (float)t;
+                    Ah[6 * a + bb] = (float)t;
                 }
+            float4* up = reinterpret_cast<float4*>(b.offrow + 36 * (size_t)su);
+            float4* lo = reinterpret_cast<float4*>(b.offrow + 36 * (size_t)sl);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                up[q] = make_float4(Ah[4 * q], Ah[4 * q + 1], Ah[4 * q + 2], Ah[4 * q + 3]);
+                // transposed: word w = 6 bb + a holds Ah[6 a + bb]
+                float tq[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int w = 4 * q + t;
+                    tq[t] = Ah[6 * (w % 6) + w / 6];
+                }
+                lo[q] = make_float4(tq[0], tq[1], tq[2], tq[3]);
+            }
+            b.row_col[su] = j;
+            b.row_col[sl] = i;
         } else {
             const uint32_t n = x - P;
+            b.row_ptr[n] = b.low_offset[n] + b.pair_offset[n];
+            if (n + 1 == N) b.row_ptr[N] = b.low_offset[N] + b.pair_offset[N];
             const double* L = b.linv64 + kTri * (size_t)n;
 #pragma unroll
             for (int a = 0; a < 6; ++a) {
@@ -384,9 +411,27 @@ __global__ void __launch_bounds__(256) cg_transform_kernel(const Buffers* __rest
 #pragma unroll
                 for (int k = 0; k <= a; ++k) t += L[tri_index(a, k)] * (double)b.jtr[6 * (size_t)n + k];
                 b.cg_b[6 * (size_t)n + a] = (float)t;
-                b.cg_r[0][6 * (size_t)n + a] = (float)t;
+                b.cg_rws[0][kRws * (size_t)n + a] = (float)t;
             }
         }
+    }
+}
+
+// B_ij in place of S_ij (offdiag) for inspection and the materialized SpMV
+// stage; the same arithmetic as the transform.
+__global__ void __launch_bounds__(256) pair_blocks_kernel(const Buffers* __restrict__ bs, Penalty pen) {
+    const Buffers b = frame_buffers(bs);
+    const FrameDev& f = *b.frame;
+    const Counts& c = *b.counts;
+    if (c.error) return;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < c.pairs; p += stride) {
+        const uint32_t key = b.pair_key[p];
+        float B[36];
+        pair_block(b, c, f, pen, p, key >> 16, key & 0xFFFF, B);
+        float4* out = reinterpret_cast<float4*>(b.offdiag + 36 * (size_t)p);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) out[q] = make_float4(B[4 * q], B[4 * q + 1], B[4 * q + 2], B[4 * q + 3]);
     }
 }
 
@@ -396,26 +441,12 @@ template <typename Load, typename Acc = double>
 __device__ inline void hat_row_apply(const Buffers& b, uint32_t i, uint32_t sub, const Load& load, Acc y[6]) {
 #pragma unroll
     for (int a = 0; a < 6; ++a) y[a] = 0;
-    const uint32_t lo_beg = b.low_offset[i], lo_n = b.low_offset[i + 1] - lo_beg;
-    const uint32_t up_beg = b.pair_offset[i], up_n = b.pair_offset[i + 1] - up_beg;
-    for (uint32_t e = sub; e < lo_n + up_n; e += kRowLanes) {
-        const float* B;
-        bool tr;
-        uint32_t j;
-        if (e < lo_n) {
-            const uint2 le = b.low_ent[lo_beg + e];
-            B = b.offhat + 36 * (size_t)le.y;
-            tr = true;
-            j = le.x;
-        } else {
-            const uint32_t p = up_beg + (e - lo_n);
-            B = b.offhat + 36 * (size_t)p;
-            tr = false;
-            j = b.pair_key[p] & 0xFFFF;
-        }
+    const uint32_t beg = b.row_ptr[i], n = b.row_ptr[i + 1] - beg;
+    for (uint32_t e = sub; e < n; e += kRowLanes) {
+        const uint32_t j = b.row_col[beg + e];
         float v[6];
         load(j, v);
-        block_mv<Acc>(B, tr, v, y);
+        block_mv<Acc>(b.offrow + 36 * (size_t)(beg + e), false, v, y);
     }
 #pragma unroll
     for (int a = 0; a < 6; ++a)
@@ -431,10 +462,11 @@ __global__ void __launch_bounds__(256) cg_init_kernel(const Buffers* __restrict_
     const uint32_t sub = threadIdx.x % kRowLanes;
     double g = 0.0, d = 0.0;
     if (!b.counts->error) {
-        const float* R = b.cg_r[0];
+        float* RWS = b.cg_rws[0];
         auto load = [&](uint32_t j, float v[6]) {
-#pragma unroll
-            for (int a = 0; a < 6; ++a) v[a] = R[6 * (size_t)j + a];
+            const float4 r03 = *reinterpret_cast<const float4*>(RWS + kRws * (size_t)j);
+            const float2 r45 = *reinterpret_cast<const float2*>(RWS + kRws * (size_t)j + 4);
+            v[0] = r03.x; v[1] = r03.y; v[2] = r03.z; v[3] = r03.w; v[4] = r45.x; v[5] = r45.y;
         };
         const uint32_t groups = gridDim.x * kRowsPerBlock;
         for (uint32_t base = wi.block * kRowsPerBlock; base < f.N; base += groups) {
@@ -445,10 +477,11 @@ __global__ void __launch_bounds__(256) cg_init_kernel(const Buffers* __restrict_
 #pragma unroll
                 for (int a = 0; a < 6; ++a) {
                     const size_t k = 6 * (size_t)i + a;
-                    const double r = R[k];
+                    float* rec = RWS + kRws * (size_t)i;
+                    const double r = rec[a];
                     const double w = (double)(float)(y[a] + r);
-                    b.cg_w[0][k] = (float)w;
-                    b.cg_s[1][k] = 0.f;
+                    rec[6 + a] = (float)w;  // w_0
+                    rec[12 + a] = 0.f;      // s_-1
                     b.cg_p[k] = 0.f;
                     b.cg_x[k] = 0.f;
                     g += r * r;
@@ -495,23 +528,26 @@ __global__ void __launch_bounds__(256) cg_iter_kernel(const Buffers* __restrict_
     cg_scalars(b, i, gridDim.x, alpha, beta, wi.block);
     double g = 0.0, d = 0.0;
     if (!b.counts->error) {
+        // reads (r_i, w_i, s_{i-1}), writes (r_{i+1}, w_{i+1}, s_i)
+        // (selects, not an index into the by-value struct: that would move it to scratch)
         const bool odd = (i & 1) != 0;
-        const float* R = odd ? b.cg_r[1] : b.cg_r[0];
-        const float* W = odd ? b.cg_w[1] : b.cg_w[0];
-        const float* S = odd ? b.cg_s[0] : b.cg_s[1];
-        float* Ro = odd ? b.cg_r[0] : b.cg_r[1];
-        float* Wo = odd ? b.cg_w[0] : b.cg_w[1];
-        float* So = odd ? b.cg_s[1] : b.cg_s[0];
+        const float* RWS = odd ? b.cg_rws[1] : b.cg_rws[0];
+        float* RWSo = odd ? b.cg_rws[0] : b.cg_rws[1];
         // r_{i+1} of node j, rounded as it is stored
         // FP32 vector updates (one rounding each, fused): every row forms its
         // columns' r_{i+1} exactly as their owners store it
         const float af = (float)alpha, bf = (float)beta;
         auto next_r = [&](uint32_t j, float v[6]) {
+            const float4* q = reinterpret_cast<const float4*>(RWS + kRws * (size_t)j);
+            const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+            const float2 q4 = *reinterpret_cast<const float2*>(RWS + kRws * (size_t)j + 16);
+            const float r[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+            const float w[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+            const float sp[6] = {q3.x, q3.y, q3.z, q3.w, q4.x, q4.y};
 #pragma unroll
             for (int a = 0; a < 6; ++a) {
-                const size_t k = 6 * (size_t)j + a;
-                const float s_i = __builtin_fmaf(bf, S[k], W[k]);
-                v[a] = __builtin_fmaf(-af, s_i, R[k]);
+                const float s_i = __builtin_fmaf(bf, sp[a], w[a]);
+                v[a] = __builtin_fmaf(-af, s_i, r[a]);
             }
         };
         const uint32_t groups = gridDim.x * kRowsPerBlock;
@@ -521,11 +557,12 @@ __global__ void __launch_bounds__(256) cg_iter_kernel(const Buffers* __restrict_
             // before the row apply so that their latency overlaps it
             const bool own = n < f.N && sub < 6;
             const size_t ka = 6 * (size_t)(n < f.N ? n : 0) + (sub < 6 ? sub : 0);
+            const size_t kr = kRws * (size_t)(n < f.N ? n : 0) + (sub < 6 ? sub : 0);
             float rk = 0.f, wk = 0.f, sk = 0.f, pk = 0.f, xk = 0.f;
             if (own) {
-                rk = R[ka];
-                wk = W[ka];
-                sk = S[ka];
+                rk = RWS[kr];
+                wk = RWS[kr + 6];
+                sk = RWS[kr + 12];
                 pk = b.cg_p[ka];
                 xk = b.cg_x[ka];
             }
@@ -537,9 +574,9 @@ __global__ void __launch_bounds__(256) cg_iter_kernel(const Buffers* __restrict_
                 const float r_next = __builtin_fmaf(-af, s_i, rk);
                 const float p_i = __builtin_fmaf(bf, pk, rk);
                 const double w_next = (double)(ya + r_next);
-                So[ka] = s_i;
-                Ro[ka] = r_next;
-                Wo[ka] = (float)w_next;
+                RWSo[kr] = r_next;
+                RWSo[kr + 6] = (float)w_next;
+                RWSo[kr + 12] = s_i;
                 b.cg_p[ka] = p_i;
                 b.cg_x[ka] = __builtin_fmaf(af, p_i, xk);
                 g += (double)r_next * (double)r_next;
@@ -562,8 +599,9 @@ __global__ void __launch_bounds__(256) cg_refine_residual_kernel(const Buffers* 
     if (!b.counts->error) {
         const float* X = b.cg_x;
         auto load = [&](uint32_t j, float v[6]) {
-#pragma unroll
-            for (int a = 0; a < 6; ++a) v[a] = X[6 * (size_t)j + a];
+            const float2* p = reinterpret_cast<const float2*>(X + 6 * (size_t)j);
+            const float2 p0 = p[0], p1 = p[1], p2 = p[2];
+            v[0] = p0.x; v[1] = p0.y; v[2] = p1.x; v[3] = p1.y; v[4] = p2.x; v[5] = p2.y;
         };
         const uint32_t groups = gridDim.x * kRowsPerBlock;
         for (uint32_t base = wi.block * kRowsPerBlock; base < f.N; base += groups) {
@@ -594,8 +632,9 @@ __global__ void __launch_bounds__(256) cg_refine_curvature_kernel(const Buffers*
     if (!b.counts->error) {
         const float* Q = b.q;
         auto load = [&](uint32_t j, float v[6]) {
-#pragma unroll
-            for (int a = 0; a < 6; ++a) v[a] = Q[6 * (size_t)j + a];
+            const float2* p = reinterpret_cast<const float2*>(Q + 6 * (size_t)j);
+            const float2 p0 = p[0], p1 = p[1], p2 = p[2];
+            v[0] = p0.x; v[1] = p0.y; v[2] = p1.x; v[3] = p1.y; v[4] = p2.x; v[5] = p2.y;
         };
         const uint32_t groups = gridDim.x * kRowsPerBlock;
         for (uint32_t base = wi.block * kRowsPerBlock; base < f.N; base += groups) {
@@ -830,6 +869,11 @@ void launch_cg_solve(const Batch& B, uint32_t iters, uint32_t refine, uint32_t k
 }
 
 uint32_t row_group_grid(uint32_t nodes) { return spmv_grid(nodes); }
+
+void launch_pair_blocks(const Batch& B, const Penalty& pen, hipStream_t stream) {
+    const uint32_t grid = std::min<uint32_t>((8 * B.nodes + 255) / 256, 2048u);
+    hipLaunchKernelGGL(pair_blocks_kernel, dim3(grid, B.frames), dim3(256), 0, stream, B.dev, pen);
+}
 
 void launch_pcg_init(const Batch& B, bool fp64, hipStream_t stream) {
     if (fp64) hipLaunchKernelGGL(pcg_init_kernel<double>, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev);

@@ -518,12 +518,13 @@ struct Lambda2 {
           feature(p.feature * p.feature), cross(p.cross * p.cross) {}
 };
 
-// LDS rows of one staged entry: G_t values 0..15 (lo), values 16..26 and
-// zeros (hi), the weight row (16 columns and a padding column).  The row
-// strides (18, 14, 18 words) keep the lanes' float2 stores of their own rows
-// and the product's column reads spread over the banks.
-constexpr uint32_t kLo = 18, kHi = 14, kW = 18;
-static_assert(kW > kGramCols, "the weight row needs a padding column");
+// LDS rows of one staged entry, 16 words each: G_t values 0..15 (lo), values
+// 16..26 and zeros (hi), the weight row.  Word c of row r sits at
+// 16 r + (c ^ swz(r)), swz(r) = 2 ((r >> 1) & 7): a lane's float2 stores of
+// its own row and the product's column reads (4 rows x 16 columns per step)
+// both fall on distinct banks.
+constexpr uint32_t kRowWords = 16;
+__device__ inline uint32_t swz(uint32_t r) { return 2 * ((r >> 1) & 7); }
 
 // G_t of `rows` Jacobian rows (world frame; moved to the node frame of g here)
 // with their lambda^2 and residuals, stored to the lane's LDS rows.  Row 0
@@ -557,12 +558,14 @@ __device__ inline void store_gram(float J[3][6], const float res[3], const float
             }
         }
     }
+    const uint32_t f = swz(threadIdx.x);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) *reinterpret_cast<float2*>(gl + 2 * q) = make_float2(G[2 * q], G[2 * q + 1]);
+    for (int q = 0; q < 8; ++q) *reinterpret_cast<float2*>(gl + ((2 * q) ^ f)) = make_float2(G[2 * q], G[2 * q + 1]);
 #pragma unroll
-    for (int q = 0; q < 5; ++q) *reinterpret_cast<float2*>(gh + 2 * q) = make_float2(G[16 + 2 * q], G[17 + 2 * q]);
-    *reinterpret_cast<float2*>(gh + 10) = make_float2(G[26], 0.f);
-    *reinterpret_cast<float2*>(gh + 12) = make_float2(0.f, 0.f);
+    for (int q = 0; q < 5; ++q) *reinterpret_cast<float2*>(gh + ((2 * q) ^ f)) = make_float2(G[16 + 2 * q], G[17 + 2 * q]);
+    *reinterpret_cast<float2*>(gh + (10 ^ f)) = make_float2(G[26], 0.f);
+#pragma unroll
+    for (int q = 6; q < 8; ++q) *reinterpret_cast<float2*>(gh + ((2 * q) ^ f)) = make_float2(0.f, 0.f);
 }
 
 // Partner columns of row i held by the index or searched in the row's pair keys.
@@ -577,11 +580,12 @@ struct GramColumns {
 // holding j (every such slot stores the same sum).  Slots outside this pass
 // store into the padding column.
 __device__ inline void store_knn_weights(const Buffers& b, const GramColumns& gc, ushort4 knn, float4 w, uint32_t i,
-                                         uint32_t pc, float* wl) {
+                                         uint32_t pc, float* wl, float* wdummy) {
     const float wi = slot_weight(knn, w, i);
+    const uint32_t f = swz(threadIdx.x);
     if (gc.c0 == 0) {
-        wl[0] = wi * wi;
-        wl[1] = wi;
+        wl[0 ^ f] = wi * wi;
+        wl[1 ^ f] = wi;
     }
     if (gc.mode == 0) return;
     const uint32_t n[4] = {knn.x, knn.y, knn.z, knn.w};
@@ -627,7 +631,7 @@ __device__ inline void store_knn_weights(const Buffers& b, const GramColumns& gc
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const uint32_t r = col[q] - gc.c0;
-        wl[wq[q] != 0.f && r < kGramCols ? r : kGramCols] = sum[q];
+        *(wq[q] != 0.f && r < kGramCols ? wl + (r ^ f) : wdummy) = sum[q];
     }
 }
 
@@ -635,14 +639,16 @@ __device__ inline void store_knn_weights(const Buffers& b, const GramColumns& gc
 // row and its weight row (zeros for an empty lane).
 __device__ inline void stage_entry(const Buffers& b, const TermBounds& c, const Lambda2& l2, bool density,
                                    const GramColumns& gc, uint32_t t, const RawRec& raw, uint32_t pc, uint32_t i,
-                                   const float4 g, float* gl, float* gh, float* wl) {
+                                   const float4 g, float* gl, float* gh, float* wl, float* wdummy) {
+    // zero rows (swizzled positions: the lanes' stores stay on distinct banks)
+    const uint32_t fz = swz(threadIdx.x);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) *reinterpret_cast<float2*>(wl + 2 * q) = make_float2(0.f, 0.f);
+    for (int q = 0; q < 8; ++q) *reinterpret_cast<float2*>(wl + ((2 * q) ^ fz)) = make_float2(0.f, 0.f);
     if (t == 0xFFFFFFFFu) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) *reinterpret_cast<float2*>(gl + 2 * q) = make_float2(0.f, 0.f);
+        for (int q = 0; q < 8; ++q) *reinterpret_cast<float2*>(gl + ((2 * q) ^ fz)) = make_float2(0.f, 0.f);
 #pragma unroll
-        for (int q = 0; q < 7; ++q) *reinterpret_cast<float2*>(gh + 2 * q) = make_float2(0.f, 0.f);
+        for (int q = 0; q < 8; ++q) *reinterpret_cast<float2*>(gh + ((2 * q) ^ fz)) = make_float2(0.f, 0.f);
         return;
     }
     uint32_t k;
@@ -656,8 +662,9 @@ __device__ inline void stage_entry(const Buffers& b, const TermBounds& c, const 
         lam2[0] = lam2[1] = lam2[2] = l2.smooth;
         store_gram<3>(J, res, lam2, 3, g, gl, gh);
         if (gc.c0 == 0) {
-            wl[0] = 1.f;
-            wl[1] = 1.f;
+            const uint32_t f = swz(threadIdx.x);
+            wl[0 ^ f] = 1.f;
+            wl[1 ^ f] = 1.f;
         }
         return;
     }
@@ -695,7 +702,7 @@ __device__ inline void stage_entry(const Buffers& b, const TermBounds& c, const 
         lam2[0] = lam2[1] = lam2[2] = l;
         store_gram<3>(J, res, lam2, 3, g, gl, gh);
     }
-    store_knn_weights(b, gc, knn, w, i, pc, wl);
+    store_knn_weights(b, gc, knn, w, i, pc, wl, wdummy);
 }
 
 // diag_only: the diagonal blocks and J^T r alone (the matrix-free solve).
@@ -704,15 +711,21 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
-    __shared__ float Glo[64 * kLo];
-    __shared__ float Ghi[64 * kHi + 2];  // a read of values 30, 31 of row 63 lands in the zeroed tail
-    __shared__ float Wl[64 * kW];
+    __shared__ float Glo[64 * kRowWords];
+    __shared__ float Ghi[64 * kRowWords];
+    __shared__ float Wl[64 * kRowWords];
+    __shared__ float Wd[64];  // weight stores of slots outside the pass
     const uint32_t lane = threadIdx.x;
-    if (lane < 2) Ghi[64 * kHi + lane] = 0.f;
     const uint32_t m = lane & 15, kq = lane >> 4;
-    float* gl = Glo + kLo * lane;
-    float* gh = Ghi + kHi * lane;
-    float* wl = Wl + kW * lane;
+    float* gl = Glo + kRowWords * lane;
+    float* gh = Ghi + kRowWords * lane;
+    float* wl = Wl + kRowWords * lane;
+    float* wd = Wd + lane;
+    // read offsets of lane (m, kq): row 4 u + kq, word m ^ swz(row); swz
+    // depends on u only through u mod 4
+    uint32_t roff[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) roff[j] = kRowWords * kq + (m ^ swz(4 * j + kq));
     const Lambda2 l2(pen);
     const bool density = b.density && pen.density != 0.f;
     const TermBounds tb(c);
@@ -737,7 +750,7 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
             uint32_t t_nxt = beg + 64 + lane < end ? b.n2t_index[beg + 64 + lane] : 0xFFFFFFFFu;
             uint32_t pc_nxt = cols_in && beg + 64 + lane < end ? b.n2t_col[beg + 64 + lane] : 0xFFFFFFFFu;
             for (uint32_t e0 = beg; e0 < end; e0 += 64) {
-                stage_entry(b, tb, l2, density, gc, t_cur, rec_cur, pc_cur, i, g, gl, gh, wl);
+                stage_entry(b, tb, l2, density, gc, t_cur, rec_cur, pc_cur, i, g, gl, gh, wl, wd);
                 const RawRec rec_nxt = load_record(b, tb, t_nxt);
                 const uint32_t enn = e0 + 128 + lane;
                 const uint32_t t_nn = enn < end ? b.n2t_index[enn] : 0xFFFFFFFFu;
@@ -750,10 +763,10 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
                     float a0[16], a1[16], bv[16];
 #pragma unroll
                     for (int u = 0; u < 16; ++u) {
-                        const uint32_t k = 4 * u + kq;
-                        a0[u] = Glo[kLo * k + m];
-                        a1[u] = Ghi[kHi * k + m];
-                        bv[u] = Wl[kW * k + m];
+                        const uint32_t o = 4 * kRowWords * u + roff[u & 3];
+                        a0[u] = Glo[o];
+                        a1[u] = Ghi[o];
+                        bv[u] = Wl[o];
                     }
 #pragma unroll
                     for (int u = 0; u < 16; ++u) {
@@ -766,10 +779,10 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
                     float a0[4], a1[4], bv[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        const uint32_t k = 4 * (st + u) + kq;
-                        a0[u] = Glo[kLo * k + m];
-                        a1[u] = Ghi[kHi * k + m];
-                        bv[u] = Wl[kW * k + m];
+                        const uint32_t o = 4 * kRowWords * (st + u) + roff[u];  // st is a multiple of 4
+                        a0[u] = Glo[o];
+                        a1[u] = Ghi[o];
+                        bv[u] = Wl[o];
                     }
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
@@ -778,8 +791,9 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
                     }
                 }
                 for (; st < steps; ++st) {
-                    const uint32_t k = 4 * st + kq;
-                    const float a0 = Glo[kLo * k + m], a1 = Ghi[kHi * k + m], bv = Wl[kW * k + m];
+                    const uint32_t j = st & 3;
+                    const uint32_t o = 4 * kRowWords * st + (j == 0 ? roff[0] : j == 1 ? roff[1] : j == 2 ? roff[2] : roff[3]);
+                    const float a0 = Glo[o], a1 = Ghi[o], bv = Wl[o];
                     lo = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bv, lo, 0, 0, 0);
                     hi = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bv, hi, 0, 0, 0);
                 }

@@ -16,6 +16,7 @@ void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, int
 void launch_pair_term_lists(const Batch& B, hipStream_t stream);
 uint32_t index_chunks(uint64_t max_terms);
 void launch_eval_terms(const Batch& B, uint32_t which, bool fp64_geometry, hipStream_t stream);
+void launch_pair_blocks(const Batch& B, const Penalty& pen, hipStream_t stream);
 void launch_assemble(const Batch& B, const Penalty& pen, bool gram, bool diag_only, hipStream_t stream);
 void launch_pcg_matrix_free(const Batch& B, uint32_t iters, uint32_t refine, bool fp64, const Penalty& pen,
                             hipStream_t stream);
@@ -46,6 +47,7 @@ struct mvdr_solver {
     bool has_input = false;
     bool indexed = false;
     bool pair_terms = false;            // NodePair2Term lists of the current index present
+    bool pair_blocks = true;            // offdiag holds B_ij (mixed mode: S_ij until formed on demand)
     uint32_t graph_nodes = 0;           // Batch::nodes the captured graph was sized for
     uint32_t iter = 0;                  // GN iterations done on the current frame
     hipEvent_t done = nullptr;          // end of the last enqueued work
@@ -170,6 +172,11 @@ void enqueue_batch(const Batch& B, const mvdr_solver_config& c, hipStream_t stre
 
 // Grid sizing node count: the frame's node count rounded up to 64.
 uint32_t live_nodes(uint32_t n) { return std::max<uint32_t>(64, (n + 63) / 64 * 64); }
+
+// After an assembly: the mixed mode's fused CG leaves S_ij in offdiag.
+bool blocks_after_assembly(const mvdr_solver_config& c) {
+    return !(c.precision == MVDR_PRECISION_MIXED && c.jtj_mode != MVDR_JTJ_MATRIX_FREE);
+}
 
 void enqueue_solve(mvdr_solver* s, hipStream_t stream, bool index, uint32_t first_iter, uint32_t iters) {
     s->self.nodes = live_nodes(s->frame_host.N);
@@ -404,14 +411,20 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     A(b.offdiag, 36 * (size_t)b.cap_pairs);
     for (float** v : {&b.x, &b.r, &b.s, &b.q, &b.d0, &b.d1}) A(*v, 6 * (size_t)c.max_nodes);
     b.linv64 = nullptr;
-    b.offhat = nullptr;
+    b.offrow = nullptr;
+    b.row_col = nullptr;
+    b.row_ptr = nullptr;
+    b.low_pos = nullptr;
     b.cg_scal = nullptr;
     if (factors64) A(b.linv64, kTri * (size_t)c.max_nodes);
     if (c.precision == MVDR_PRECISION_MIXED && !mf) {
-        A(b.offhat, 36 * (size_t)b.cap_pairs);
-        for (float** v : {&b.cg_r[0], &b.cg_r[1], &b.cg_w[0], &b.cg_w[1], &b.cg_s[0], &b.cg_s[1], &b.cg_p, &b.cg_x,
-                          &b.cg_b})
-            A(*v, 6 * (size_t)c.max_nodes);
+        A(b.offrow, 2 * 36 * (size_t)b.cap_pairs);
+        A(b.row_col, 2 * (size_t)b.cap_pairs);
+        A(b.row_ptr, (size_t)c.max_nodes + 1);
+        A(b.low_pos, b.cap_pairs);
+        A(b.cg_rws[0], kRws * (size_t)c.max_nodes);
+        A(b.cg_rws[1], kRws * (size_t)c.max_nodes);
+        for (float** v : {&b.cg_p, &b.cg_x, &b.cg_b}) A(*v, 6 * (size_t)c.max_nodes);
         A(b.cg_scal, 2 * 72);
     }
     A(b.partial, (size_t)kPartialSlots * kPartialStride);
@@ -545,6 +558,7 @@ mvdr_status mvdr_solver_gn_iteration(mvdr_solver* s, void* stream) {
     HIP_TRY(s, hipGetLastError());
     HIP_TRY(s, hipEventRecord(s->done, st));
     ++s->iter;
+    s->pair_blocks = blocks_after_assembly(s->cfg);
     return MVDR_OK;
 }
 
@@ -585,6 +599,7 @@ mvdr_status mvdr_solver_solve(mvdr_solver* s, void* stream) {
     HIP_TRY(s, hipEventRecord(s->done, st));
     s->indexed = true;
     s->iter = gn;
+    s->pair_blocks = gn == 0 || blocks_after_assembly(s->cfg);
     return MVDR_OK;
 }
 
@@ -653,6 +668,13 @@ mvdr_status mvdr_solver_debug_get(mvdr_solver* s, int tap, void* dst, size_t byt
         HIP_TRY(s, hipStreamSynchronize(nullptr));
         s->pair_terms = true;
     }
+    if (tap == MVDR_TAP_JTJ_OFFDIAG && !s->pair_blocks && s->iter > 0) {
+        s->self.nodes = live_nodes(s->frame_host.N);
+        launch_pair_blocks(s->self, kernel_penalty(s->cfg, s->iter - 1), nullptr);
+        HIP_TRY(s, hipGetLastError());
+        HIP_TRY(s, hipStreamSynchronize(nullptr));
+        s->pair_blocks = true;
+    }
     Counts c;
     HIP_TRY(s, hipMemcpy(&c, s->b.counts, sizeof(Counts), hipMemcpyDeviceToHost));
     const Buffers& b = s->b;
@@ -715,7 +737,7 @@ mvdr_status mvdr_solver_debug_get(mvdr_solver* s, int tap, void* dst, size_t byt
 // launches of one stage on the data of GN iteration k, timed with hipEvents
 // on `st`.
 static mvdr_status bench_stage_impl(const Batch& B, const mvdr_solver_config& c, uint32_t k, int stage, uint32_t reps,
-                                    hipStream_t st, float* avg_ms, std::string& err) {
+                                    hipStream_t st, float* avg_ms, std::string& err, bool& pair_blocks) {
     const bool fp64 = c.precision == MVDR_PRECISION_MIXED;
     const bool mf = c.jtj_mode == MVDR_JTJ_MATRIX_FREE;
     if (stage < MVDR_STAGE_EVAL_TERMS || stage > MVDR_STAGE_CG_ITERATION) {
@@ -727,6 +749,10 @@ static mvdr_status bench_stage_impl(const Batch& B, const mvdr_solver_config& c,
         return MVDR_E_INVALID;
     }
     const Penalty pen = kernel_penalty(c, k);
+    if (stage == MVDR_STAGE_SPMV && !mf && !pair_blocks) {
+        launch_pair_blocks(B, pen, st);  // the materialized SpMV reads B_ij
+        pair_blocks = true;
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
@@ -755,6 +781,7 @@ static mvdr_status bench_stage_impl(const Batch& B, const mvdr_solver_config& c,
         err = std::string("HIP: ") + hipGetErrorString(e);
         return MVDR_E_HIP;
     }
+    if (stage == MVDR_STAGE_ASSEMBLE) pair_blocks = blocks_after_assembly(c);
     *avg_ms = ms / (float)reps;
     return MVDR_OK;
 }
@@ -763,7 +790,7 @@ mvdr_status mvdr_solver_bench_stage(mvdr_solver* s, int stage, uint32_t reps, vo
     if (!s || !avg_ms || reps == 0) return MVDR_E_INVALID;
     if (!s->indexed || s->iter == 0) return fail(s, MVDR_E_STATE, "bench_stage needs a completed solve");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const mvdr_status rc = bench_stage_impl(s->self, s->cfg, s->iter - 1, stage, reps, st, avg_ms, s->error);
+    const mvdr_status rc = bench_stage_impl(s->self, s->cfg, s->iter - 1, stage, reps, st, avg_ms, s->error, s->pair_blocks);
     if (rc == MVDR_OK) HIP_TRY(s, hipEventRecord(s->done, st));
     return rc;
 }
@@ -825,7 +852,9 @@ mvdr_status mvdr_batch_bench_stage(mvdr_batch* bt, int stage, uint32_t reps, voi
             return MVDR_E_STATE;
         }
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const mvdr_status rc = bench_stage_impl(bt->batch, bt->cfg, m0->iter - 1, stage, reps, st, avg_ms, bt->error);
+    bool pair_blocks = m0->pair_blocks;
+    const mvdr_status rc = bench_stage_impl(bt->batch, bt->cfg, m0->iter - 1, stage, reps, st, avg_ms, bt->error, pair_blocks);
+    for (mvdr_solver* m : bt->members) m->pair_blocks = pair_blocks;
     if (rc == MVDR_OK)
         for (mvdr_solver* m : bt->members) (void)hipEventRecord(m->done, st);
     return rc;
@@ -876,6 +905,7 @@ mvdr_status mvdr_batch_solve(mvdr_batch* bt, void* stream) {
         m->indexed = true;
         m->pair_terms = bt->cfg.precision != MVDR_PRECISION_MIXED && bt->cfg.jtj_mode != MVDR_JTJ_MATRIX_FREE;
         m->iter = gn;
+        m->pair_blocks = gn == 0 || blocks_after_assembly(bt->cfg);
     }
     if (e != hipSuccess) {
         bt->error = std::string("HIP: ") + hipGetErrorString(e);
