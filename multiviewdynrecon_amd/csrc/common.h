@@ -346,11 +346,12 @@ struct Buffers {
     float *x, *r, *s, *q, *d0, *d1;
     // fused CG on A^ = L^-1 A L^-T (mixed precision mode)
     double* linv64;          // 21 N: L^-1 of each diagonal block, packed lower triangle (tri_index)
-    // L^-1 A L^-T by block rows: row i holds its lower blocks (low_ent order)
-    // then its upper blocks (pair order), every block row-major for row i;
+    // L^-1 A L^-T: the upper blocks A^_ij (pair order, row-major) and, by
+    // block rows, the slots of row i -- its lower entries (low_ent order), then
+    // its upper entries (pair order) -- as (column node, pair | transposed << 31);
     // row i starts at slot low_offset[i] + pair_offset[i]
-    float* offrow;           // 36 x 2P
-    uint32_t* row_col;       // 2P: column node of each slot
+    float* offhat;           // 36 x P
+    uint2* row_ent;          // 2P
     uint32_t* row_ptr;       // N + 1: first slot of each block row
     // fused CG: per node one record (r 0..5, w 6..11, s 12..17, 2 pad) of the
     // set read by iteration i (cg_rws[i & 1]) and of the set it writes

@@ -261,9 +261,10 @@ constexpr uint32_t kSlotCgDelta = 72;  // delta_i partials: slot 72 + i
 // B_ij = S_ij M_ij (the change of centre g_i -> g_j: rows e_c x (g_j - g_i) add
 // the translation part to the rotation part) plus the pair blocks of the graph
 // edges joining i and j, in edge order, gives A^_ij = L_i^-1 B_ij L_j^-T (FP64
-// factors) in block row i and its transpose A^_ji in block row j; the nodes get
-// b^ = L^-1 b and their block-row starts.  B_ij itself (J^T J off-diagonal,
-// node frame) is not stored: pair_blocks_kernel forms it on demand.
+// factors), stored once and named by a slot of block row i and a transposed
+// slot of block row j; the nodes get b^ = L^-1 b and their block-row starts.
+// B_ij itself (J^T J off-diagonal, node frame) is not stored:
+// pair_blocks_kernel forms it on demand.
 // B_ij of pair p = (i, j): the row Gram assembly's S_ij (offdiag, 21 values)
 // times the change of centre M_ij, plus the pair blocks of the graph edges
 // joining i and j, in edge order.
@@ -372,7 +373,7 @@ __global__ void __launch_bounds__(256) cg_transform_kernel(const Buffers* __rest
                     for (int k = 0; k <= a; ++k) t += Li[tri_index(a, k)] * (double)B[6 * k + cc];
                     T[6 * a + cc] = t;
                 }
-            // A^_ij in row i's upper slot, its transpose in row j's lower slot
+            // A^_ij stored once; row i's upper slot and row j's lower slot name it
             const uint32_t su = b.low_offset[i + 1] + p, sl = b.pair_offset[j] + b.low_pos[p];
             float Ah[36];
 #pragma unroll
@@ -384,22 +385,11 @@ __global__ void __launch_bounds__(256) cg_transform_kernel(const Buffers* __rest
                     for (int cc = 0; cc <= bb; ++cc) t += T[6 * a + cc] * Lj[tri_index(bb, cc)];
                     Ah[6 * a + bb] = (float)t;
                 }
-            float4* up = reinterpret_cast<float4*>(b.offrow + 36 * (size_t)su);
-            float4* lo = reinterpret_cast<float4*>(b.offrow + 36 * (size_t)sl);
+            float4* up = reinterpret_cast<float4*>(b.offhat + 36 * (size_t)p);
 #pragma unroll
-            for (int q = 0; q < 9; ++q) {
-                up[q] = make_float4(Ah[4 * q], Ah[4 * q + 1], Ah[4 * q + 2], Ah[4 * q + 3]);
-                // transposed: word w = 6 bb + a holds Ah[6 a + bb]
-                float tq[4];
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int w = 4 * q + t;
-                    tq[t] = Ah[6 * (w % 6) + w / 6];
-                }
-                lo[q] = make_float4(tq[0], tq[1], tq[2], tq[3]);
-            }
-            b.row_col[su] = j;
-            b.row_col[sl] = i;
+            for (int q = 0; q < 9; ++q) up[q] = make_float4(Ah[4 * q], Ah[4 * q + 1], Ah[4 * q + 2], Ah[4 * q + 3]);
+            b.row_ent[su] = make_uint2(j, p);
+            b.row_ent[sl] = make_uint2(i, p | 0x80000000u);
         } else {
             const uint32_t n = x - P;
             b.row_ptr[n] = b.low_offset[n] + b.pair_offset[n];
@@ -443,10 +433,10 @@ __device__ inline void hat_row_apply(const Buffers& b, uint32_t i, uint32_t sub,
     for (int a = 0; a < 6; ++a) y[a] = 0;
     const uint32_t beg = b.row_ptr[i], n = b.row_ptr[i + 1] - beg;
     for (uint32_t e = sub; e < n; e += kRowLanes) {
-        const uint32_t j = b.row_col[beg + e];
+        const uint2 s = b.row_ent[beg + e];
         float v[6];
-        load(j, v);
-        block_mv<Acc>(b.offrow + 36 * (size_t)(beg + e), false, v, y);
+        load(s.x, v);
+        block_mv<Acc>(b.offhat + 36 * (size_t)(s.y & 0x7FFFFFFFu), (s.y >> 31) != 0, v, y);
     }
 #pragma unroll
     for (int a = 0; a < 6; ++a)

@@ -411,15 +411,15 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     A(b.offdiag, 36 * (size_t)b.cap_pairs);
     for (float** v : {&b.x, &b.r, &b.s, &b.q, &b.d0, &b.d1}) A(*v, 6 * (size_t)c.max_nodes);
     b.linv64 = nullptr;
-    b.offrow = nullptr;
-    b.row_col = nullptr;
+    b.offhat = nullptr;
+    b.row_ent = nullptr;
     b.row_ptr = nullptr;
     b.low_pos = nullptr;
     b.cg_scal = nullptr;
     if (factors64) A(b.linv64, kTri * (size_t)c.max_nodes);
     if (c.precision == MVDR_PRECISION_MIXED && !mf) {
-        A(b.offrow, 2 * 36 * (size_t)b.cap_pairs);
-        A(b.row_col, 2 * (size_t)b.cap_pairs);
+        A(b.offhat, 36 * (size_t)b.cap_pairs);
+        A(b.row_ent, 2 * (size_t)b.cap_pairs);
         A(b.row_ptr, (size_t)c.max_nodes + 1);
         A(b.low_pos, b.cap_pairs);
         A(b.cg_rws[0], kRws * (size_t)c.max_nodes);
