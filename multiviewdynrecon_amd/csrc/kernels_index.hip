@@ -468,6 +468,36 @@ __global__ void __launch_bounds__(64 * kScanWaves) n2t_colscan_kernel(const Buff
     if (w == kScanWaves - 1) b.n2t_count[n] = total;
 }
 
+// Bitonic sort of a wave's 256 keys, ascending; key e sits in lane e / 4,
+// register e % 4.  Pairs 4 or more keys apart are exchanged across lanes.
+__device__ inline void wave_sort256(uint32_t k[4]) {
+    const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+    for (uint32_t size = 2; size <= 256; size <<= 1) {
+#pragma unroll
+        for (uint32_t d = size >> 1; d > 0; d >>= 1) {
+            if (d >= 4) {
+                // bits d and size of e = 4 lane + j come from the lane
+                const bool keep_min = (((4 * lane) & size) == 0) == (((4 * lane) & d) == 0);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t o = (uint32_t)__shfl_xor((int)k[j], (int)(d >> 2), 64);
+                    k[j] = keep_min ? min(k[j], o) : max(k[j], o);
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (j & d) continue;
+                    const bool up = ((4 * lane + j) & size) == 0;
+                    const uint32_t lo = min(k[j], k[j ^ d]), hi = max(k[j], k[j ^ d]);
+                    k[j] = up ? lo : hi;
+                    k[j ^ d] = up ? hi : lo;
+                }
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(64) n2t_place_kernel(const Buffers* __restrict__ bs, uint32_t tile) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
@@ -479,7 +509,7 @@ __global__ void __launch_bounds__(64) n2t_place_kernel(const Buffers* __restrict
     const uint32_t N = f.N;
     const uint32_t lane = threadIdx.x;
     extern __shared__ uint32_t cur[];
-    __shared__ uint4 round_nodes[64];  // entry (lane, slot) of the current round
+    __shared__ uint4 rank_of[64];  // rank of entry (lane, slot) among the round's entries of its node
     const uint32_t* H = b.n2t_hist + (size_t)blockIdx.x * N;
     for (uint32_t n0 = 0; n0 < N; n0 += tile) {
         const uint32_t span = min(tile, N - n0);
@@ -487,41 +517,65 @@ __global__ void __launch_bounds__(64) n2t_place_kernel(const Buffers* __restrict
         for (uint32_t k = lane; k < span; k += 64) cur[k] = b.n2t_offset[n0 + k] + H[n0 + k];
         for (uint32_t tb = t0; tb < t1; tb += 64) {
             const uint32_t t = tb + lane;
-            ushort4 k = make_ushort4(0, 0, 0, 0);
-            const int m = t < t1 ? term_nodes(b, f, c.off, t, k) : 0;
-            uint32_t n[4] = {k.x, k.y, k.z, k.w};
+            ushort4 kn = make_ushort4(0, 0, 0, 0);
+            const int m = t < t1 ? term_nodes(b, f, c.off, t, kn) : 0;
+            uint32_t n[4] = {kn.x, kn.y, kn.z, kn.w};
 #pragma unroll
             for (int q = 0; q < 4; ++q)
                 if (!(q < m && n[q] >= n0 && n[q] - n0 < span)) n[q] = 0xFFFFFFFFu;  // not placed here
-            round_nodes[lane] = make_uint4(n[0], n[1], n[2], n[3]);
-            wave_lds_sync();
-            // stable rank among the round's entries of the same node, ordered by
-            // (term, slot); the counters then advance by the round's entries
-            // (LDS adds: their order does not matter)
-            uint32_t before[4] = {0, 0, 0, 0};
-            for (uint32_t l = 0; l < lane; ++l) {
-                const uint4 o = round_nodes[l];
+            // the round's 256 entries sorted by (node, term, slot): key = local
+            // node << 8 | 4 lane + slot; entries not placed here sort last
+            uint32_t k[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) before[q] += (o.x == n[q]) + (o.y == n[q]) + (o.z == n[q]) + (o.w == n[q]);
+            for (int q = 0; q < 4; ++q) k[q] = n[q] == 0xFFFFFFFFu ? 0xFFFFFFFFu : ((n[q] - n0) << 8) | (4 * lane + q);
+            wave_sort256(k);
+            // runs of equal nodes: rank = sorted position - start of the run
+            uint32_t nd[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) nd[j] = k[j] >> 8;
+            const uint32_t prev = (uint32_t)__shfl_up((int)nd[3], 1, 64);
+            const uint32_t next = (uint32_t)__shfl_down((int)nd[0], 1, 64);
+            uint32_t start[4], run = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const bool head = j == 0 ? (lane == 0 || prev != nd[0]) : nd[j - 1] != nd[j];
+                if (head) run = 4 * lane + j;
+                start[j] = run;  // start of the run within this lane (0 if it began earlier)
             }
+            // inclusive max-scan of the lanes' last run starts, then exclusive
+            uint32_t scan = run;
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t v = (uint32_t)__shfl_up((int)scan, o, 64);
+                if ((int)lane >= o) scan = max(scan, v);
+            }
+            uint32_t before = (uint32_t)__shfl_up((int)scan, 1, 64);
+            if (lane == 0) before = 0;
+            uint32_t rank[4];
 #pragma unroll
-                for (int qq = 0; qq < 4; ++qq) before[q] += (qq < q && n[qq] == n[q]) ? 1u : 0u;
-            uint32_t base[4];
+            for (int j = 0; j < 4; ++j) rank[j] = 4 * lane + j - max(start[j], before);
+            uint32_t* rk = reinterpret_cast<uint32_t*>(rank_of);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (k[j] != 0xFFFFFFFFu) rk[k[j] & 255] = rank[j];
+            wave_lds_sync();
+            const uint4 r4 = rank_of[lane];
+            const uint32_t mine[4] = {r4.x, r4.y, r4.z, r4.w};
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                base[q] = 0;
                 if (n[q] != 0xFFFFFFFFu) {
-                    base[q] = cur[n[q] - n0];
-                    b.n2t_index[base[q] + before[q]] = t;
-                    b.n2t_node[base[q] + before[q]] = (uint16_t)n[q];
+                    const uint32_t at = cur[n[q] - n0] + mine[q];
+                    b.n2t_index[at] = t;
+                    b.n2t_node[at] = (uint16_t)n[q];
                 }
             }
             wave_lds_sync();  // every lane has read cur before the updates
+            // the last entry of each run advances its node's counter
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (n[q] != 0xFFFFFFFFu) atomicAdd(&cur[n[q] - n0], 1u);
+            for (int j = 0; j < 4; ++j) {
+                const bool tail = j < 3 ? nd[j + 1] != nd[j] : (lane == 63 || next != nd[3]);
+                if (k[j] != 0xFFFFFFFFu && tail) cur[nd[j]] += rank[j] + 1;
+            }
             wave_lds_sync();
         }
     }
