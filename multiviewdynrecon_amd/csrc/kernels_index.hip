@@ -622,12 +622,16 @@ constexpr uint32_t kRowHash = kRowPartners;  // partner set capacity per row
 
 // Partners of node i in term t with the reference's pair multiplicity: every
 // knn slot q with knn[q] != i, or the other end of a non-loop graph edge.
-__device__ inline ushort4 term_partners(const Buffers& b, const FrameDev& f, const uint32_t off[5], uint32_t t, uint32_t i) {
+__device__ inline ushort4 partners_of(ushort4 k, int m, uint32_t i) {
     constexpr uint16_t kNone = 0xFFFF;
-    ushort4 k;
-    const int m = term_nodes(b, f, off, t, k);
     if (m == 2) return make_ushort4(k.x != k.y ? (k.x == i ? k.y : k.x) : kNone, kNone, kNone, kNone);
     return make_ushort4(k.x != i ? k.x : kNone, k.y != i ? k.y : kNone, k.z != i ? k.z : kNone, k.w != i ? k.w : kNone);
+}
+
+__device__ inline ushort4 term_partners(const Buffers& b, const FrameDev& f, const uint32_t off[5], uint32_t t, uint32_t i) {
+    ushort4 k;
+    const int m = term_nodes(b, f, off, t, k);
+    return partners_of(k, m, i);
 }
 
 __device__ inline uint32_t lower_bound_lds(const uint32_t* a, uint32_t n, uint32_t v) {
@@ -680,11 +684,31 @@ __global__ void __launch_bounds__(64 * kWaves) pair_rows_kernel(const Buffers* _
     const uint32_t beg = b.n2t_offset[i], end = b.n2t_offset[i + 1];
     uint32_t upper_entries = 0;
     bool overflow = false;
-    for (uint32_t p = beg + tid; p < end; p += 64 * kWaves) {
-        const ushort4 o = term_partners(b, f, c.off, b.n2t_index[p], i);
+    // four entries per lane in flight: their indices, then their records; a
+    // row of at most one batch keeps its entries' term nodes for the column pass
+    constexpr uint32_t kBatch = 4;
+    const bool one_batch = end - beg <= kBatch * 64 * kWaves;
+    ushort4 kq[kBatch];
+    int mq[kBatch];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint32_t v = slot(o, q);
+    for (int u = 0; u < (int)kBatch; ++u) {
+        kq[u] = make_ushort4(0, 0, 0, 0);
+        mq[u] = 0;
+    }
+    for (uint32_t p0 = beg + tid; p0 < end; p0 += kBatch * 64 * kWaves) {
+#pragma unroll
+        for (int u = 0; u < (int)kBatch; ++u) {
+            const uint32_t p = p0 + u * 64 * kWaves;
+            kq[u] = make_ushort4(0, 0, 0, 0);
+            mq[u] = p < end ? term_nodes(b, f, c.off, b.n2t_index[p], kq[u]) : 0;
+        }
+        ushort4 oq[kBatch];
+#pragma unroll
+        for (int u = 0; u < (int)kBatch; ++u)
+            oq[u] = mq[u] ? partners_of(kq[u], mq[u], i) : make_ushort4(0xFFFF, 0xFFFF, 0xFFFF, 0xFFFF);
+#pragma unroll
+        for (int uq = 0; uq < (int)(4 * kBatch); ++uq) {
+            const uint32_t v = slot(oq[uq / 4], uq % 4);
             if (v == 0xFFFF) continue;
             upper_entries += v > i ? 1u : 0u;
             uint32_t h = (v * 2654435761u) >> 23;  // 9 bits
@@ -718,12 +742,12 @@ __global__ void __launch_bounds__(64 * kWaves) pair_rows_kernel(const Buffers* _
 #pragma unroll
         for (int k = 0; k < (int)(kRowHash / 64); ++k) mine[k] = set[lane + 64 * k];
         uint32_t up = 0, low = 0;
-        int64_t last = -1;
+        uint32_t next_min = 0;  // the last extracted partner + 1
         for (;;) {
             uint32_t m = 0xFFFFFFFFu;
 #pragma unroll
             for (int k = 0; k < (int)(kRowHash / 64); ++k)
-                if ((int64_t)mine[k] > last && mine[k] < m) m = mine[k];
+                if (mine[k] >= next_min && mine[k] < m) m = mine[k];
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) {
                 const uint32_t u = (uint32_t)__shfl_xor((int)m, o, 64);
@@ -732,7 +756,7 @@ __global__ void __launch_bounds__(64 * kWaves) pair_rows_kernel(const Buffers* _
             if (m == 0xFFFFFFFFu) break;
             if (m > i) ul[up++] = m;  // same value from every lane
             else ll[low++] = m;
-            last = m;
+            next_min = m + 1;
         }
         if (lane == 0) {
             nup = up;
@@ -753,20 +777,31 @@ __global__ void __launch_bounds__(64 * kWaves) pair_rows_kernel(const Buffers* _
         uint16_t* rp = b.row_part + (size_t)i * kRowHash;
         for (uint32_t u = tid; u < up; u += 64 * kWaves) rp[u] = (uint16_t)ul[u];
         for (uint32_t k = tid; k < low; k += 64 * kWaves) rp[up + k] = (uint16_t)ll[k];
-        for (uint32_t p = beg + tid; p < end; p += 64 * kWaves) {
-            ushort4 k;
-            const int m = term_nodes(b, f, c.off, b.n2t_index[p], k);
-            uint32_t packed = 0xFFFFFFFFu;
-            if (m == 4 && up + 2 < 0xFF) {
-                packed = 0;
+        for (uint32_t p0 = beg + tid; p0 < end; p0 += kBatch * 64 * kWaves) {
+            if (!one_batch) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const uint32_t v = slot(k, q);
-                    const uint32_t col = (v != 0xFFFF && v > i) ? 2 + lower_bound_lds(ul, up, v) : 0xFFu;
-                    packed |= col << (8 * q);
+                for (int u = 0; u < (int)kBatch; ++u) {
+                    const uint32_t p = p0 + u * 64 * kWaves;
+                    kq[u] = make_ushort4(0, 0, 0, 0);
+                    mq[u] = p < end ? term_nodes(b, f, c.off, b.n2t_index[p], kq[u]) : 0;
                 }
             }
-            b.n2t_col[p] = packed;
+#pragma unroll
+            for (int u = 0; u < (int)kBatch; ++u) {
+                const uint32_t p = p0 + u * 64 * kWaves;
+                if (p >= end) continue;
+                uint32_t packed = 0xFFFFFFFFu;
+                if (mq[u] == 4 && up + 2 < 0xFF) {
+                    packed = 0;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const uint32_t v = slot(kq[u], q);
+                        const uint32_t col = (v != 0xFFFF && v > i) ? 2 + lower_bound_lds(ul, up, v) : 0xFFu;
+                        packed |= col << (8 * q);
+                    }
+                }
+                b.n2t_col[p] = packed;
+            }
         }
         return;
     }
