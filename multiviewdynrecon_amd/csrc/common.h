@@ -295,6 +295,7 @@ struct Buffers {
     uint32_t cap_pixels, cap_feat, cap_cross, cap_edges, cap_kv1, cap_kv2, cap_pairs, cap_nodes;
     // term storage
     ushort4* dense_px;       // x, y, view, 0
+    ushort4* dense_knn;      // knn of each dense term (a packed copy of the records' first field)
     ScalarRec* dense;
     ScalarRec* density;
     ushort4* fg_px;

@@ -175,6 +175,7 @@ __global__ void __launch_bounds__(kTileThreads) compact_scatter_kernel(const Buf
                 rec.pad1[0] = rec.pad1[1] = 0.f;
                 if (which == 0) {
                     b.dense_px[dst] = px;
+                    b.dense_knn[dst] = rec.knn;
                     b.dense[dst] = rec;
                     if (b.density) b.density[dst] = rec;
                 } else {
@@ -360,7 +361,7 @@ __device__ inline void term_offsets(const Counts& c, uint32_t E, uint32_t off[5]
 // Node keys of term t (buildTermKeyValueKernel, Node2TermsIndex.cu:8-110):
 // four knn nodes, or the two ends of a graph edge.  Returns the key count.
 __device__ inline int term_nodes(const Buffers& b, const FrameDev& f, const uint32_t off[5], uint32_t t, ushort4& k) {
-    if (t < off[0]) k = b.dense[t].knn;
+    if (t < off[0]) k = b.dense_knn[t];  // 8 B per term: a sector serves four neighbouring terms
     else if (t < off[1]) {
         const ushort2 e = f.graph[t - off[0]];
         k = make_ushort4(e.x, e.y, 0, 0);

@@ -355,6 +355,7 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     A(s->frame_dev, 1);
     A(b.counts, 1);
     A(b.dense_px, b.cap_pixels);
+    A(b.dense_knn, b.cap_pixels);
     A(b.dense, b.cap_pixels);
     b.density = nullptr;
     if (c.density_in_jtj) A(b.density, b.cap_pixels);
