@@ -30,18 +30,23 @@ __device__ inline PixelCoord pixel_of(const FrameDev& f, uint32_t p) {
     return c;
 }
 
-// Dense-term predicate: the index map holds a surfel (ImageTermKNNFetcher.cu:19-31).
-struct DensePred {
-    __device__ bool operator()(const FrameDev& f, const Buffers& b, PixelCoord c) const {
-        const uint32_t s = f.index[c.v][c.y * f.W + c.x];
-        if (s == kInvalidIndex) return false;
-        if (s >= f.num_surfels) {
-            atomicOr(&b.counts->error, kErrNodes);
-            return false;
-        }
-        return true;
-    }
-};
+// View and in-view offset of flat pixel p (compares, no division).
+__device__ inline uint32_t view_of(const FrameDev& f, uint32_t p, uint32_t wh) {
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 1; k < kMaxViews; ++k) v += ((uint32_t)k < f.V && p >= (uint32_t)k * wh) ? 1u : 0u;
+    return v;
+}
+
+// The index map of view v: a select among the frame's view pointers (no
+// per-lane load of the pointer table).
+__device__ inline const uint32_t* index_map(const FrameDev& f, uint32_t v) {
+    const uint32_t* m = f.index[0];
+#pragma unroll
+    for (int k = 1; k < kMaxViews; ++k)
+        if ((uint32_t)k == v) m = f.index[k];
+    return m;
+}
 
 __device__ inline bool knn_ok(const FrameDev& f, ushort4 k) { return k.x < f.N && k.y < f.N && k.z < f.N && k.w < f.N; }
 
@@ -182,6 +187,116 @@ __global__ void __launch_bounds__(kTileThreads) compact_scatter_kernel(const Buf
                     b.fg_px[dst] = px;
                     b.fg[dst] = rec;
                 }
+            }
+        }
+        running += round_total;
+    }
+}
+
+// Dense terms: the pixels whose index map holds a surfel
+// (ImageTermKNNFetcher.cu:19-31); the count and scatter passes above,
+// specialised -- a tile's 16 index-map reads per thread are issued
+// together, and the scatter takes every round's positions from one exchange
+// of per-wave counts (one workgroup barrier per tile).
+__device__ inline void dense_tile_surfels(const FrameDev& f, uint32_t base, uint32_t sv[kTileRounds]) {
+    const uint32_t wh = f.W * f.H, M = f.V * wh;
+#pragma unroll
+    for (int r = 0; r < kTileRounds; ++r) {
+        const uint32_t p = base + r * kTileThreads + threadIdx.x;
+        sv[r] = kInvalidIndex;
+        if (p < M) {
+            const uint32_t v = view_of(f, p, wh);
+            sv[r] = index_map(f, v)[p - v * wh];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kTileThreads) dense_count_kernel(const Buffers* __restrict__ bs) {
+    const Buffers b = frame_buffers(bs);
+    const FrameDev& f = *b.frame;
+    const uint32_t M = f.V * f.W * f.H;
+    // clears the counters the rest of the index build accumulates into
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= b.cap_nodes; i += stride) {
+        b.n2t_count[i] = 0;
+        b.row_up[i] = 0;
+        b.row_low[i] = 0;
+        b.row_kv2[i] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Counts* c = b.counts;
+        c->dense = c->smooth = c->fg = c->feat = c->cross = 0;
+        c->kv1 = c->kv2 = c->pairs = c->max_seg = c->max_row = 0;
+        c->error = 0;
+    }
+    const uint32_t base = blockIdx.x * kTilePixels;
+    if (base >= M) return;
+    uint32_t sv[kTileRounds];
+    dense_tile_surfels(f, base, sv);
+    uint32_t count = 0;
+    bool bad = false;
+#pragma unroll
+    for (int r = 0; r < kTileRounds; ++r) {
+        count += (sv[r] != kInvalidIndex && sv[r] < f.num_surfels) ? 1u : 0u;
+        bad |= sv[r] != kInvalidIndex && sv[r] >= f.num_surfels;
+    }
+    if (bad) atomicOr(&b.counts->error, kErrNodes);
+    count = block_sum_u32<kTileThreads>(count);
+    if (threadIdx.x == 0) b.tile_count[blockIdx.x] = count;
+}
+
+__global__ void __launch_bounds__(kTileThreads) dense_scatter_kernel(const Buffers* __restrict__ bs) {
+    const Buffers b = frame_buffers(bs);
+    const FrameDev& f = *b.frame;
+    const uint32_t wh = f.W * f.H, M = f.V * wh;
+    const uint32_t base = blockIdx.x * kTilePixels;
+    if (base >= M) return;
+    constexpr int kWaves = kTileThreads / 64;
+    __shared__ uint32_t wave_cnt[kTileRounds][kWaves];
+    const uint32_t wave = threadIdx.x >> 6;
+    uint32_t sv[kTileRounds];
+    dense_tile_surfels(f, base, sv);
+    uint64_t mask[kTileRounds];
+#pragma unroll
+    for (int r = 0; r < kTileRounds; ++r) {
+        mask[r] = __ballot(sv[r] != kInvalidIndex && sv[r] < f.num_surfels);
+        if ((threadIdx.x & 63) == 0) wave_cnt[r][wave] = (uint32_t)__popcll(mask[r]);
+    }
+    __syncthreads();
+    uint32_t running = b.tile_offset[blockIdx.x];
+    const uint32_t cap = b.cap_pixels;
+#pragma unroll
+    for (int r = 0; r < kTileRounds; ++r) {
+        uint32_t before = 0, round_total = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t cw = wave_cnt[r][w];
+            before += (w < (int)wave) ? cw : 0u;
+            round_total += cw;
+        }
+        const uint32_t s = sv[r];
+        if (s != kInvalidIndex && s < f.num_surfels) {
+            const uint32_t dst = running + before + lane_prefix(mask[r]);
+            if (dst < cap) {
+                const uint32_t p = base + r * kTileThreads + threadIdx.x;
+                const uint32_t v = view_of(f, p, wh), o = p - v * wh;
+                const uint32_t y = o / f.W, x = o - y * f.W;
+                ScalarRec rec;
+                rec.knn = f.surfel_knn[s];
+                if (!knn_ok(f, rec.knn)) {
+                    atomicOr(&b.counts->error, kErrNodes);
+                    rec.knn = make_ushort4(0, 0, 0, 0);
+                }
+                rec.w = f.surfel_w[s];
+                rec.r = 0.f;
+                rec.pad0 = 0;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) rec.J[k] = 0.f;
+                rec.pad1[0] = rec.pad1[1] = 0.f;
+                b.dense_px[dst] = make_ushort4((uint16_t)x, (uint16_t)y, (uint16_t)v, 0);
+                b.dense_knn[dst] = rec.knn;
+                b.dense[dst] = rec;
+                if (b.density) b.density[dst] = rec;
             }
         }
         running += round_total;
@@ -977,11 +1092,9 @@ __global__ void __launch_bounds__(256) pair_link_kernel(const Buffers* __restric
 void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, int mode, hipStream_t stream) {
     const bool gram = mode == kIndexGram;
     const uint32_t F = B.frames;
-    hipLaunchKernelGGL(compact_count_kernel<DensePred>, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream, B.dev,
-                       DensePred{}, 1);
+    hipLaunchKernelGGL(dense_count_kernel, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream, B.dev);
     hipLaunchKernelGGL(compact_scan_kernel, dim3(1, F), dim3(1024), 0, stream, B.dev, 0);
-    hipLaunchKernelGGL(compact_scatter_kernel<DensePred>, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream, B.dev,
-                       DensePred{}, 0);
+    hipLaunchKernelGGL(dense_scatter_kernel, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream, B.dev);
     if (use_foreground) {
         hipLaunchKernelGGL(compact_count_kernel<ForegroundPred>, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream,
                            B.dev, ForegroundPred{}, 0);
