@@ -284,6 +284,8 @@ struct P2PRec {         // feature, cross view
     float4 warped;
 };
 static_assert(sizeof(ScalarRec) == 64, "ScalarRec layout");
+static_assert(offsetof(ScalarRec, r) == 8 && offsetof(ScalarRec, w) == 16 && offsetof(ScalarRec, J) == 32,
+              "ScalarRec layout");
 static_assert(sizeof(SmoothRec) == 32, "SmoothRec layout");
 static_assert(sizeof(P2PRec) == 64, "P2PRec layout");
 

@@ -281,22 +281,28 @@ __global__ void __launch_bounds__(kTileThreads) dense_scatter_kernel(const Buffe
                 const uint32_t p = base + r * kTileThreads + threadIdx.x;
                 const uint32_t v = view_of(f, p, wh), o = p - v * wh;
                 const uint32_t y = o / f.W, x = o - y * f.W;
-                ScalarRec rec;
-                rec.knn = f.surfel_knn[s];
-                if (!knn_ok(f, rec.knn)) {
+                ushort4 knn = f.surfel_knn[s];
+                if (!knn_ok(f, knn)) {
                     atomicOr(&b.counts->error, kErrNodes);
-                    rec.knn = make_ushort4(0, 0, 0, 0);
+                    knn = make_ushort4(0, 0, 0, 0);
                 }
-                rec.w = f.surfel_w[s];
-                rec.r = 0.f;
-                rec.pad0 = 0;
-#pragma unroll
-                for (int k = 0; k < 6; ++k) rec.J[k] = 0.f;
-                rec.pad1[0] = rec.pad1[1] = 0.f;
+                const float4 w = f.surfel_w[s];
                 b.dense_px[dst] = make_ushort4((uint16_t)x, (uint16_t)y, (uint16_t)v, 0);
-                b.dense_knn[dst] = rec.knn;
-                b.dense[dst] = rec;
-                if (b.density) b.density[dst] = rec;
+                b.dense_knn[dst] = knn;
+                // the record's first half (knn, r = 0, weights); J is written by
+                // the term evaluation before any use
+                const uint4 h0 = make_uint4((uint32_t)knn.x | ((uint32_t)knn.y << 16),
+                                            (uint32_t)knn.z | ((uint32_t)knn.w << 16), 0u, 0u);
+                const uint4 h1 = make_uint4(__float_as_uint(w.x), __float_as_uint(w.y), __float_as_uint(w.z),
+                                            __float_as_uint(w.w));
+                uint4* d = reinterpret_cast<uint4*>(b.dense + dst);
+                d[0] = h0;
+                d[1] = h1;
+                if (b.density) {
+                    d = reinterpret_cast<uint4*>(b.density + dst);
+                    d[0] = h0;
+                    d[1] = h1;
+                }
             }
         }
         running += round_total;
