@@ -506,7 +506,7 @@ __device__ inline uint32_t slot(ushort4 k, int q) { return q == 0 ? k.x : (q == 
 //   n2t_place   each chunk writes its entries at offset[n] + H[c][n] + rank,
 //               rank = position among the chunk's earlier entries of node n
 // No global atomics and no sorts: the result is deterministic by construction.
-constexpr uint32_t kChunkTerms = 512;
+constexpr uint32_t kChunkTerms = 1024;
 constexpr uint32_t kHistNodes = 16384;  // largest LDS node tile (64 KB of u32 counters)
 
 // LDS node tile of the counting-sort kernels (dynamic shared memory).
