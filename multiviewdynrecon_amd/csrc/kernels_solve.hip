@@ -158,25 +158,39 @@ __global__ void __launch_bounds__(256) eval_terms_kernel(const Buffers* __restri
 // -------------------------------------------------- 6x6 SPD inverse (Cholesky)
 // inv(A) = L^-T L^-1 with A = L L^T; A is symmetric positive definite (the
 // reference adds 1e-2 to every diagonal, PreconditionerBlocked.cu:272-275).
+// In FP64 the six pivots are inverted once and the other divisions become
+// products; the FP32 instance divides as the reference does.
 template <typename T>
 __device__ inline void spd_inverse6(const float* A, float* out, double* out64, double* linv64 = nullptr) {
+    constexpr bool kRecip = sizeof(T) == 8;
     T L[21];  // packed lower triangle, row-major
+    T rd[6];  // 1 / L(r, r) (FP64)
 #define LI(r, c) L[(r) * ((r) + 1) / 2 + (c)]
+#pragma unroll
     for (int r = 0; r < 6; ++r) {
+#pragma unroll
         for (int col = 0; col <= r; ++col) {
             T s = (T)A[6 * r + col];
+#pragma unroll
             for (int k = 0; k < col; ++k) s -= LI(r, k) * LI(col, k);
-            if (r == col) LI(r, r) = s > (T)0 ? sqrt(s) : (T)1e-30;
-            else LI(r, col) = s / LI(col, col);
+            if (r == col) {
+                LI(r, r) = s > (T)0 ? sqrt(s) : (T)1e-30;
+                if (kRecip) rd[r] = (T)1 / LI(r, r);
+            } else {
+                LI(r, col) = kRecip ? s * rd[col] : s / LI(col, col);
+            }
         }
     }
     // X = L^-1 (lower), in place of a dense 6x6
     T X[36];
+#pragma unroll
     for (int c2 = 0; c2 < 6; ++c2)
+#pragma unroll
         for (int r = 0; r < 6; ++r) {
             T s = (r == c2) ? (T)1 : (T)0;
+#pragma unroll
             for (int k = 0; k < r; ++k) s -= LI(r, k) * X[6 * k + c2];
-            X[6 * r + c2] = s / LI(r, r);
+            X[6 * r + c2] = kRecip ? s * rd[r] : s / LI(r, r);
         }
 #undef LI
     if (linv64)
@@ -830,7 +844,9 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
 // (blockDiagonalInverseKernel, BlockDiagonalPreconditionerInverse.cu:6-41).
 // The pair blocks are finished by the transform that starts the CG
 // (cg_transform_kernel), which needs both nodes' factors.
-__global__ void __launch_bounds__(256) gram_fixup_kernel(const Buffers* __restrict__ bs) {
+// inverse64: also the FP64 inverse (the matrix-free solve's preconditioner);
+// the fused CG needs only the factors.
+__global__ void __launch_bounds__(256) gram_fixup_kernel(const Buffers* __restrict__ bs, int inverse64) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
@@ -854,7 +870,8 @@ __global__ void __launch_bounds__(256) gram_fixup_kernel(const Buffers* __restri
         add_node_frame_damping(D, f.nodes[n]);
 #pragma unroll
         for (int q = 0; q < 36; ++q) b.diag[36 * (size_t)n + q] = D[q];
-        spd_inverse6<double>(D, b.diag_inv + 36 * (size_t)n, b.diag_inv64 + 36 * (size_t)n, b.linv64 + kTri * (size_t)n);
+        spd_inverse6<double>(D, b.diag_inv + 36 * (size_t)n, inverse64 ? b.diag_inv64 + 36 * (size_t)n : nullptr,
+                             b.linv64 + kTri * (size_t)n);
     }
 }
 
@@ -916,7 +933,8 @@ void launch_assemble(const Batch& B, const Penalty& pen, bool gram, bool diag_on
     if (gram) {
         const uint32_t rows = (B.nodes + kXcds - 1) / kXcds * kXcds;
         hipLaunchKernelGGL(gram_rows_kernel, dim3(rows, B.frames), dim3(64), 0, stream, B.dev, pen, diag_only ? 1 : 0);
-        hipLaunchKernelGGL(gram_fixup_kernel, dim3((B.nodes + 63) / 64, B.frames), dim3(64), 0, stream, B.dev);
+        hipLaunchKernelGGL(gram_fixup_kernel, dim3((B.nodes + 63) / 64, B.frames), dim3(64), 0, stream, B.dev,
+                           diag_only ? 1 : 0);
     } else {
         hipLaunchKernelGGL(assemble_chunks_kernel, dim3(2048, B.frames), dim3(256), 0, stream, B.dev, pen);
         hipLaunchKernelGGL(assemble_fixup_kernel<float>, dim3(fix_blocks, B.frames), dim3(256), 0, stream, B.dev);
