@@ -18,7 +18,8 @@
 // Pinning (what checks each part, tests/test_oracle_*.py):
 //   dual-quaternion / SE3 math ...... closed forms, scipy rotations, round trips
 //   term selection .................. brute-force Python restatement on tiny maps
-//   term Jacobians .................. central finite differences (FP64)
+//   term Jacobians .................. central finite differences (FP64); image terms
+//                                     on a linear image with bilinear_images set
 //   Node2Term / NodePair2Term ....... brute-force closed form (sorted term lists)
 //   JtJ / JtR blocks ................ dense J^T J and J^T r from explicit J
 //   6x6 inverse ..................... A * inv(A) = I
@@ -360,6 +361,11 @@ public:
     std::vector<uint16_t> graph;   // 2 per edge
     uint32_t iter = 0;
     Penalty pen{};
+    // Checker switch (never the reference's behaviour): image terms sample
+    // their map bilinearly at the unrounded projection, so that on a linear
+    // image the residual is a smooth function of the warp and its Jacobian
+    // can be pinned by finite differences.  Off by default.
+    bool bilinear_images = false;
 
     std::vector<ScalarTerm<R>> dense, density, fg;
     std::vector<SmoothTerm<R>> smooth;
@@ -738,7 +744,15 @@ public:
         const mvdr_intrinsic K = f.in.intrinsic[v];
         const int u = project_round(pc.x, pc.z, K.fx, K.cx), q = project_round(pc.y, pc.z, K.fy, K.cy);
         if (!(u >= 0 && u < (int)f.W && q >= 0 && q < (int)f.H)) return;
-        const R I0 = f.fetch1(map[v], u, q);
+        R I0 = f.fetch1(map[v], u, q);
+        if (bilinear_images) {
+            const double U = pc.x / (pc.z + 1e-10) * K.fx + K.cx, Q = pc.y / (pc.z + 1e-10) * K.fy + K.cy;
+            const int x0 = (int)std::floor(U), y0 = (int)std::floor(Q);
+            if (!(f.inside(x0, y0) && f.inside(x0 + 1, y0 + 1))) return;
+            const double ax = U - x0, ay = Q - y0;
+            I0 = R((1 - ay) * ((1 - ax) * f.fetch1(map[v], x0, y0) + ax * f.fetch1(map[v], x0 + 1, y0)) +
+                   ay * ((1 - ax) * f.fetch1(map[v], x0, y0 + 1) + ax * f.fetch1(map[v], x0 + 1, y0 + 1)));
+        }
         float g[2];
         f.fetch2(grad[v], u, q, g);
         V3<R> d;
@@ -1208,6 +1222,11 @@ int set_array(Oracle<R>& o, const std::string& name, const double* src, size_t c
     if (name == "twist") {
         if (count != 6 * (size_t)o.N) return -1;
         o.twist.assign(src, src + count);
+        return 0;
+    }
+    if (name == "bilinear_images") {
+        if (count != 1) return -1;
+        o.bilinear_images = src[0] != 0.0;
         return 0;
     }
     return -1;

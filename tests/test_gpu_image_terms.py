@@ -7,9 +7,9 @@ iterations, PenaltyConstants.cpp:9-51).
 Per GN iteration: the foreground / density term records against the oracle's
 FP64 records from the same warp field, the J^T J blocks and J^T r assembled
 from the GPU's records (<= 1e-5 per block), and the twist (<= 1e-4 vs FP64).
-The index (Node2Term with the foreground terms) stays bit-exact.  Parity of
-these Jacobians is pinned here against the oracle only (no FD through the
-nearest-pixel lookups).
+The index (Node2Term with the foreground terms) stays bit-exact.  The
+oracle's image-term Jacobians themselves are pinned by finite differences on a
+linear image (tests/test_oracle_terms.py).
 """
 import numpy as np
 import pytest

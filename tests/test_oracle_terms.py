@@ -188,3 +188,98 @@ def test_dense_residual_matches_independent_restatement(scene0):
     ri = np.where(ok, (dn * (P[inside] - dv)).sum(1), 0.0)
     r[inside] = ri
     np.testing.assert_allclose(d[:, 14], r, atol=1e-9)
+
+
+# Image terms (density_map_jacobian.cuh:12-79): the residual reads the map at
+# the rounded projection, a step function of the warp, so finite differences
+# see zero.  On a linear image I(u, q) = A u + B q + C with a constant gradient
+# map (A, B), the oracle's checker switch `bilinear_images` samples the map at
+# the unrounded projection instead: the residual is then exactly
+# A U(xi) + B Q(xi) + C - g, whose derivative is the reference's chain (focal
+# scaling, 1/z, the initial-camera and world-to-camera rotations, the twist's
+# cross product).  A, B, C are powers of two, so every sample of the image is
+# exact in float32 and bilinear sampling reproduces the plane.  fx != fy, so a
+# swapped focal length fails too.
+LIN_A, LIN_B, LIN_C = 2.0 ** -7, -(2.0 ** -8), 8.0
+
+
+@pytest.fixture(scope="module")
+def linear_image_oracle(scene1):
+    import copy
+
+    sc = copy.deepcopy(scene1)
+    q, u = np.mgrid[0:sc.height, 0:sc.width]
+    img = (LIN_A * u + LIN_B * q + LIN_C).astype(np.float32)
+    assert np.array_equal(img.astype(np.float64), LIN_A * u + LIN_B * q + LIN_C)
+    grad = np.empty((sc.height, sc.width, 2), np.float32)
+    grad[..., 0], grad[..., 1] = LIN_A, LIN_B
+    for v in sc.views:
+        v.intrinsic = v.intrinsic.copy()
+        v.intrinsic[1] = 540.0
+        for name in ("gray", "filtered_foreground"):
+            v.maps[name] = img.copy()
+        for name in ("gray_gradient", "foreground_gradient"):
+            v.maps[name] = grad.copy()
+    cfg = workload_config(1, use_foreground=1, density_in_jtj=1)
+    o = Oracle(cfg, HostFrame(sc), "fp64")
+    o.scene = sc  # the frame points into these maps
+    o.build_index()
+    o.set_iteration(0)
+    return o
+
+
+def image_terms_at(o, name, xi, bilinear):
+    o.set("bilinear_images", [1.0 if bilinear else 0.0])
+    try:
+        o.set("node_se3", dq_rows(*twist_se3(xi), o.scene.num_nodes))
+        o.eval_terms()
+        return o.get(name)
+    finally:
+        o.set("bilinear_images", [0.0])
+
+
+@pytest.mark.parametrize("name", ["density_terms", "foreground_terms"])
+def test_image_jacobian_matches_finite_differences(linear_image_oracle, name):
+    o = linear_image_oracle
+    base = image_terms_at(o, name, np.zeros(6), True)
+    valid = np.abs(base[:, 8:14]).sum(1) > 0
+    assert valid.sum() > 1000, valid.sum()
+    h = 1e-5
+    for k in range(6):
+        xi = np.zeros(6)
+        xi[k] = h
+        rp = image_terms_at(o, name, xi, True)
+        rm = image_terms_at(o, name, -xi, True)
+        both = valid & (np.abs(rp[:, 8:14]).sum(1) > 0) & (np.abs(rm[:, 8:14]).sum(1) > 0)
+        assert both.sum() > 0.99 * valid.sum()
+        fd = (rp[both, 14] - rm[both, 14]) / (2 * h)
+        np.testing.assert_allclose(fd, base[both, 8 + k], rtol=1e-6, atol=1e-7, err_msg=f"{name} twist {k}")
+
+
+@pytest.mark.parametrize("name", ["density_terms", "foreground_terms"])
+def test_image_residual_reads_the_rounded_projection(linear_image_oracle, name):
+    """The reference lookup (nearest pixel) against the smooth one: within half a
+    pixel step of the plane, on the plane's 2^-8 lattice once the rendered
+    density g = 0.299 R + 0.587 G + 0.114 B (ColorTypeTransfer.h:51-56) is
+    added back, and with the same Jacobian (the gradient map is constant)."""
+    o = linear_image_oracle
+    xi = np.array([0.01, -0.02, 0.015, 0.01, 0.005, -0.01])
+    near = image_terms_at(o, name, xi, False)
+    smooth = image_terms_at(o, name, xi, True)
+    both = (np.abs(near[:, 8:14]).sum(1) > 0) & (np.abs(smooth[:, 8:14]).sum(1) > 0)
+    assert both.sum() > 1000
+    np.testing.assert_array_equal(near[both, 8:14], smooth[both, 8:14])
+    diff = np.abs(near[both, 14] - smooth[both, 14])
+    assert diff.max() <= (abs(LIN_A) + abs(LIN_B)) / 2 + 1e-9
+    assert np.median(diff) > 1e-4  # the lookup does round
+    g = np.zeros(len(near))
+    if name == "density_terms":
+        code = near[:, 15].astype(np.int64)
+        px, py, view = code & 0xFFFF, (code >> 16) & 0xFFFF, code >> 32
+        for v in range(len(o.scene.views)):
+            sel = view == v
+            rgb = o.scene.views[v].maps["normalized_rgb"][py[sel], px[sel], :3].astype(np.float64)
+            coef = np.array([0.299, 0.587, 0.114], np.float32).astype(np.float64)
+            g[sel] = rgb @ coef
+    lattice = (near[both, 14] + g[both]) * 256.0
+    np.testing.assert_allclose(lattice, np.rint(lattice), atol=1e-6)
