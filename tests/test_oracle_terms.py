@@ -97,15 +97,19 @@ def test_smooth_jacobian_matches_finite_differences(o0, scene0):
                 np.testing.assert_allclose(fd, J[:, k], atol=1e-6)
 
 
-def test_feature_warp_jacobian_matches_finite_differences(scene1):
+@pytest.mark.parametrize("name", ["feature_terms", "cross_terms"])
+def test_feature_warp_jacobian_matches_finite_differences(scene1, name):
+    """Feature and cross-view terms share the point-to-point warp
+    (forwardWarpFeatureVertexKernel, SparseCorrespondenceHandler.cu:193-220)."""
     cfg = workload_config(1)
+    assert cfg.use_cross_view
     o = Oracle(cfg, HostFrame(scene1), "fp64")
     o.build_index()
     N = scene1.num_nodes
     o.set("node_se3", dq_rows(np.eye(3), np.zeros(3), N))
     o.eval_terms()
-    base = o.get("feature_terms")
-    assert len(base) > 200
+    base = o.get(name)
+    assert len(base) > 200, len(base)
     h = 1e-5
     w = base[:, 14:17]
     for k in range(6):
@@ -115,7 +119,7 @@ def test_feature_warp_jacobian_matches_finite_differences(scene1):
         for sign in (1, -1):
             o.set("node_se3", dq_rows(*twist_se3(sign * xi), N))
             o.eval_terms()
-            out.append(o.get("feature_terms")[:, 14:17])
+            out.append(o.get(name)[:, 14:17])
         fd = (out[0] - out[1]) / (2 * h)
         # d warped / d xi = [-[w]x | I]: column k
         expect = np.zeros_like(w)
