@@ -41,6 +41,28 @@ __device__ inline uint32_t xcd_remap(uint32_t v, uint32_t grid, uint32_t used) {
 // Number of virtual items a grid-stride loop over `used` items visits.
 __device__ inline uint32_t xcd_span(uint32_t grid, uint32_t used) { return ((used + grid - 1) / grid) * grid; }
 
+// Frame-to-XCD affinity for a grid (blocks, frames) whose frame count is a
+// multiple of the XCD count: every workgroup of a frame runs on one XCD
+// (workgroup id mod 8), so the frame's blocks and vectors share that L2.
+// Within a frame the blocks keep their dispatch order.
+struct WorkItem {
+    uint32_t frame, block;
+};
+__device__ inline WorkItem frame_affine() {
+    const uint32_t F = gridDim.y, blocks = gridDim.x;
+    if (F % kXcds != 0) return WorkItem{blockIdx.y, blockIdx.x};
+    const uint32_t L = blockIdx.x + blocks * blockIdx.y;
+    const uint32_t x = L % kXcds, r = L / kXcds;
+    return WorkItem{x + kXcds * (r / blocks), r % blocks};
+}
+__device__ inline Buffers frame_buffers_at(const Buffers* bs, uint32_t frame) {
+#if __HIP_DEVICE_COMPILE__
+    return *((const __attribute__((address_space(4))) Buffers*)bs + frame);
+#else
+    return bs[frame];
+#endif
+}
+
 template <typename T>
 __device__ inline T wave_sum(T v) {
 #pragma unroll

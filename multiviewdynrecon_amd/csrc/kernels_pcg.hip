@@ -347,14 +347,17 @@ __device__ inline void pair_block(const Buffers& b, const Counts& c, const Frame
     }
 }
 
+// One XCD per frame (frame_affine): the frame's FP64 factors, read by every
+// pair of their nodes, stay in that XCD's L2.
 __global__ void __launch_bounds__(256) cg_transform_kernel(const Buffers* __restrict__ bs, Penalty pen) {
-    const Buffers b = frame_buffers(bs);
+    const WorkItem wi = frame_affine();
+    const Buffers b = frame_buffers_at(bs, wi.frame);
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
     const uint32_t N = f.N, P = c.pairs;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < P + N; x += stride) {
+    for (uint32_t x = wi.block * blockDim.x + threadIdx.x; x < P + N; x += stride) {
         if (x < P) {
             const uint32_t p = x;
             const uint32_t key = b.pair_key[p];

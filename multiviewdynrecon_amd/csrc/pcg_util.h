@@ -20,27 +20,6 @@ constexpr uint32_t kMaxIters = 64;
 
 __device__ inline double* slot_ptr(const Buffers& b, uint32_t slot) { return b.partial + (size_t)slot * kPartialStride; }
 
-// Frame-to-XCD affinity for a grid (blocks, frames) whose frame count is a
-// multiple of the XCD count: every workgroup of a frame runs on one XCD
-// (workgroup id mod 8), so the frame's blocks and vectors share that L2.
-struct WorkItem {
-    uint32_t frame, block;
-};
-__device__ inline WorkItem frame_affine() {
-    const uint32_t F = gridDim.y, blocks = gridDim.x;
-    if (F % kXcds != 0) return WorkItem{blockIdx.y, blockIdx.x};
-    const uint32_t L = blockIdx.x + blocks * blockIdx.y;
-    const uint32_t x = L % kXcds, r = L / kXcds;
-    return WorkItem{x + kXcds * (r / blocks), r % blocks};
-}
-__device__ inline Buffers frame_buffers_at(const Buffers* bs, uint32_t frame) {
-#if __HIP_DEVICE_COMPILE__
-    return *((const __attribute__((address_space(4))) Buffers*)bs + frame);
-#else
-    return bs[frame];
-#endif
-}
-
 // Sums of the partials of two slots (n0 and n1 of them), computed identically
 // by every block (fixed order: strided per-thread sums, a wave butterfly, the
 // waves in order).  One workgroup barrier; at most one call per kernel (the
