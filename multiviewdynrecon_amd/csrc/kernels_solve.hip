@@ -722,9 +722,17 @@ __device__ inline void stage_entry(const Buffers& b, const TermBounds& c, const 
 // diag_only: the diagonal blocks and J^T r alone (the matrix-free solve).
 __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict__ bs, Penalty pen, int diag_only) {
     const Buffers b = frame_buffers(bs);
+    // The grid covers the batch's largest node count in one sweep, so this
+    // workgroup's row is known before its frame's sizes arrive: the row's
+    // offsets load alongside them (clamped to the capacity; a row past the
+    // frame's node count exits below).
+    const uint32_t i = xcd_remap(blockIdx.x, gridDim.x, gridDim.x);
+    const uint32_t is = min(i, b.cap_nodes - 1);
+    const uint32_t beg = b.n2t_offset[is], end = b.n2t_offset[is + 1];
+    const uint32_t pbeg = diag_only ? 0 : b.pair_offset[is], pend = diag_only ? 0 : b.pair_offset[is + 1];
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
-    if (c.error) return;
+    if (c.error || i >= f.N) return;
     __shared__ float Glo[64 * kRowWords];
     __shared__ float Ghi[64 * kRowWords];
     __shared__ float Wl[64 * kRowWords];
@@ -743,11 +751,8 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
     const Lambda2 l2(pen);
     const bool density = b.density && pen.density != 0.f;
     const TermBounds tb(c);
-    for (uint32_t v = blockIdx.x; v < xcd_span(gridDim.x, f.N); v += gridDim.x) {
-        const uint32_t i = xcd_remap(v, gridDim.x, f.N);
-        if (i == 0xFFFFFFFFu) continue;
-        const uint32_t beg = b.n2t_offset[i], end = b.n2t_offset[i + 1];
-        const uint32_t ubeg = diag_only ? 0 : b.pair_offset[i], U = diag_only ? 0 : b.pair_offset[i + 1] - ubeg;
+    {
+        const uint32_t ubeg = pbeg, U = pend - pbeg;
         const float4 g = f.nodes[i];
         const uint32_t cols = 2 + U;
         // partner columns held by n2t_col while they fit its u8 columns
