@@ -429,13 +429,14 @@ __global__ void __launch_bounds__(256) pair_blocks_kernel(const Buffers* __restr
 }
 
 // y = sum over the off-diagonal blocks of row i of A^ v_j, with v_j = V(j)
-// supplied by the caller; split over the row's kRowLanes lanes and reduced.
+// supplied by the caller; split over the row's kHatLanes lanes and reduced.
 template <typename Load, typename Acc = double>
 __device__ inline void hat_row_apply(const Buffers& b, uint32_t i, uint32_t sub, const Load& load, Acc y[6]) {
 #pragma unroll
     for (int a = 0; a < 6; ++a) y[a] = 0;
     const uint32_t beg = b.row_ptr[i], n = b.row_ptr[i + 1] - beg;
-    for (uint32_t e = sub; e < n; e += kRowLanes) {
+#pragma unroll 2  // two slots of a lane in flight (a row has ~12 slots over 8 lanes)
+    for (uint32_t e = sub; e < n; e += kHatLanes) {
         const uint2 s = b.row_ent[beg + e];
         float v[6];
         load(s.x, v);
@@ -444,7 +445,7 @@ __device__ inline void hat_row_apply(const Buffers& b, uint32_t i, uint32_t sub,
 #pragma unroll
     for (int a = 0; a < 6; ++a)
 #pragma unroll
-        for (int o = kRowLanes / 2; o > 0; o >>= 1) y[a] += __shfl_xor(y[a], o, 64);
+        for (int o = kHatLanes / 2; o > 0; o >>= 1) y[a] += __shfl_xor(y[a], o, 64);
 }
 
 // w_0 = A^ r_0 (r_0 = b^), x^ = p_{-1} = s_{-1} = 0, partials of gamma_0, delta_0.
@@ -452,7 +453,7 @@ __global__ void __launch_bounds__(256) cg_init_kernel(const Buffers* __restrict_
     const WorkItem wi = frame_affine();
     const Buffers b = frame_buffers_at(bs, wi.frame);
     const FrameDev& f = *b.frame;
-    const uint32_t sub = threadIdx.x % kRowLanes;
+    const uint32_t sub = threadIdx.x % kHatLanes;
     double g = 0.0, d = 0.0;
     if (!b.counts->error) {
         float* RWS = b.cg_rws[0];
@@ -461,9 +462,9 @@ __global__ void __launch_bounds__(256) cg_init_kernel(const Buffers* __restrict_
             const float2 r45 = *reinterpret_cast<const float2*>(RWS + kRws * (size_t)j + 4);
             v[0] = r03.x; v[1] = r03.y; v[2] = r03.z; v[3] = r03.w; v[4] = r45.x; v[5] = r45.y;
         };
-        const uint32_t groups = gridDim.x * kRowsPerBlock;
-        for (uint32_t base = wi.block * kRowsPerBlock; base < f.N; base += groups) {
-            const uint32_t i = base + threadIdx.x / kRowLanes;
+        const uint32_t groups = gridDim.x * kHatRowsPerBlock;
+        for (uint32_t base = wi.block * kHatRowsPerBlock; base < f.N; base += groups) {
+            const uint32_t i = base + threadIdx.x / kHatLanes;
             double y[6];
             hat_row_apply(b, i < f.N ? i : 0, sub, load, y);
             if (i < f.N && sub == 0) {
@@ -516,7 +517,7 @@ __global__ void __launch_bounds__(256) cg_iter_kernel(const Buffers* __restrict_
     const WorkItem wi = frame_affine();
     const Buffers b = frame_buffers_at(bs, wi.frame);
     const FrameDev& f = *b.frame;
-    const uint32_t sub = threadIdx.x % kRowLanes;
+    const uint32_t sub = threadIdx.x % kHatLanes;
     double alpha, beta;
     cg_scalars(b, i, gridDim.x, alpha, beta, wi.block);
     double g = 0.0, d = 0.0;
@@ -543,9 +544,9 @@ __global__ void __launch_bounds__(256) cg_iter_kernel(const Buffers* __restrict_
                 v[a] = __builtin_fmaf(-af, s_i, r[a]);
             }
         };
-        const uint32_t groups = gridDim.x * kRowsPerBlock;
-        for (uint32_t base = wi.block * kRowsPerBlock; base < f.N; base += groups) {
-            const uint32_t n = base + threadIdx.x / kRowLanes;
+        const uint32_t groups = gridDim.x * kHatRowsPerBlock;
+        for (uint32_t base = wi.block * kHatRowsPerBlock; base < f.N; base += groups) {
+            const uint32_t n = base + threadIdx.x / kHatLanes;
             // component a = sub of the row's own vectors (lanes 0..5), loaded
             // before the row apply so that their latency overlaps it
             const bool own = n < f.N && sub < 6;
@@ -587,7 +588,7 @@ __global__ void __launch_bounds__(256) cg_refine_residual_kernel(const Buffers* 
     const WorkItem wi = frame_affine();
     const Buffers b = frame_buffers_at(bs, wi.frame);
     const FrameDev& f = *b.frame;
-    const uint32_t sub = threadIdx.x % kRowLanes;
+    const uint32_t sub = threadIdx.x % kHatLanes;
     double acc = 0.0;
     if (!b.counts->error) {
         const float* X = b.cg_x;
@@ -596,9 +597,9 @@ __global__ void __launch_bounds__(256) cg_refine_residual_kernel(const Buffers* 
             const float2 p0 = p[0], p1 = p[1], p2 = p[2];
             v[0] = p0.x; v[1] = p0.y; v[2] = p1.x; v[3] = p1.y; v[4] = p2.x; v[5] = p2.y;
         };
-        const uint32_t groups = gridDim.x * kRowsPerBlock;
-        for (uint32_t base = wi.block * kRowsPerBlock; base < f.N; base += groups) {
-            const uint32_t n = base + threadIdx.x / kRowLanes;
+        const uint32_t groups = gridDim.x * kHatRowsPerBlock;
+        for (uint32_t base = wi.block * kHatRowsPerBlock; base < f.N; base += groups) {
+            const uint32_t n = base + threadIdx.x / kHatLanes;
             double y[6];
             hat_row_apply(b, n < f.N ? n : 0, sub, load, y);
             if (n < f.N && sub == 0) {
@@ -620,7 +621,7 @@ __global__ void __launch_bounds__(256) cg_refine_curvature_kernel(const Buffers*
     const WorkItem wi = frame_affine();
     const Buffers b = frame_buffers_at(bs, wi.frame);
     const FrameDev& f = *b.frame;
-    const uint32_t sub = threadIdx.x % kRowLanes;
+    const uint32_t sub = threadIdx.x % kHatLanes;
     double acc = 0.0;
     if (!b.counts->error) {
         const float* Q = b.q;
@@ -629,9 +630,9 @@ __global__ void __launch_bounds__(256) cg_refine_curvature_kernel(const Buffers*
             const float2 p0 = p[0], p1 = p[1], p2 = p[2];
             v[0] = p0.x; v[1] = p0.y; v[2] = p1.x; v[3] = p1.y; v[4] = p2.x; v[5] = p2.y;
         };
-        const uint32_t groups = gridDim.x * kRowsPerBlock;
-        for (uint32_t base = wi.block * kRowsPerBlock; base < f.N; base += groups) {
-            const uint32_t n = base + threadIdx.x / kRowLanes;
+        const uint32_t groups = gridDim.x * kHatRowsPerBlock;
+        for (uint32_t base = wi.block * kHatRowsPerBlock; base < f.N; base += groups) {
+            const uint32_t n = base + threadIdx.x / kHatLanes;
             double y[6];
             hat_row_apply(b, n < f.N ? n : 0, sub, load, y);
             if (n < f.N && sub == 0) {
@@ -838,10 +839,14 @@ void launch_apply_update(const Batch& B, uint32_t k, hipStream_t stream) {
 
 // Fused CG of the mixed precision mode: transform, init, one launch per
 // iteration, refinement, warp update of GN iteration k.
-// Workgroups per frame of the fused CG's row kernels: at most 128, each
-// looping over its rows, so a launch of 32 frames is ~4k workgroups and every
-// workgroup reduces 128 partials at its start.
-static uint32_t cg_grid(uint32_t nodes) { return std::min<uint32_t>(spmv_grid(nodes), 128u); }
+// Workgroups per frame of the fused CG's row kernels: at most 64, each
+// looping over its rows (two passes of 32 rows at ~4k nodes), so a launch of
+// 32 frames is ~2k workgroups and every workgroup reduces 64 partials at its
+// start.
+static uint32_t cg_grid(uint32_t nodes) {
+    const uint32_t g = (nodes + kHatRowsPerBlock - 1) / kHatRowsPerBlock;
+    return std::max<uint32_t>(kXcds, std::min<uint32_t>((g + kXcds - 1) / kXcds * kXcds, 64u));
+}
 
 void launch_cg_solve(const Batch& B, uint32_t iters, uint32_t refine, uint32_t k, const Penalty& pen,
                      hipStream_t stream) {

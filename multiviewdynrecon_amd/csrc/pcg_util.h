@@ -9,6 +9,11 @@ namespace mvdr {
 
 constexpr int kRowLanes = 16;         // lanes per block row in the SpMV
 constexpr int kRowsPerBlock = 256 / kRowLanes;
+// lanes per block row in the fused CG's row kernels (a row has ~12 slots)
+constexpr int kHatLanes = 8;
+constexpr int kHatRowsPerBlock = 256 / kHatLanes;
+static_assert(kHatLanes >= 6 && (kHatLanes & (kHatLanes - 1)) == 0,
+              "lanes 0..5 of a row own its six components; the row sum is a butterfly");
 constexpr uint32_t kVecBlocks = 64;   // grid of the per-node vector kernels
 
 // partial-sum slots (kPartialStride doubles each)
