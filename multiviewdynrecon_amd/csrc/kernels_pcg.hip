@@ -465,22 +465,23 @@ __global__ void __launch_bounds__(256) cg_init_kernel(const Buffers* __restrict_
         const uint32_t groups = gridDim.x * kHatRowsPerBlock;
         for (uint32_t base = wi.block * kHatRowsPerBlock; base < f.N; base += groups) {
             const uint32_t i = base + threadIdx.x / kHatLanes;
+            // component a = sub of the row (lanes 0..5), its r_0 loaded before
+            // the row apply
+            const bool own = i < f.N && sub < 6;
+            float* rec = RWS + kRws * (size_t)(i < f.N ? i : 0);
+            const double r = own ? rec[sub] : 0.0;
             double y[6];
             hat_row_apply(b, i < f.N ? i : 0, sub, load, y);
-            if (i < f.N && sub == 0) {
-#pragma unroll
-                for (int a = 0; a < 6; ++a) {
-                    const size_t k = 6 * (size_t)i + a;
-                    float* rec = RWS + kRws * (size_t)i;
-                    const double r = rec[a];
-                    const double w = (double)(float)(y[a] + r);
-                    rec[6 + a] = (float)w;  // w_0
-                    rec[12 + a] = 0.f;      // s_-1
-                    b.cg_p[k] = 0.f;
-                    b.cg_x[k] = 0.f;
-                    g += r * r;
-                    d += w * r;
-                }
+            if (own) {
+                const double ya = sub == 0 ? y[0] : sub == 1 ? y[1] : sub == 2 ? y[2] : sub == 3 ? y[3] : sub == 4 ? y[4] : y[5];
+                const size_t k = 6 * (size_t)i + sub;
+                const double w = (double)(float)(ya + r);
+                rec[6 + sub] = (float)w;  // w_0
+                rec[12 + sub] = 0.f;      // s_-1
+                b.cg_p[k] = 0.f;
+                b.cg_x[k] = 0.f;
+                g += r * r;
+                d += w * r;
             }
         }
     }
@@ -600,16 +601,18 @@ __global__ void __launch_bounds__(256) cg_refine_residual_kernel(const Buffers* 
         const uint32_t groups = gridDim.x * kHatRowsPerBlock;
         for (uint32_t base = wi.block * kHatRowsPerBlock; base < f.N; base += groups) {
             const uint32_t n = base + threadIdx.x / kHatLanes;
+            // component a = sub of the row (lanes 0..5), its inputs loaded
+            // before the row apply
+            const bool own = n < f.N && sub < 6;
+            const size_t k = 6 * (size_t)(n < f.N ? n : 0) + (sub < 6 ? sub : 0);
+            const double bk = own ? (double)b.cg_b[k] : 0.0, xk = own ? (double)X[k] : 0.0;
             double y[6];
             hat_row_apply(b, n < f.N ? n : 0, sub, load, y);
-            if (n < f.N && sub == 0) {
-#pragma unroll
-                for (int a = 0; a < 6; ++a) {
-                    const size_t k = 6 * (size_t)n + a;
-                    const float rr = (float)((double)b.cg_b[k] - (y[a] + (double)X[k]));
-                    b.q[k] = rr;
-                    acc += (double)rr * (double)rr;
-                }
+            if (own) {
+                const double ya = sub == 0 ? y[0] : sub == 1 ? y[1] : sub == 2 ? y[2] : sub == 3 ? y[3] : sub == 4 ? y[4] : y[5];
+                const float rr = (float)(bk - (ya + xk));
+                b.q[k] = rr;
+                acc += (double)rr * (double)rr;
             }
         }
     }
@@ -633,14 +636,13 @@ __global__ void __launch_bounds__(256) cg_refine_curvature_kernel(const Buffers*
         const uint32_t groups = gridDim.x * kHatRowsPerBlock;
         for (uint32_t base = wi.block * kHatRowsPerBlock; base < f.N; base += groups) {
             const uint32_t n = base + threadIdx.x / kHatLanes;
+            const bool own = n < f.N && sub < 6;
+            const double q = own ? (double)Q[6 * (size_t)n + sub] : 0.0;
             double y[6];
             hat_row_apply(b, n < f.N ? n : 0, sub, load, y);
-            if (n < f.N && sub == 0) {
-#pragma unroll
-                for (int a = 0; a < 6; ++a) {
-                    const double q = Q[6 * (size_t)n + a];
-                    acc += q * (y[a] + q);
-                }
+            if (own) {
+                const double ya = sub == 0 ? y[0] : sub == 1 ? y[1] : sub == 2 ? y[2] : sub == 3 ? y[3] : sub == 4 ? y[4] : y[5];
+                acc += q * (ya + q);
             }
         }
     }
@@ -842,7 +844,8 @@ void launch_apply_update(const Batch& B, uint32_t k, hipStream_t stream) {
 // Workgroups per frame of the fused CG's row kernels: at most 64, each
 // looping over its rows (two passes of 32 rows at ~4k nodes), so a launch of
 // 32 frames is ~2k workgroups and every workgroup reduces 64 partials at its
-// start.
+// start.  The count depends on the node count alone: a frame's partial sums,
+// and so its results, do not depend on the batch it is solved in.
 static uint32_t cg_grid(uint32_t nodes) {
     const uint32_t g = (nodes + kHatRowsPerBlock - 1) / kHatRowsPerBlock;
     return std::max<uint32_t>(kXcds, std::min<uint32_t>((g + kXcds - 1) / kXcds * kXcds, 64u));
