@@ -196,12 +196,13 @@ __device__ inline void spd_inverse6(const float* A, float* out, double* out64, d
     if (linv64)
         for (int a = 0; a < 6; ++a)
             for (int k = 0; k <= a; ++k) linv64[tri_index(a, k)] = (double)X[6 * a + k];
+    if (!out && !out64) return;
     // inv = X^T X
     for (int r = 0; r < 6; ++r)
         for (int col = 0; col < 6; ++col) {
             T s = 0;
             for (int k = 0; k < 6; ++k) s += X[6 * k + r] * X[6 * k + col];
-            out[6 * r + col] = (float)s;
+            if (out) out[6 * r + col] = (float)s;
             if (out64) out64[6 * r + col] = (double)s;
         }
 }
@@ -850,8 +851,11 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
 // The pair blocks are finished by the transform that starts the CG
 // (cg_transform_kernel), which needs both nodes' factors.
 // inverse64: also the FP64 inverse (the matrix-free solve's preconditioner);
-// the fused CG needs only the factors.
-__global__ void __launch_bounds__(256) gram_fixup_kernel(const Buffers* __restrict__ bs, int inverse64) {
+// the fused CG needs only the factors.  full: also the damped blocks (diag,
+// in place of the Gram sums) and their FP32 inverses (diag_inv); without it
+// the Gram sums stay in diag, and a later call with full forms both (inspection
+// and the materialized SpMV stage).
+__global__ void __launch_bounds__(256) gram_fixup_kernel(const Buffers* __restrict__ bs, int inverse64, int full) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
@@ -873,10 +877,12 @@ __global__ void __launch_bounds__(256) gram_fixup_kernel(const Buffers* __restri
                 ++o;
             }
         add_node_frame_damping(D, f.nodes[n]);
+        if (full) {
 #pragma unroll
-        for (int q = 0; q < 36; ++q) b.diag[36 * (size_t)n + q] = D[q];
-        spd_inverse6<double>(D, b.diag_inv + 36 * (size_t)n, inverse64 ? b.diag_inv64 + 36 * (size_t)n : nullptr,
-                             b.linv64 + kTri * (size_t)n);
+            for (int q = 0; q < 36; ++q) b.diag[36 * (size_t)n + q] = D[q];
+        }
+        spd_inverse6<double>(D, full ? b.diag_inv + 36 * (size_t)n : nullptr,
+                             inverse64 ? b.diag_inv64 + 36 * (size_t)n : nullptr, b.linv64 + kTri * (size_t)n);
     }
 }
 
@@ -939,11 +945,17 @@ void launch_assemble(const Batch& B, const Penalty& pen, bool gram, bool diag_on
         const uint32_t rows = (B.nodes + kXcds - 1) / kXcds * kXcds;
         hipLaunchKernelGGL(gram_rows_kernel, dim3(rows, B.frames), dim3(64), 0, stream, B.dev, pen, diag_only ? 1 : 0);
         hipLaunchKernelGGL(gram_fixup_kernel, dim3((B.nodes + 63) / 64, B.frames), dim3(64), 0, stream, B.dev,
-                           diag_only ? 1 : 0);
+                           diag_only ? 1 : 0, diag_only ? 1 : 0);
     } else {
         hipLaunchKernelGGL(assemble_chunks_kernel, dim3(2048, B.frames), dim3(256), 0, stream, B.dev, pen);
         hipLaunchKernelGGL(assemble_fixup_kernel<float>, dim3(fix_blocks, B.frames), dim3(256), 0, stream, B.dev);
     }
+}
+
+// The damped diagonal blocks and their FP32 inverses after a fused-CG
+// assembly (which leaves the Gram sums in diag): once per assembly.
+void launch_diag_blocks(const Batch& B, hipStream_t stream) {
+    hipLaunchKernelGGL(gram_fixup_kernel, dim3((B.nodes + 63) / 64, B.frames), dim3(64), 0, stream, B.dev, 0, 1);
 }
 
 // Returns the number of partials written to slot kSlotCost (per frame).

@@ -17,6 +17,7 @@ void launch_pair_term_lists(const Batch& B, hipStream_t stream);
 uint32_t index_chunks(uint64_t max_terms);
 void launch_eval_terms(const Batch& B, uint32_t which, bool fp64_geometry, hipStream_t stream);
 void launch_pair_blocks(const Batch& B, const Penalty& pen, hipStream_t stream);
+void launch_diag_blocks(const Batch& B, hipStream_t stream);
 void launch_assemble(const Batch& B, const Penalty& pen, bool gram, bool diag_only, hipStream_t stream);
 void launch_pcg_matrix_free(const Batch& B, uint32_t iters, uint32_t refine, bool fp64, const Penalty& pen,
                             hipStream_t stream);
@@ -47,7 +48,9 @@ struct mvdr_solver {
     bool has_input = false;
     bool indexed = false;
     bool pair_terms = false;            // NodePair2Term lists of the current index present
-    bool pair_blocks = true;            // offdiag holds B_ij (mixed mode: S_ij until formed on demand)
+    // offdiag holds B_ij, diag / diag_inv the damped blocks and their inverses
+    // (mixed mode: S_ij and the Gram sums until formed on demand)
+    bool pair_blocks = true;
     uint32_t graph_nodes = 0;           // Batch::nodes the captured graph was sized for
     uint32_t iter = 0;                  // GN iterations done on the current frame
     hipEvent_t done = nullptr;          // end of the last enqueued work
@@ -173,7 +176,8 @@ void enqueue_batch(const Batch& B, const mvdr_solver_config& c, hipStream_t stre
 // Grid sizing node count: the frame's node count rounded up to 64.
 uint32_t live_nodes(uint32_t n) { return std::max<uint32_t>(64, (n + 63) / 64 * 64); }
 
-// After an assembly: the mixed mode's fused CG leaves S_ij in offdiag.
+// After an assembly: the mixed mode's fused CG leaves S_ij in offdiag and the
+// Gram sums in diag (no FP32 inverses).
 bool blocks_after_assembly(const mvdr_solver_config& c) {
     return !(c.precision == MVDR_PRECISION_MIXED && c.jtj_mode != MVDR_JTJ_MATRIX_FREE);
 }
@@ -669,9 +673,11 @@ mvdr_status mvdr_solver_debug_get(mvdr_solver* s, int tap, void* dst, size_t byt
         HIP_TRY(s, hipStreamSynchronize(nullptr));
         s->pair_terms = true;
     }
-    if (tap == MVDR_TAP_JTJ_OFFDIAG && !s->pair_blocks && s->iter > 0) {
+    const bool block_tap = tap == MVDR_TAP_JTJ_OFFDIAG || tap == MVDR_TAP_JTJ_DIAG || tap == MVDR_TAP_JTJ_DIAG_INV;
+    if (block_tap && !s->pair_blocks && s->iter > 0) {
         s->self.nodes = live_nodes(s->frame_host.N);
         launch_pair_blocks(s->self, kernel_penalty(s->cfg, s->iter - 1), nullptr);
+        launch_diag_blocks(s->self, nullptr);
         HIP_TRY(s, hipGetLastError());
         HIP_TRY(s, hipStreamSynchronize(nullptr));
         s->pair_blocks = true;
@@ -751,7 +757,8 @@ static mvdr_status bench_stage_impl(const Batch& B, const mvdr_solver_config& c,
     }
     const Penalty pen = kernel_penalty(c, k);
     if (stage == MVDR_STAGE_SPMV && !mf && !pair_blocks) {
-        launch_pair_blocks(B, pen, st);  // the materialized SpMV reads B_ij
+        launch_pair_blocks(B, pen, st);  // the materialized SpMV reads B_ij and the damped D_i
+        launch_diag_blocks(B, st);
         pair_blocks = true;
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
