@@ -70,6 +70,9 @@ def algorithmic_bytes(sz, mixed, matrix_free=False):
     # term records: knn ids + weights + J + r (dense / foreground), edge ids + valid
     # + warped point + residual (smooth), knn + weights + target + warped (p2p)
     r_terms = 52 * (Td + Tfg) + 29 * E + 56 * (Tf + Tc)
+    # preconditioner out: FP32 block inverses (plain), the FP64 Cholesky
+    # factors L^-1 the fused CG scales by (mixed), FP32 + FP64 inverses
+    # (matrix-free, mixed)
     inv = 144 * N + (288 * N if mixed else 0)
     if matrix_free:
         return {
@@ -80,8 +83,9 @@ def algorithmic_bytes(sz, mixed, matrix_free=False):
         }
     return {
         # Node2Term lists + offsets, pair keys + offsets, node positions in;
-        # diagonal blocks, inverses, J^T r and off-diagonal blocks out
-        "assemble": r_terms + 4 * kv1 + 8 * (N + 1) + 4 * P + 16 * N + 144 * N + inv + 24 * N + 144 * P,
+        # diagonal blocks, preconditioner, J^T r and off-diagonal blocks out
+        "assemble": r_terms + 4 * kv1 + 8 * (N + 1) + 4 * P + 16 * N + 144 * N + (168 * N if mixed else 144 * N)
+        + 24 * N + 144 * P,
         # blocks + block-row structure + the direction vector in, A d out
         "spmv": 144 * (N + P) + 4 * (N + P) + 8 * (N + 1) + 8 * P + 24 * N + 24 * N,
         # fused CG iteration: off-diagonal blocks of L^-1 A L^-T and the block-row
