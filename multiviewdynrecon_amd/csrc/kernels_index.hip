@@ -620,7 +620,9 @@ __device__ inline void wave_sort256(uint32_t k[4]) {
     }
 }
 
-__global__ void __launch_bounds__(64) n2t_place_kernel(const Buffers* __restrict__ bs, uint32_t tile) {
+// with_nodes: also each entry's node (n2t_node), which only the segmented
+// assembly of the plain path reads.
+__global__ void __launch_bounds__(64) n2t_place_kernel(const Buffers* __restrict__ bs, uint32_t tile, int with_nodes) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
@@ -688,7 +690,7 @@ __global__ void __launch_bounds__(64) n2t_place_kernel(const Buffers* __restrict
                 if (n[q] != 0xFFFFFFFFu) {
                     const uint32_t at = cur[n[q] - n0] + mine[q];
                     b.n2t_index[at] = t;
-                    b.n2t_node[at] = (uint16_t)n[q];
+                    if (with_nodes) b.n2t_node[at] = (uint16_t)n[q];
                 }
             }
             wave_lds_sync();  // every lane has read cur before the updates
@@ -1113,7 +1115,8 @@ void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, int
     hipLaunchKernelGGL(n2t_hist_kernel, dim3(B.cap_chunks, F), dim3(64), tile * sizeof(uint32_t), stream, B.dev, tile);
     hipLaunchKernelGGL(n2t_colscan_kernel, dim3((B.nodes + 63) / 64, F), dim3(64 * kScanWaves), 0, stream, B.dev);
     hipLaunchKernelGGL(node_scan_kernel, dim3(1, F), dim3(1024), 0, stream, B.dev, 0);
-    hipLaunchKernelGGL(n2t_place_kernel, dim3(B.cap_chunks, F), dim3(64), tile * sizeof(uint32_t), stream, B.dev, tile);
+    hipLaunchKernelGGL(n2t_place_kernel, dim3(B.cap_chunks, F), dim3(64), tile * sizeof(uint32_t), stream, B.dev, tile,
+                       mode == kIndexLists ? 1 : 0);
     if (mode == kIndexNodes) return;
     const uint32_t row_grid = (B.nodes + kXcds - 1) / kXcds * kXcds;
     if (gram) {
