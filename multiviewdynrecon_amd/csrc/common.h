@@ -307,6 +307,11 @@ struct Buffers {
     float4* feat_canon;
     P2PRec* cross;
     float4* cross_canon;
+    // selection staging, one slot per feature then cross-view candidate
+    // (cap_feat + cap_cross): flag, record, canonical vertex
+    uint32_t* sel_ok;
+    P2PRec* sel_rec;
+    float4* sel_canon;
     // compaction scratch
     uint32_t* tile_count;
     uint32_t* tile_offset;

@@ -419,8 +419,28 @@ __device__ inline bool choose_cross(const FrameDev& f, uint32_t k, P2PRec& rec, 
     return true;
 }
 
-// Block 0 compacts feature pairs, block 1 cross-view pairs, each in candidate
-// order with a running block-wide prefix.
+// Selection, step 1: every candidate (feature pairs, then cross-view pairs)
+// evaluated by its own thread into the staging slot of its candidate index.
+// The candidate counts were checked against cap_feat / cap_cross at set-up.
+__global__ void __launch_bounds__(256) select_eval_kernel(const Buffers* __restrict__ bs, int use_cross) {
+    const Buffers b = frame_buffers(bs);
+    const FrameDev& f = *b.frame;
+    const uint32_t nf = f.feat_base[f.V], total = nf + (use_cross ? f.num_cross : 0u);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+        P2PRec rec;
+        float4 canon;
+        const bool ok = g < nf ? choose_feature(f, g, rec, canon) : choose_cross(f, g - nf, rec, canon);
+        b.sel_ok[g] = ok ? 1u : 0u;
+        if (ok) {
+            b.sel_rec[g] = rec;
+            b.sel_canon[g] = canon;
+        }
+    }
+}
+
+// Step 2: block 0 compacts feature pairs, block 1 cross-view pairs, each in
+// candidate order with a running block-wide prefix.
 __global__ void __launch_bounds__(1024) select_pairs_kernel(const Buffers* __restrict__ bs, int use_cross) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
@@ -430,6 +450,7 @@ __global__ void __launch_bounds__(1024) select_pairs_kernel(const Buffers* __res
         return;
     }
     const uint32_t total = is_cross ? f.num_cross : f.feat_base[f.V];
+    const uint32_t first = is_cross ? f.feat_base[f.V] : 0u;  // staging slot of candidate 0
     const uint32_t cap = is_cross ? b.cap_cross : b.cap_feat;
     P2PRec* out = is_cross ? b.cross : b.feat;
     float4* canon_out = is_cross ? b.cross_canon : b.feat_canon;
@@ -439,10 +460,7 @@ __global__ void __launch_bounds__(1024) select_pairs_kernel(const Buffers* __res
     int parity = 0;
     for (uint32_t base = 0; base < total; base += 1024, parity ^= 1) {
         const uint32_t g = base + threadIdx.x;
-        P2PRec rec;
-        float4 canon;
-        bool ok = false;
-        if (g < total) ok = is_cross ? choose_cross(f, g, rec, canon) : choose_feature(f, g, rec, canon);
+        const bool ok = g < total && b.sel_ok[first + g] != 0u;
         const uint64_t mask = __ballot(ok);
         if ((threadIdx.x & 63) == 0) wave_cnt[parity][wave] = (uint32_t)__popcll(mask);
         __syncthreads();
@@ -455,8 +473,8 @@ __global__ void __launch_bounds__(1024) select_pairs_kernel(const Buffers* __res
         if (ok) {
             const uint32_t dst = running + before + lane_prefix(mask);
             if (dst < cap) {
-                out[dst] = rec;
-                canon_out[dst] = canon;
+                out[dst] = b.sel_rec[first + g];
+                canon_out[dst] = b.sel_canon[first + g];
             }
         }
         running += chunk;
@@ -1110,6 +1128,7 @@ void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, int
         hipLaunchKernelGGL(compact_scatter_kernel<ForegroundPred>, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream,
                            B.dev, ForegroundPred{}, 1);
     }
+    hipLaunchKernelGGL(select_eval_kernel, dim3(32, F), dim3(256), 0, stream, B.dev, use_cross ? 1 : 0);
     hipLaunchKernelGGL(select_pairs_kernel, dim3(2, F), dim3(1024), 0, stream, B.dev, use_cross ? 1 : 0);
     const uint32_t tile = hist_tile(B.nodes);
     hipLaunchKernelGGL(n2t_hist_kernel, dim3(B.cap_chunks, F), dim3(64), tile * sizeof(uint32_t), stream, B.dev, tile);

@@ -371,6 +371,9 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     A(b.feat_canon, b.cap_feat);
     A(b.cross, b.cap_cross);
     A(b.cross_canon, b.cap_cross);
+    A(b.sel_ok, (size_t)b.cap_feat + b.cap_cross);
+    A(b.sel_rec, (size_t)b.cap_feat + b.cap_cross);
+    A(b.sel_canon, (size_t)b.cap_feat + b.cap_cross);
     A(b.tile_count, s->cap_tiles);
     A(b.tile_offset, s->cap_tiles);
     A(b.feat_flag, 1);
