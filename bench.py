@@ -219,15 +219,14 @@ def bench_uneven(args):
 
     from multiviewdynrecon_amd.config import workload_config
     from multiviewdynrecon_amd.frame import DeviceFrame
-    from multiviewdynrecon_amd.scene import make_scene
+    from multiviewdynrecon_amd.scene import make_uneven_scenes, uneven_scene_params
     from multiviewdynrecon_amd.solver import NonRigidBatch, NonRigidSolver
 
-    rng = np.random.default_rng(0x4D564452 + 100)
     count = 16
-    nodes = rng.integers(1000, 8001, count)
-    views = rng.integers(1, 4, count)
-    scenes = [make_scene(4, seed=0x4D564452 + 100 + s, target_nodes=int(nodes[s]), target_surfels=int(37.5 * nodes[s]),
-                         num_views=int(views[s]), cross_pairs=2000 if views[s] > 1 else 0) for s in range(count)]
+    params = uneven_scene_params(count)
+    nodes = np.array([p["target_nodes"] for p in params])
+    views = np.array([p["num_views"] for p in params])
+    scenes = make_uneven_scenes(count)
     cfg = workload_config(4, use_graph=args.graph)
     frames = [DeviceFrame(sc) for sc in scenes]
     solvers = []

@@ -73,7 +73,8 @@ const char* mvdr_scene_last_error(void);
  *            filtered_foreground f32, foreground_gradient f2, gray f32,
  *            gray_gradient f2, edge_mask u8, reference_vertex f4,
  *            reference_normal f4, warp_vertex f4, index_map u32,
- *            normalized_rgb f4, feature_pairs u16x4                       */
+ *            normalized_rgb f4, feature_pairs u16x4, corr_map u16x4 (per
+ *            pixel: the feature pair observed there, else zero)            */
 long long mvdr_scene_get(const mvdr_scene* scene, const char* name, uint32_t view, void* dst, size_t bytes);
 
 #ifdef __cplusplus

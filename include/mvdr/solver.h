@@ -44,6 +44,10 @@ extern "C" {
 
 typedef struct mvdr_solver mvdr_solver;
 
+/* Largest pcg_iterations a solver accepts (one FP64 partial-sum slot per
+ * iteration of each inner product). */
+#define MVDR_MAX_PCG_ITERATIONS 64
+
 typedef enum {
     MVDR_OK = 0,
     MVDR_E_INVALID = 1,   /* null pointer / malformed argument              */
@@ -77,7 +81,8 @@ typedef struct {
     uint32_t max_cross_pairs;      /* candidate cross-view pairs                       */
     /* schedule (values of the absent Constants::*; base/Constants.h:30-62) */
     uint32_t gn_iterations;        /* kNumGaussNewtonIterations                        */
-    uint32_t pcg_iterations;       /* SolvePCGMaterialized(pcg_iterations)             */
+    uint32_t pcg_iterations;       /* SolvePCGMaterialized(pcg_iterations), at most
+                                      MVDR_MAX_PCG_ITERATIONS (MVDR_E_SHAPE above)     */
     uint32_t global_iterations;    /* kNumGlobalSolverItarations                       */
     uint32_t use_elastic_penalty;  /* kUseElasticPenalty                               */
     /* term switches (ConfigParser::Use*Term, base/ConfigParser.h:43-53) */
@@ -133,6 +138,17 @@ typedef struct {
     uint32_t num_feature_pairs[MVDR_MAX_VIEWS];
     const void* cross_pairs;                           /* mvdr_cross_pair crossCorrPairs */
     uint32_t num_cross_pairs;
+
+    /* CameraObservation::corrMap[v] (base/CameraObservation.h:55), passed to
+     * the cross-view handler (NonRigidSolver.cpp:422,
+     * CrossViewCorrespondenceHandler.h:30-34): per view a W*H ushort4 map whose
+     * entry at current-frame pixel (z, w) holds the feature pair (x, y, z, w)
+     * that ends there (PatchColliderRGBCorrespondence.cu:171); other entries
+     * are zero.  Nullable per view.  When set for view v1 of a cross-view pair,
+     * pixel 1 is back-traced through it: an entry whose (z, w) is pixel 1 moves
+     * the geometry search to its previous-frame pixel (x, y); without an entry
+     * (or a map) the search stays at pixel 1. */
+    const void* corr_map[MVDR_MAX_VIEWS];
 } mvdr_frame_input;
 
 /* Sizes the index build found (valid after build_index / solve). */
@@ -173,8 +189,9 @@ typedef enum {
     MVDR_TAP_PENALTY = 17,           /* f32[5] lambda: smooth, density, foreground, feature, cross */
     MVDR_TAP_DENSITY_TERMS = 18,     /* 16 x u32/f32 per dense term (density channel)     */
     MVDR_TAP_SPMV_INPUT = 19,        /* f32[6N] vector d of the last J^T J apply (reference twist coordinates) */
-    MVDR_TAP_SPMV_OUTPUT = 20        /* f32[6N] its product J^T J d (reference coordinates); a STAGE_SPMV bench
+    MVDR_TAP_SPMV_OUTPUT = 20,       /* f32[6N] its product J^T J d (reference coordinates); a STAGE_SPMV bench
                                         run leaves d = M^-1 b of the last GN iteration here */
+    MVDR_TAP_FOREGROUND_PIXELS = 21  /* u16[4] (x, y, view, 0) per foreground term (use_foreground) */
 } mvdr_tap;
 
 const char* mvdr_status_string(mvdr_status status);

@@ -55,6 +55,7 @@ class FrameInput(C.Structure):
         ("node_se3", vp), ("canonical_nodes", vp), ("node_graph", vp), ("num_nodes", u32), ("num_graph_edges", u32),
         ("feature_pairs", vp * V), ("num_feature_pairs", u32 * V),
         ("cross_pairs", vp), ("num_cross_pairs", u32),
+        ("corr_map", vp * V),
     ]
 
 
@@ -90,8 +91,10 @@ TAP = {
     "nodepair_term_index": 4, "dense_terms": 5, "smooth_terms": 6, "feature_terms": 7, "cross_terms": 8,
     "foreground_terms": 9, "dense_pixels": 10, "jtj_diag": 11, "jtj_diag_inv": 12, "jtr": 13,
     "jtj_offdiag": 14, "twist": 15, "node_se3": 16, "penalty": 17, "density_terms": 18,
-    "spmv_input": 19, "spmv_output": 20,
+    "spmv_input": 19, "spmv_output": 20, "foreground_pixels": 21,
 }
+
+MAX_PCG_ITERATIONS = 64  # MVDR_MAX_PCG_ITERATIONS
 
 STATUS = {0: "MVDR_OK", 1: "MVDR_E_INVALID", 2: "MVDR_E_SHAPE", 3: "MVDR_E_CAPACITY", 4: "MVDR_E_HIP",
           5: "MVDR_E_STATE"}

@@ -236,6 +236,7 @@ struct FrameDev {
     const ushort2* graph;
     const ushort4* feat[kMaxViews];
     const mvdr_cross_pair* cross;
+    const ushort4* corr[kMaxViews];     // corrMap per view (nullable)
 };
 
 // Penalty constants of one GN iteration (PenaltyConstants.h): lambda values.

@@ -386,12 +386,18 @@ __device__ inline bool choose_feature(const FrameDev& f, uint32_t g, P2PRec& rec
 // Cross-view candidate k.  The reference handler's kernels are absent
 // (CrossViewCorrespondenceHandler.h:69-79, window half size 2, step 2); this
 // is the feature rule applied across views: depth pixel in view 2, nearest
-// warp vertex around pixel 1 in view 1, reference vertex within 5 cm of the
-// observed point expressed in camera 1.
+// warp vertex around pixel 1 in view 1 (back-traced to its previous-frame
+// pixel through corrMap[v1] when the map holds a pair ending at pixel 1),
+// reference vertex within 5 cm of the observed point expressed in camera 1.
 __device__ inline bool choose_cross(const FrameDev& f, uint32_t k, P2PRec& rec, float4& canon) {
     const mvdr_cross_pair c = f.cross[k];
     const uint32_t v1 = c.view[0], v2 = c.view[1];
     if (v1 >= f.V || v2 >= f.V) return false;
+    uint32_t cx = c.pixel[0], cy = c.pixel[1];
+    if (f.corr[v1] && cx < f.W && cy < f.H) {
+        const ushort4 m = f.corr[v1][cy * f.W + cx];
+        if (m.z == cx && m.w == cy) { cx = m.x; cy = m.y; }
+    }
     int dx, dy;
     depth_pixel_in_window(f, v2, c.pixel[2], c.pixel[3], dx, dy);
     if (fetch_edge(f, v2, dx, dy) == 1) return false;
@@ -400,7 +406,7 @@ __device__ inline bool choose_cross(const FrameDev& f, uint32_t k, P2PRec& rec, 
     const f3 p0 = f.init[v2].apply(xyz(obs2));
     const f3 p1 = f.init_inv[v1].apply(p0);
     int gx, gy;
-    geometry_pixel_nearest(f, v1, p1, c.pixel[0], c.pixel[1], 2, 2, gx, gy);
+    geometry_pixel_nearest(f, v1, p1, cx, cy, 2, 2, gx, gy);
     if (!(gx < (int)f.W && gy < (int)f.H)) return false;
     const float4 ref = f.ref_v[v1][gy * f.W + gx];
     const float d2 = (ref.x - p1.x) * (ref.x - p1.x) + (ref.y - p1.y) * (ref.y - p1.y) + (ref.z - p1.z) * (ref.z - p1.z);

@@ -21,7 +21,8 @@ constexpr uint32_t kSlotDelta = 0;     // delta_k   : slot k, k <= kMaxIters
 constexpr uint32_t kSlotDQ = 72;       // dot(d, q)_k: slot 72 + k
 constexpr uint32_t kSlotRefNum = 144;  // refinement numerator / denominator
 constexpr uint32_t kSlotRefDen = 145;
-constexpr uint32_t kMaxIters = 64;
+constexpr uint32_t kMaxIters = MVDR_MAX_PCG_ITERATIONS;
+static_assert(kMaxIters < kSlotDQ, "delta_k slots 0..kMaxIters precede the dot(d, q) slots");
 
 __device__ inline double* slot_ptr(const Buffers& b, uint32_t slot) { return b.partial + (size_t)slot * kPartialStride; }
 

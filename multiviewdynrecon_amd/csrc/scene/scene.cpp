@@ -255,6 +255,7 @@ struct View {
     std::vector<uint8_t> fg_mask, edge;
     std::vector<uint32_t> index;
     std::vector<uint16_t> features;  // ushort4 per pair
+    std::vector<uint16_t> corr;      // ushort4 per pixel: the feature pair ending there, else 0
 };
 
 }  // namespace
@@ -789,6 +790,7 @@ int mvdr_scene_create(const mvdr_scene_params* params, mvdr_scene** out) {
             c4[0] = (float)sf.color[0]; c4[1] = (float)sf.color[1]; c4[2] = (float)sf.color[2]; c4[3] = 1.f;
         }
         // ---- sparse feature pairs: rendered pixel -> observed pixel of the same surfel
+        view.corr.assign(WH * 4, 0);
         std::vector<uint32_t> valid_px;
         for (size_t o = 0; o < WH; ++o) if (view.index[o] != 0xFFFFFFFFu) valid_px.push_back((uint32_t)o);
         for (uint32_t f = 0, tries = 0; f < P.features_per_view && tries < 100 * P.features_per_view + 100 && !valid_px.empty();
@@ -805,6 +807,9 @@ int mvdr_scene_create(const mvdr_scene_params* params, mvdr_scene** out) {
             view.features.push_back((uint16_t)(o / P.width));
             view.features.push_back((uint16_t)ui);
             view.features.push_back((uint16_t)wi);
+            // correspondence map (PatchColliderRGBCorrespondence.cu:171): the pair at its observed pixel
+            uint16_t* cm = &view.corr[4 * ((size_t)wi * P.width + ui)];
+            cm[0] = (uint16_t)(o % P.width); cm[1] = (uint16_t)(o / P.width); cm[2] = (uint16_t)ui; cm[3] = (uint16_t)wi;
             ++f;
         }
     }
@@ -916,6 +921,7 @@ long long mvdr_scene_get(const mvdr_scene* sc, const char* name, uint32_t view, 
         else if (n == "index_map") pick(v.index);
         else if (n == "normalized_rgb") pick(v.rgb);
         else if (n == "feature_pairs") pick(v.features);
+        else if (n == "corr_map") pick(v.corr);
         else return -1;
     }
     if (dst && bytes >= size && size) std::memcpy(dst, src, size);

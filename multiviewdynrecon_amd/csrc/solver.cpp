@@ -330,7 +330,8 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     *out = nullptr;
     const mvdr_solver_config& c = *config;
     if (c.max_views < 1 || c.max_views > MVDR_MAX_VIEWS || c.max_width < 1 || c.max_height < 1 || c.max_width > 65535 ||
-        c.max_height > 65535 || c.max_nodes < 1 || c.max_nodes > 65535 || c.precision > 1 || c.jtj_mode > 1)
+        c.max_height > 65535 || c.max_nodes < 1 || c.max_nodes > 65535 || c.precision > 1 || c.jtj_mode > 1 ||
+        c.pcg_iterations > MVDR_MAX_PCG_ITERATIONS)
         return MVDR_E_SHAPE;
     auto* s = new mvdr_solver();
     s->cfg = c;
@@ -530,6 +531,7 @@ mvdr_status mvdr_solver_set_input(mvdr_solver* s, const mvdr_frame_input* in) {
         f.index[v] = static_cast<const uint32_t*>(in->index_map[v]);
         f.rgb[v] = static_cast<const float4*>(in->normalized_rgb[v]);
         f.feat[v] = static_cast<const ushort4*>(in->feature_pairs[v]);
+        f.corr[v] = static_cast<const ushort4*>(in->corr_map[v]);
     }
     f.surfel_knn = static_cast<const ushort4*>(in->surfel_knn);
     f.surfel_w = static_cast<const float4*>(in->surfel_knn_weight);
@@ -624,6 +626,15 @@ mvdr_status mvdr_solver_total_residual(mvdr_solver* s, void* stream, double* cos
     if (!s->indexed) return fail(s, MVDR_E_STATE, "total_residual before build_index");
     hipStream_t st = static_cast<hipStream_t>(stream);
     const Penalty pen = penalty_for(s->cfg, s->iter);
+    if (!s->pair_blocks && s->iter > 0) {
+        // The mixed mode forms B_ij on demand from S_ij and the graph-edge rows
+        // of the smooth records, which the evaluation below overwrites at the
+        // new warp: form the blocks of the last iteration first.
+        s->self.nodes = live_nodes(s->frame_host.N);
+        launch_pair_blocks(s->self, kernel_penalty(s->cfg, s->iter - 1), st);
+        launch_diag_blocks(s->self, st);
+        s->pair_blocks = true;
+    }
     launch_eval_terms(s->self, s->iter & 1, s->cfg.precision == MVDR_PRECISION_MIXED, st);
     Penalty kp = pen;
     if (!s->cfg.density_in_jtj) kp.density = 0.f;
@@ -729,6 +740,7 @@ mvdr_status mvdr_solver_debug_get(mvdr_solver* s, int tap, void* dst, size_t byt
         case MVDR_TAP_DENSITY_TERMS: src = b.density; size = b.density ? sizeof(ScalarRec) * (size_t)c.dense : 0; break;
         case MVDR_TAP_SPMV_INPUT: src = b.d1; size = 24ull * N; break;
         case MVDR_TAP_SPMV_OUTPUT: src = b.q; size = 24ull * N; break;
+        case MVDR_TAP_FOREGROUND_PIXELS: src = b.fg_px; size = b.fg_px ? 8ull * c.fg : 0; break;
         default: return fail(s, MVDR_E_INVALID, "unknown tap");
     }
     if (needed) *needed = size;

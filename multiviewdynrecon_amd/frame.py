@@ -44,9 +44,10 @@ def _fill_common(fi, scene):
 
 
 class HostFrame:
-    """mvdr_frame_input over host (numpy) memory."""
+    """mvdr_frame_input over host (numpy) memory.  `corr_map` attaches each
+    view's correspondence map (scene map "corr_map") to the frame."""
 
-    def __init__(self, scene, node_se3=None):
+    def __init__(self, scene, node_se3=None, corr_map=False):
         self.scene = scene
         self.keep = []
         fi = _abi.FrameInput()
@@ -61,6 +62,8 @@ class HostFrame:
             for name, fld in MAP_FIELDS:
                 getattr(fi, fld)[v] = ptr(view.maps[name])
             fi.feature_pairs[v] = ptr(view.feature_pairs)
+            if corr_map:
+                fi.corr_map[v] = ptr(view.maps["corr_map"])
         A = scene.arrays
         fi.surfel_knn = ptr(A["surfel_knn"])
         fi.surfel_knn_weight = ptr(A["surfel_knn_weight"])
@@ -75,7 +78,7 @@ class HostFrame:
 class DeviceFrame:
     """mvdr_frame_input over device memory (torch tensors on `device`)."""
 
-    def __init__(self, scene, device="cuda", node_se3=None):
+    def __init__(self, scene, device="cuda", node_se3=None, corr_map=False):
         import torch
 
         self.scene = scene
@@ -95,6 +98,8 @@ class DeviceFrame:
             for name, fld in MAP_FIELDS:
                 getattr(fi, fld)[v] = ptr(view.maps[name])
             fi.feature_pairs[v] = ptr(view.feature_pairs)
+            if corr_map:
+                fi.corr_map[v] = ptr(view.maps["corr_map"])
         A = scene.arrays
         fi.surfel_knn = ptr(A["surfel_knn"])
         fi.surfel_knn_weight = ptr(A["surfel_knn_weight"])

@@ -119,8 +119,29 @@ def make_scene(workload=1, **overrides):
                 a = _get(h, k, v, dt, c)
                 vw.maps[k] = a.reshape(info.height, info.width, c) if c > 1 else a.reshape(info.height, info.width)
             vw.feature_pairs = _get(h, "feature_pairs", v, np.uint16, 4)
+            vw.maps["corr_map"] = _get(h, "corr_map", v, np.uint16, 4).reshape(info.height, info.width, 4)
             views.append(vw)
         params = {name: getattr(p, name) for name, _ in _abi.SceneParams._fields_}
         return Scene(params, info.width, info.height, info.num_views, float(info.voxel_size), arrays, views)
     finally:
         L.mvdr_scene_destroy(h)
+
+
+UNEVEN_SEED = 0x4D564452 + 100
+
+
+def uneven_scene_params(count=16):
+    """Sizes of the uneven-batch workload: scene s has N ~ U[1000, 8000] nodes
+    (about 37.5 surfels per node) and 1-3 views, drawn from one seeded stream."""
+    rng = np.random.default_rng(UNEVEN_SEED)
+    nodes = rng.integers(1000, 8001, count)
+    views = rng.integers(1, 4, count)
+    return [dict(seed=UNEVEN_SEED + s, target_nodes=int(nodes[s]), target_surfels=int(37.5 * nodes[s]),
+                 num_views=int(views[s]), cross_pairs=2000 if views[s] > 1 else 0) for s in range(count)]
+
+
+def make_uneven_scenes(count=16, which=None):
+    """The scenes of the uneven-batch workload (all, or the indices in `which`)."""
+    params = uneven_scene_params(count)
+    idx = range(count) if which is None else which
+    return [make_scene(4, **params[s]) for s in idx]

@@ -27,6 +27,7 @@ TAP_FORMAT = {
     "jtj_diag_inv": (np.float32, 36), "jtr": (np.float32, 6), "jtj_offdiag": (np.float32, 36),
     "twist": (np.float32, 6), "node_se3": (np.float32, 8), "penalty": (np.float32, 1),
     "density_terms": (np.uint32, 16), "spmv_input": (np.float32, 6), "spmv_output": (np.float32, 6),
+    "foreground_pixels": (np.uint16, 4),
 }
 
 

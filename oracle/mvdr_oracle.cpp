@@ -494,7 +494,11 @@ public:
     // is the depth vertex of view 2 at pixel 2, the geometry pixel is searched
     // around pixel 1 in view 1 for the warp vertex nearest to that point
     // expressed in camera 1, and the pair is kept when the reference vertex is
-    // within 5 cm of it (the feature term's 2.5e-3 m^2 test).
+    // within 5 cm of it (the feature term's 2.5e-3 m^2 test).  With a corrMap
+    // for view 1 (CrossViewCorrespondenceHandler.h:30-34), pixel 1 is first
+    // back-traced to the previous-frame pixel of the feature pair that ends
+    // at it (PatchColliderRGBCorrespondence.cu:171 stores (x, y, z, w) at
+    // (z, w)).
     void select_cross() {
         cross.clear();
         if (!f.in.num_cross_pairs) return;
@@ -503,6 +507,11 @@ public:
             const mvdr_cross_pair& c = cp[k];
             const uint32_t v1 = c.view[0], v2 = c.view[1];
             if (v1 >= f.V || v2 >= f.V) continue;
+            int cx = c.pixel[0], cy = c.pixel[1];
+            if (f.in.corr_map[v1] && cx < (int)f.W && cy < (int)f.H) {
+                const uint16_t* m = static_cast<const uint16_t*>(f.in.corr_map[v1]) + 4 * ((size_t)cy * f.W + cx);
+                if (m[2] == cx && m[3] == cy) { cx = m[0]; cy = m[1]; }
+            }
             int dx, dy;
             depth_pixel_in_window(v2, c.pixel[2], c.pixel[3], dx, dy);
             if (f.edge(v2, dx, dy) == 1) continue;
@@ -515,7 +524,7 @@ public:
             const V3<float> p1 = I1.inverse().apply(p0);
             const float obs1[3] = {p1.x, p1.y, p1.z};
             int gx, gy;
-            geometry_pixel_nearest(v1, obs1, c.pixel[0], c.pixel[1], 2, 2, gx, gy);
+            geometry_pixel_nearest(v1, obs1, cx, cy, 2, 2, gx, gy);
             if (!(gx < (int)f.W && gy < (int)f.H)) continue;
             float ref[4];
             f.fetch4(f.in.reference_vertex[v1], gx, gy, ref);

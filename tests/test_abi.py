@@ -49,7 +49,7 @@ int main(void) {
   P(mvdr_frame_input) P(mvdr_solver_sizes) P(mvdr_scene_params) P(mvdr_scene_info)
   O(mvdr_frame_input, intrinsic) O(mvdr_frame_input, world2camera) O(mvdr_frame_input, vertex_confid)
   O(mvdr_frame_input, surfel_knn) O(mvdr_frame_input, node_se3) O(mvdr_frame_input, feature_pairs)
-  O(mvdr_frame_input, num_cross_pairs) O(mvdr_solver_config, use_graph) O(mvdr_scene_params, feature_jitter_px)
+  O(mvdr_frame_input, num_cross_pairs) O(mvdr_frame_input, corr_map) O(mvdr_solver_config, use_graph) O(mvdr_scene_params, feature_jitter_px)
   return 0;
 }
 """
@@ -79,6 +79,7 @@ def test_ctypes_layout_matches_c_compiler():
         "mvdr_frame_input.node_se3": _abi.FrameInput.node_se3.offset,
         "mvdr_frame_input.feature_pairs": _abi.FrameInput.feature_pairs.offset,
         "mvdr_frame_input.num_cross_pairs": _abi.FrameInput.num_cross_pairs.offset,
+        "mvdr_frame_input.corr_map": _abi.FrameInput.corr_map.offset,
         "mvdr_solver_config.use_graph": _abi.SolverConfig.use_graph.offset,
         "mvdr_scene_params.feature_jitter_px": _abi.SceneParams.feature_jitter_px.offset,
     }
@@ -100,7 +101,8 @@ def test_create_rejects_bad_config_without_gpu(built_libs):
     # shape validation happens before any device allocation
     from multiviewdynrecon_amd.solver import lib
 
-    for bad in (dict(max_views=0), dict(max_views=4), dict(max_nodes=70000), dict(precision=5)):
+    for bad in (dict(max_views=0), dict(max_views=4), dict(max_nodes=70000), dict(precision=5),
+                dict(pcg_iterations=65)):
         cfg = config.default_config(**bad)
         h = C.c_void_p()
         assert lib().mvdr_solver_create(C.byref(cfg), C.byref(h)) == 2  # MVDR_E_SHAPE
