@@ -132,8 +132,29 @@ def cpu_baseline_leg(scene, workload, frames):
         if time.perf_counter() - t_all > 30.0:
             break
     cpu = {"value": float(np.median(times) * 1e3), "unit": "ms/frame", "cores": 1, "kind": "port",
-           "sample": f"{len(times)} full solves (index + GN x PCG) of the benchmark frame, FP32, single thread, median"}
+           "sample": f"{len(times)} full solves (index + GN x PCG) of the benchmark frame, FP32, single thread, median",
+           "cpu_model": cpu_model()}
     return cpu
+
+
+def cpu_model():
+    """The host CPU's model name (lscpu's "Model name", read from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def baseline_threads():
+    """Every CPU this process may run on, capped by OMP_NUM_THREADS when the
+    environment sets a CPU share for the job."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(avail, int(share))) if share and share.isdigit() else avail
 
 
 def cpu_baseline_threads(scene, workload, threads):
@@ -154,7 +175,8 @@ def cpu_baseline_threads(scene, workload, threads):
         list(ex.map(lambda o: o.solve(), oracles))
     wall = time.perf_counter() - t0
     return {"value": float(wall * 1e3 / len(oracles)), "unit": "ms/frame", "cores": threads, "kind": "port",
-            "sample": f"{len(oracles)} full solves of the benchmark frame on {threads} threads, FP32, aggregate"}
+            "sample": f"{len(oracles)} full solves of the benchmark frame on {threads} threads, FP32, aggregate",
+            "host_cpus": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)), "cpu_model": cpu_model()}
 
 
 def twist_errors(scene, cfg, frame_dev):
@@ -398,7 +420,7 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline_leg(scene, args.workload, args.cpu_frames)
         print(f"[bench] cpu baseline {cpu['value']:.1f} ms/frame", file=sys.stderr, flush=True)
-        cpu_mt = cpu_baseline_threads(scene, args.workload, min(16, os.cpu_count() or 1))
+        cpu_mt = cpu_baseline_threads(scene, args.workload, baseline_threads())
         print(f"[bench] cpu baseline, {cpu_mt['cores']} threads: {cpu_mt['value']:.1f} ms/frame", file=sys.stderr,
               flush=True)
         errs = twist_errors(scene, cfg, frame)

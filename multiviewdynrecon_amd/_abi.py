@@ -98,3 +98,16 @@ MAX_PCG_ITERATIONS = 64  # MVDR_MAX_PCG_ITERATIONS
 
 STATUS = {0: "MVDR_OK", 1: "MVDR_E_INVALID", 2: "MVDR_E_SHAPE", 3: "MVDR_E_CAPACITY", 4: "MVDR_E_HIP",
           5: "MVDR_E_STATE"}
+
+
+class CrossViewConfig(C.Structure):
+    _fields_ = [("max_views", u32), ("max_width", u32), ("max_height", u32), ("max_pairs", u32),
+                ("search_radius", u32), ("search_step", u32), ("track_radius", u32), ("max_distance2", f32)]
+
+
+class CrossViewInput(C.Structure):
+    _fields_ = [("num_views", u32), ("width", u32), ("height", u32), ("intrinsic", Intrinsic * V),
+                ("initial_camera_se3", Mat34 * V), ("vertex", vp * V), ("edge_mask", vp * V), ("corr_map", vp * V)]
+
+
+CROSSVIEW_LISTS = {"candidates": 0, "unique": 1, "backtraced": 2}

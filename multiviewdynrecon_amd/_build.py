@@ -22,7 +22,8 @@ ARCH = os.environ.get("MVDR_OFFLOAD_ARCH", "gfx950")
 # packed FP32 instructions disabled for device code (v_pk_*_f32 forms)
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-Wall", "-Wno-unused-function", "-I", INCLUDE]
-HIP_SOURCES = ["kernels_index.hip", "kernels_solve.hip", "kernels_pcg.hip", "kernels_mf.hip", "solver.cpp"]
+HIP_SOURCES = ["kernels_index.hip", "kernels_solve.hip", "kernels_pcg.hip", "kernels_mf.hip", "crossview.hip",
+               "solver.cpp"]
 
 
 def _run(cmd):

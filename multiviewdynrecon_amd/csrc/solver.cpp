@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -99,15 +100,6 @@ mvdr_status fail(mvdr_solver* s, mvdr_status st, const std::string& msg) {
         hipError_t e_ = (call);                                                                     \
         if (e_ != hipSuccess) return fail((s), MVDR_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
-
-template <typename T>
-hipError_t alloc(mvdr_solver* s, T*& ptr, size_t count) {
-    void* p = nullptr;
-    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
-    if (e == hipSuccess) s->allocations.push_back(p);
-    ptr = static_cast<T*>(p);
-    return e;
-}
 
 // setElasticPenaltyValue (SolverIterationData.cpp:104-125) with the constants
 // of PenaltyConstants.cpp:9-51.  Iteration 0 uses the reference's default flags
@@ -325,19 +317,27 @@ void mvdr_solver_default_config(mvdr_solver_config* c) {
 
 const char* mvdr_solver_last_error(const mvdr_solver* s) { return s ? s->error.c_str() : "null solver"; }
 
-mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** out) {
-    if (!config || !out) return MVDR_E_INVALID;
-    *out = nullptr;
-    const mvdr_solver_config& c = *config;
+// Shape checks of a config that precede any device work.
+static bool config_shape_ok(const mvdr_solver_config& c) {
     if (c.max_views < 1 || c.max_views > MVDR_MAX_VIEWS || c.max_width < 1 || c.max_height < 1 || c.max_width > 65535 ||
         c.max_height > 65535 || c.max_nodes < 1 || c.max_nodes > 65535 || c.precision > 1 || c.jtj_mode > 1 ||
         c.pcg_iterations > MVDR_MAX_PCG_ITERATIONS)
-        return MVDR_E_SHAPE;
-    auto* s = new mvdr_solver();
-    s->cfg = c;
-    Buffers& b = s->b;
+        return false;
     const uint64_t pixels = (uint64_t)c.max_views * c.max_width * c.max_height;
-    if (pixels > 0x7FFFFFFFull) { delete s; return MVDR_E_SHAPE; }
+    if (pixels > 0x7FFFFFFFull) return false;
+    const uint64_t fg = c.use_foreground ? pixels : 0;
+    const uint64_t knn_terms = pixels + fg + std::max<uint32_t>(c.max_feature_pairs, 1) + std::max<uint32_t>(c.max_cross_pairs, 1);
+    return 6 * knn_terms + std::max<uint32_t>(c.max_graph_edges, 1) <= 0xFFFFFFFFull;
+}
+
+// Every per-frame device buffer of config c as 256-byte aligned pieces of one
+// block at `base` (base == nullptr: pointers stay null and only the size is
+// computed).  Also places the device copies of the frame table and of b.
+// Returns the block's size in bytes.
+static size_t layout_buffers(const mvdr_solver_config& c, Buffers& b, FrameDev*& frame_dev, Buffers*& b_dev,
+                             uint32_t& cap_tiles, uint32_t& cap_chunks, char* base) {
+    std::memset(&b, 0, sizeof(b));
+    const uint64_t pixels = (uint64_t)c.max_views * c.max_width * c.max_height;
     b.cap_pixels = (uint32_t)pixels;
     b.cap_feat = std::max<uint32_t>(c.max_feature_pairs, 1);
     b.cap_cross = std::max<uint32_t>(c.max_cross_pairs, 1);
@@ -347,25 +347,27 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     const uint64_t knn_terms = pixels + fg + b.cap_feat + b.cap_cross;
     const uint64_t kv1 = 4 * knn_terms + 2ull * b.cap_edges;
     const uint64_t kv2 = 6 * knn_terms + b.cap_edges;
-    if (kv2 > 0xFFFFFFFFull) { delete s; return MVDR_E_SHAPE; }
     b.cap_kv1 = (uint32_t)kv1;
     b.cap_kv2 = (uint32_t)kv2;
     b.cap_pairs = (uint32_t)std::min<uint64_t>(kv2, 64ull * c.max_nodes);
-    s->cap_tiles = (uint32_t)((pixels + 4095) / 4096);
-    s->cap_chunks = index_chunks(knn_terms + b.cap_edges);
+    cap_tiles = (uint32_t)((pixels + 4095) / 4096);
+    cap_chunks = index_chunks(knn_terms + b.cap_edges);
     const uint32_t N1 = c.max_nodes + 1;
 
-    hipError_t e = hipSuccess;
-    auto A = [&](auto*& p, size_t n) { if (e == hipSuccess) e = alloc(s, p, n); };
-    A(s->frame_dev, 1);
+    size_t off = 0;
+    auto A = [&](auto*& p, size_t n) {
+        using T = std::remove_reference_t<decltype(*p)>;
+        off = (off + 255) & ~(size_t)255;
+        p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += std::max<size_t>(n, 1) * sizeof(T);
+    };
+    A(frame_dev, 1);
+    A(b_dev, 1);
     A(b.counts, 1);
     A(b.dense_px, b.cap_pixels);
     A(b.dense_knn, b.cap_pixels);
     A(b.dense, b.cap_pixels);
-    b.density = nullptr;
     if (c.density_in_jtj) A(b.density, b.cap_pixels);
-    b.fg_px = nullptr;
-    b.fg = nullptr;
     if (c.use_foreground) { A(b.fg_px, b.cap_pixels); A(b.fg, b.cap_pixels); }
     A(b.smooth, b.cap_edges);
     A(b.feat, b.cap_feat);
@@ -375,10 +377,10 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     A(b.sel_ok, (size_t)b.cap_feat + b.cap_cross);
     A(b.sel_rec, (size_t)b.cap_feat + b.cap_cross);
     A(b.sel_canon, (size_t)b.cap_feat + b.cap_cross);
-    A(b.tile_count, s->cap_tiles);
-    A(b.tile_offset, s->cap_tiles);
+    A(b.tile_count, cap_tiles);
+    A(b.tile_offset, cap_tiles);
     A(b.feat_flag, 1);
-    A(b.n2t_hist, (size_t)s->cap_chunks * c.max_nodes);
+    A(b.n2t_hist, (size_t)cap_chunks * c.max_nodes);
     A(b.n2t_count, N1);
     A(b.n2t_offset, N1);
     A(b.n2t_index, b.cap_kv1);
@@ -392,11 +394,7 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     A(b.pair_range, b.cap_pairs);
     A(b.pair_terms, b.cap_kv2);
     A(b.kv2_pair, b.cap_kv2);
-    b.part_diag = b.part_pair = nullptr;
     const bool mf = c.jtj_mode == MVDR_JTJ_MATRIX_FREE;
-    b.pair_edge_n = b.pair_edge = b.n2t_col = nullptr;
-    b.row_part = nullptr;
-    b.mf_u = nullptr;
     if (mf) {
         A(b.mf_u, knn_terms + b.cap_edges);
     } else if (c.precision == MVDR_PRECISION_MIXED) {
@@ -412,19 +410,12 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     A(b.low_ent, b.cap_pairs);
     A(b.diag, 36 * (size_t)c.max_nodes);
     A(b.diag_inv, 36 * (size_t)c.max_nodes);
-    b.diag_inv64 = nullptr;
     // FP64 factors: the mixed mode, and the row Gram assembly of the matrix-free solve
     const bool factors64 = c.precision == MVDR_PRECISION_MIXED || mf;
     if (factors64) A(b.diag_inv64, 36 * (size_t)c.max_nodes);
     A(b.jtr, 6 * (size_t)c.max_nodes);
     A(b.offdiag, 36 * (size_t)b.cap_pairs);
     for (float** v : {&b.x, &b.r, &b.s, &b.q, &b.d0, &b.d1}) A(*v, 6 * (size_t)c.max_nodes);
-    b.linv64 = nullptr;
-    b.offhat = nullptr;
-    b.row_ent = nullptr;
-    b.row_ptr = nullptr;
-    b.low_pos = nullptr;
-    b.cg_scal = nullptr;
     if (factors64) A(b.linv64, kTri * (size_t)c.max_nodes);
     if (c.precision == MVDR_PRECISION_MIXED && !mf) {
         A(b.offhat, 36 * (size_t)b.cap_pairs);
@@ -439,16 +430,28 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
     A(b.partial, (size_t)kPartialSlots * kPartialStride);
     A(b.se3[0], c.max_nodes);
     A(b.se3[1], c.max_nodes);
-    if (e == hipSuccess) e = hipMemset(b.counts, 0, sizeof(Counts));
+    b.frame = frame_dev;
+    return (off + 255) & ~(size_t)255;
+}
+
+mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** out) {
+    if (!config || !out) return MVDR_E_INVALID;
+    *out = nullptr;
+    const mvdr_solver_config& c = *config;
+    if (!config_shape_ok(c)) return MVDR_E_SHAPE;
+    auto* s = new mvdr_solver();
+    s->cfg = c;
+    Buffers& b = s->b;
+    const size_t bytes = layout_buffers(c, b, s->frame_dev, s->b_dev, s->cap_tiles, s->cap_chunks, nullptr);
+    void* block = nullptr;
+    hipError_t e = hipMalloc(&block, bytes);
+    if (e == hipSuccess) {
+        s->allocations.push_back(block);
+        layout_buffers(c, b, s->frame_dev, s->b_dev, s->cap_tiles, s->cap_chunks, static_cast<char*>(block));
+        e = hipMemset(b.counts, 0, sizeof(Counts));
+    }
     if (e == hipSuccess) e = hipMemset(b.partial, 0, (size_t)kPartialSlots * kPartialStride * sizeof(double));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->done, hipEventDisableTiming);
-    if (e != hipSuccess) {
-        std::fprintf(stderr, "mvdr_solver_create: %s\n", hipGetErrorString(e));
-        delete s;
-        return MVDR_E_HIP;
-    }
-    b.frame = s->frame_dev;
-    if (e == hipSuccess) e = alloc(s, s->b_dev, 1);
     if (e == hipSuccess) e = hipMemcpy(s->b_dev, &b, sizeof(Buffers), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         std::fprintf(stderr, "mvdr_solver_create: %s\n", hipGetErrorString(e));
@@ -466,36 +469,37 @@ void mvdr_solver_destroy(mvdr_solver* s) {
     delete s;
 }
 
-mvdr_status mvdr_solver_set_input(mvdr_solver* s, const mvdr_frame_input* in) {
-    if (!s || !in) return MVDR_E_INVALID;
-    const mvdr_solver_config& c = s->cfg;
+// Validates a frame against config c (SetSolverInput's checks) and fills the
+// device pointer table f.  On failure `err` holds the message.
+static mvdr_status frame_from_input(const mvdr_solver_config& c, const mvdr_frame_input* in, FrameDev& f,
+                                    const char*& err) {
+    auto fail = [&](mvdr_status st, const char* msg) { err = msg; return st; };
     if (in->num_views < 1 || in->num_views > c.max_views || in->width < 1 || in->height < 1 || in->width > c.max_width ||
         in->height > c.max_height)
-        return fail(s, MVDR_E_SHAPE, "views / image size outside the configured maximum");
+        return fail(MVDR_E_SHAPE, "views / image size outside the configured maximum");
     if (in->num_nodes < 1 || in->num_nodes > c.max_nodes)
-        return fail(s, MVDR_E_SHAPE, "num_nodes outside [1, max_nodes]");
-    if (in->num_graph_edges > c.max_graph_edges) return fail(s, MVDR_E_SHAPE, "num_graph_edges > max_graph_edges");
+        return fail(MVDR_E_SHAPE, "num_nodes outside [1, max_nodes]");
+    if (in->num_graph_edges > c.max_graph_edges) return fail(MVDR_E_SHAPE, "num_graph_edges > max_graph_edges");
     uint64_t feat_total = 0;
     for (uint32_t v = 0; v < in->num_views; ++v) feat_total += in->num_feature_pairs[v];
-    if (feat_total > c.max_feature_pairs) return fail(s, MVDR_E_SHAPE, "feature pairs > max_feature_pairs");
-    if (in->num_cross_pairs > c.max_cross_pairs) return fail(s, MVDR_E_SHAPE, "cross pairs > max_cross_pairs");
+    if (feat_total > c.max_feature_pairs) return fail(MVDR_E_SHAPE, "feature pairs > max_feature_pairs");
+    if (in->num_cross_pairs > c.max_cross_pairs) return fail(MVDR_E_SHAPE, "cross pairs > max_cross_pairs");
     if (!in->surfel_knn || !in->surfel_knn_weight || !in->node_se3 || !in->canonical_nodes || in->num_surfels == 0)
-        return fail(s, MVDR_E_INVALID, "surfel knn / node arrays must be set");
-    if (in->num_graph_edges && !in->node_graph) return fail(s, MVDR_E_INVALID, "node_graph is null");
-    if (in->num_cross_pairs && c.use_cross_view && !in->cross_pairs) return fail(s, MVDR_E_INVALID, "cross_pairs is null");
+        return fail(MVDR_E_INVALID, "surfel knn / node arrays must be set");
+    if (in->num_graph_edges && !in->node_graph) return fail(MVDR_E_INVALID, "node_graph is null");
+    if (in->num_cross_pairs && c.use_cross_view && !in->cross_pairs) return fail(MVDR_E_INVALID, "cross_pairs is null");
     for (uint32_t v = 0; v < in->num_views; ++v) {
         if (!in->vertex_confid[v] || !in->normal_radius[v] || !in->reference_vertex[v] || !in->reference_normal[v] ||
             !in->warp_vertex[v] || !in->index_map[v])
-            return fail(s, MVDR_E_INVALID, "a required per-view map is null");
+            return fail(MVDR_E_INVALID, "a required per-view map is null");
         if ((in->num_feature_pairs[v] || (in->num_cross_pairs && c.use_cross_view)) && !in->edge_mask[v])
-            return fail(s, MVDR_E_INVALID, "edge_mask is required with feature or cross-view pairs");
-        if (in->num_feature_pairs[v] && !in->feature_pairs[v]) return fail(s, MVDR_E_INVALID, "feature_pairs is null");
+            return fail(MVDR_E_INVALID, "edge_mask is required with feature or cross-view pairs");
+        if (in->num_feature_pairs[v] && !in->feature_pairs[v]) return fail(MVDR_E_INVALID, "feature_pairs is null");
         if (c.use_foreground && (!in->filtered_foreground[v] || !in->foreground_gradient[v]))
-            return fail(s, MVDR_E_INVALID, "foreground maps are required when use_foreground is set");
+            return fail(MVDR_E_INVALID, "foreground maps are required when use_foreground is set");
         if (c.density_in_jtj && (!in->gray[v] || !in->gray_gradient[v] || !in->normalized_rgb[v]))
-            return fail(s, MVDR_E_INVALID, "gray / rgb maps are required when density_in_jtj is set");
+            return fail(MVDR_E_INVALID, "gray / rgb maps are required when density_in_jtj is set");
     }
-    FrameDev& f = s->frame_host;
     std::memset(&f, 0, sizeof(f));
     f.V = in->num_views;
     f.W = in->width;
@@ -539,6 +543,17 @@ mvdr_status mvdr_solver_set_input(mvdr_solver* s, const mvdr_frame_input* in) {
     f.nodes = static_cast<const float4*>(in->canonical_nodes);
     f.graph = static_cast<const ushort2*>(in->node_graph);
     f.cross = static_cast<const mvdr_cross_pair*>(in->cross_pairs);
+    return MVDR_OK;
+}
+
+
+mvdr_status mvdr_solver_set_input(mvdr_solver* s, const mvdr_frame_input* in) {
+    if (!s || !in) return MVDR_E_INVALID;
+    FrameDev f;
+    const char* err = nullptr;
+    const mvdr_status rc = frame_from_input(s->cfg, in, f, err);
+    if (rc != MVDR_OK) return fail(s, rc, err);
+    s->frame_host = f;
     // the previous solve may still read the pointer table
     HIP_TRY(s, hipEventSynchronize(s->done));
     HIP_TRY(s, hipMemcpy(s->frame_dev, &f, sizeof(FrameDev), hipMemcpyHostToDevice));

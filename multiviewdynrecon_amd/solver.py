@@ -75,6 +75,24 @@ def lib():
     return _LIB
 
 
+_HIP = None
+
+
+def _memcpy_d2h(dst, src, nbytes):
+    """hipMemcpy device -> host through the process's HIP runtime (the one torch
+    loaded, found by its soname)."""
+    global _HIP
+    if _HIP is None:
+        import torch  # noqa: F401  (loads the runtime)
+
+        _HIP = C.CDLL("libamdhip64.so.7")
+        _HIP.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _HIP.hipMemcpy.restype = C.c_int
+    rc = _HIP.hipMemcpy(dst, src, nbytes, 2)  # hipMemcpyDeviceToHost
+    if rc != 0:
+        raise SolverError(4, f"hipMemcpy failed ({rc})")
+
+
 def _stream_handle(stream):
     if stream is None:
         import torch

@@ -26,12 +26,14 @@
 //   PCG ............................. numpy PCG recurrence and a direct solve
 //   warp update ..................... Rodrigues closed form
 //   cross-view term ................. parity unpinned (reference implementation absent)
+#include "../include/mvdr/crossview.h"
 #include "../include/mvdr/solver.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -1409,6 +1411,103 @@ int mvdr_oracle_set(mvdr_oracle* h, const char* name, const double* src, size_t 
         if (n == "cross_terms") return set_p2p(o, true, src, count);
         return set_array(o, n, src, count);
     });
+}
+
+// Cross-view edge pairs (CrossViewEdgeCorrespondence.h:21-224; the
+// implementation is absent from the reference, so this follows the rules
+// stated in include/mvdr/crossview.h -- PARITY UNPINNED against the
+// reference).  Plain loops over every pixel and window tap, FP32 in the
+// device's operation order.  Writes up to `cap` pairs (6 u16 each: x1, y1,
+// x2, y2, view1, view2) of list 0 (candidates), 1 (unique) or 2
+// (back-traced); returns the list's full length.
+long long mvdr_oracle_crossview(const mvdr_crossview_config* cfg, const mvdr_crossview_input* in, int list,
+                                uint16_t* out, uint32_t cap) {
+    if (!cfg || !in || list < 0 || list > 2) return -1;
+    const uint32_t V = in->num_views, W = in->width, H = in->height, plane = W * H;
+    auto vertex = [&](uint32_t v, int x, int y, float o[4]) {
+        std::memcpy(o, static_cast<const float*>(in->vertex[v]) + 4 * ((size_t)y * W + x), 16);
+    };
+    auto is_edge = [&](uint32_t v, int x, int y, float o[4]) {
+        if (x < 0 || y < 0 || x >= (int)W || y >= (int)H) return false;
+        if (static_cast<const uint8_t*>(in->edge_mask[v])[(size_t)y * W + x] != 1) return false;
+        vertex(v, x, y, o);
+        return !is_zero_vertex<float>(o);
+    };
+    struct Cand { bool ok; uint32_t view2; int x2, y2; float d2; };
+    std::vector<Cand> cand((size_t)V * plane, Cand{false, 0, 0, 0, 0.f});
+    const int R = (int)cfg->search_radius, S = (int)cfg->search_step, T = (int)cfg->track_radius;
+    for (uint32_t v = 0; v < V && V > 1; ++v) {
+        const M34<float> Iv = M34<float>::from(in->initial_camera_se3[v]);
+        for (int y = 0; y < (int)H; ++y)
+            for (int x = 0; x < (int)W; ++x) {
+                float src[4];
+                if (!is_edge(v, x, y, src)) continue;
+                const V3<float> world = Iv.apply(v3<float>(src[0], src[1], src[2]));
+                float best = INFINITY;
+                Cand c{false, 0, 0, 0, 0.f};
+                const uint32_t next = (v + 1) % V, last = (v + V - 1) % V;
+                for (int k = 0; k < 2; ++k) {
+                    const uint32_t n = k == 0 ? next : last;
+                    if (k == 1 && last == next) break;
+                    const V3<float> q = M34<float>::from(in->initial_camera_se3[n]).inverse().apply(world);
+                    if (!(q.z > 0.f)) continue;
+                    const mvdr_intrinsic K = in->intrinsic[n];
+                    const int u = project_round(q.x, q.z, K.fx, K.cx), w = project_round(q.y, q.z, K.fy, K.cy);
+                    float nb = INFINITY;
+                    int bx = -1, by = -1;
+                    for (int yy = w - R; yy <= w + R; yy += S)
+                        for (int xx = u - R; xx <= u + R; xx += S) {
+                            float t[4];
+                            if (!is_edge(n, xx, yy, t)) continue;
+                            const float dx = t[0] - q.x, dy = t[1] - q.y, dz = t[2] - q.z;
+                            const float d2 = dx * dx + dy * dy + dz * dz;
+                            if (d2 < nb) { nb = d2; bx = xx; by = yy; }
+                        }
+                    if (nb < best) { best = nb; c = Cand{true, n, bx, by, nb}; }
+                }
+                if (c.ok && best < cfg->max_distance2) cand[(size_t)v * plane + (size_t)y * W + x] = c;
+            }
+    }
+    auto tgt = [&](const Cand& c) { return (size_t)c.view2 * plane + (size_t)c.y2 * W + c.x2; };
+    // per target: the nearest candidate, ties to the lower source pixel
+    std::map<size_t, size_t> winner;
+    for (size_t p = 0; p < cand.size(); ++p) {
+        if (!cand[p].ok) continue;
+        auto it = winner.find(tgt(cand[p]));
+        if (it == winner.end() || cand[p].d2 < cand[it->second].d2) winner[tgt(cand[p])] = p;
+    }
+    auto traceable = [&](uint32_t v, int x, int y) {
+        if (!in->corr_map[v]) return false;
+        for (int yy = y - T; yy <= y + T; ++yy)
+            for (int xx = x - T; xx <= x + T; ++xx) {
+                if (xx < 0 || yy < 0 || xx >= (int)W || yy >= (int)H) continue;
+                const uint16_t* e = static_cast<const uint16_t*>(in->corr_map[v]) + 4 * ((size_t)yy * W + xx);
+                if (e[2] == xx && e[3] == yy) return true;
+            }
+        return false;
+    };
+    long long n = 0;
+    for (size_t p = 0; p < cand.size(); ++p) {
+        const Cand& c = cand[p];
+        if (!c.ok) continue;
+        const size_t t = tgt(c);
+        bool unique = winner[t] == p;
+        if (unique && t < p && cand[t].ok && tgt(cand[t]) == p) {
+            auto it = winner.find(p);
+            if (it != winner.end() && it->second == t) unique = false;
+        }
+        const uint32_t v = (uint32_t)(p / plane), o = (uint32_t)(p % plane);
+        const bool keep = list == 0 || (unique && (list == 1 || (traceable(v, (int)(o % W), (int)(o / W)) &&
+                                                                traceable(c.view2, c.x2, c.y2))));
+        if (!keep) continue;
+        if (out && (uint64_t)n < cap) {
+            uint16_t* r = out + 6 * n;
+            r[0] = (uint16_t)(o % W); r[1] = (uint16_t)(o / W); r[2] = (uint16_t)c.x2; r[3] = (uint16_t)c.y2;
+            r[4] = (uint16_t)v; r[5] = (uint16_t)c.view2;
+        }
+        ++n;
+    }
+    return n;
 }
 
 }  // extern "C"

@@ -307,3 +307,74 @@ def cross_backtrace_frame(entry):
 
 
 CROSS_ENTRIES = {"no_entry": None, "entry_backtraces": (2, 2, 5, 5), "entry_for_another_pixel": (2, 2, 4, 5)}
+
+
+# ------------------------------------------------------- cross-view pair cases
+class PlantedCrossView:
+    """V views of W x H sharing one camera pose (identity) and intrinsics
+    (f = 20, c = (16, 16)): a vertex of view v at pixel (x, y) projects to the
+    same pixel in every other view.  Holds the per-view vertex / edge /
+    correspondence maps of a crossview input."""
+
+    def __init__(self, views=2, W=32, H=32, f=20.0):
+        self.W, self.H, self.f = W, H, f
+        self.num_views, self.width, self.height = views, W, H
+        self.views = []
+        for _ in range(views):
+            maps = {"vertex_confid": np.zeros((H, W, 4), np.float32), "edge_mask": np.zeros((H, W), np.uint8),
+                    "corr_map": np.zeros((H, W, 4), np.uint16)}
+            self.views.append(View(np.array([f, f, W / 2, H / 2], np.float32), identity34(), identity34(), maps,
+                                   np.zeros((0, 4), np.uint16)))
+
+    def ray(self, x, y, depth=1.0):
+        """The camera point at `depth` that projects to pixel (x, y)."""
+        return np.array([(x - self.W / 2) / self.f * depth, (y - self.H / 2) / self.f * depth, depth], np.float32)
+
+    def edge(self, v, x, y, p):
+        m = self.views[v].maps
+        m["edge_mask"][y, x] = 1
+        m["vertex_confid"][y, x] = [p[0], p[1], p[2], 1.0]
+
+    def flow(self, v, x, y):
+        """An optical-flow correspondence ending at (x, y) of view v."""
+        self.views[v].maps["corr_map"][y, x] = [x, y, x, y]
+
+
+def crossview_case_basic():
+    """Source (10, 10) of view 0 and two edge pixels of view 1 near its ray;
+    each view-1 pixel in turn finds (10, 10) of view 0.  Expected lists as
+    (x1, y1, x2, y2, view1, view2)."""
+    cv = PlantedCrossView()
+    p0 = cv.ray(10, 10)
+    cv.edge(0, 10, 10, p0)
+    cv.edge(1, 12, 10, p0 + np.array([0.004, 0, 0], np.float32))   # d^2 1.6e-5
+    cv.edge(1, 10, 12, p0 + np.array([0, 0.002, 0], np.float32))   # d^2 4e-6: nearer
+    cands = [(10, 10, 10, 12, 0, 1), (12, 10, 10, 10, 1, 0), (10, 12, 10, 10, 1, 0)]
+    # target (10, 10) of view 0 goes to its nearest source (10, 12); that pair
+    # and (10, 10) -> (10, 12) are one correspondence, kept from view 0
+    unique = [(10, 10, 10, 12, 0, 1)]
+    return cv, cands, unique
+
+
+def crossview_case_window():
+    """The threshold and the sampled window: view 0 sources at rows 4, 20 and 26."""
+    cv = PlantedCrossView()
+    a = cv.ray(4, 4)
+    cv.edge(0, 4, 4, a)
+    cv.edge(1, 5, 4, a + np.array([0.001, 0, 0], np.float32))    # odd offset from the projection: never sampled
+    cv.edge(1, 6, 4, a + np.array([0.009, 0, 0], np.float32))    # even offset, d^2 8.1e-5 < 1e-4
+    b = cv.ray(20, 20)
+    cv.edge(0, 20, 20, b)
+    cv.edge(1, 22, 20, b + np.array([0.0105, 0, 0], np.float32))  # d^2 1.1e-4: above the threshold
+    c = cv.ray(26, 26)
+    cv.edge(0, 26, 26, c)
+    cv.edge(1, 26, 26, c + np.array([0, 0, 0.003], np.float32))   # found; view 1 (26, 26) finds it back
+    # view 1 sources: the vertices of (5, 4) and (6, 4) project to column 4 of
+    # view 0 (1 mm and 9 mm move a projection by 0.02 / 0.18 px), so both find
+    # (4, 4); (22, 20) finds (20, 20) above the threshold; (26, 26) finds (26, 26)
+    cands = [(4, 4, 6, 4, 0, 1), (26, 26, 26, 26, 0, 1), (5, 4, 4, 4, 1, 0), (6, 4, 4, 4, 1, 0),
+             (26, 26, 26, 26, 1, 0)]
+    # target (4, 4) of view 0 keeps its nearest source (5, 4); (26, 26) <-> (26, 26)
+    # is found from both ends and kept from view 0
+    unique = [(4, 4, 6, 4, 0, 1), (26, 26, 26, 26, 0, 1), (5, 4, 4, 4, 1, 0)]
+    return cv, cands, unique

@@ -29,6 +29,26 @@ def test_solver_exports_every_declared_symbol(built_libs):
     assert not missing, missing
 
 
+def test_crossview_exports_every_declared_symbol(built_libs):
+    from multiviewdynrecon_amd.solver import lib
+
+    L = lib()
+    names = declared("crossview.h")
+    assert len(names) == 5
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_crossview_create_rejects_bad_config_without_gpu(built_libs):
+    from multiviewdynrecon_amd.crossview import default_config, lib
+
+    for bad in (dict(max_views=4), dict(search_step=0), dict(max_pairs=0), dict(max_distance2=-1.0)):
+        cfg = default_config(**bad)
+        h = C.c_void_p()
+        assert lib().mvdr_crossview_create(C.byref(cfg), C.byref(h)) == 2
+        assert not h.value
+
+
 def test_scene_exports_every_declared_symbol(built_libs):
     from multiviewdynrecon_amd.scene import lib
 
@@ -42,11 +62,13 @@ LAYOUT_C = r"""
 #include <stddef.h>
 #include "mvdr/solver.h"
 #include "mvdr/scene.h"
+#include "mvdr/crossview.h"
 #define P(T) printf(#T " %zu\n", sizeof(T));
 #define O(T, f) printf(#T "." #f " %zu\n", offsetof(T, f));
 int main(void) {
   P(mvdr_intrinsic) P(mvdr_mat34) P(mvdr_dq) P(mvdr_cross_pair) P(mvdr_solver_config)
   P(mvdr_frame_input) P(mvdr_solver_sizes) P(mvdr_scene_params) P(mvdr_scene_info)
+  P(mvdr_crossview_config) P(mvdr_crossview_input) O(mvdr_crossview_input, corr_map) O(mvdr_crossview_config, max_distance2)
   O(mvdr_frame_input, intrinsic) O(mvdr_frame_input, world2camera) O(mvdr_frame_input, vertex_confid)
   O(mvdr_frame_input, surfel_knn) O(mvdr_frame_input, node_se3) O(mvdr_frame_input, feature_pairs)
   O(mvdr_frame_input, num_cross_pairs) O(mvdr_frame_input, corr_map) O(mvdr_solver_config, use_graph) O(mvdr_scene_params, feature_jitter_px)
@@ -68,6 +90,7 @@ def test_ctypes_layout_matches_c_compiler():
         "mvdr_cross_pair": C.sizeof(_abi.CrossPair), "mvdr_solver_config": C.sizeof(_abi.SolverConfig),
         "mvdr_frame_input": C.sizeof(_abi.FrameInput), "mvdr_solver_sizes": C.sizeof(_abi.SolverSizes),
         "mvdr_scene_params": C.sizeof(_abi.SceneParams), "mvdr_scene_info": C.sizeof(_abi.SceneInfo),
+        "mvdr_crossview_config": C.sizeof(_abi.CrossViewConfig), "mvdr_crossview_input": C.sizeof(_abi.CrossViewInput),
     }
     for k, v in py.items():
         assert int(c_vals[k]) == v, k
@@ -82,6 +105,8 @@ def test_ctypes_layout_matches_c_compiler():
         "mvdr_frame_input.corr_map": _abi.FrameInput.corr_map.offset,
         "mvdr_solver_config.use_graph": _abi.SolverConfig.use_graph.offset,
         "mvdr_scene_params.feature_jitter_px": _abi.SceneParams.feature_jitter_px.offset,
+        "mvdr_crossview_input.corr_map": _abi.CrossViewInput.corr_map.offset,
+        "mvdr_crossview_config.max_distance2": _abi.CrossViewConfig.max_distance2.offset,
     }
     for k, v in offs.items():
         assert int(c_vals[k]) == v, k
