@@ -268,6 +268,27 @@ mvdr_status mvdr_batch_solve(mvdr_batch* batch, void* stream);
 /* mvdr_solver_bench_stage over all members at once (one launch per rep). */
 mvdr_status mvdr_batch_bench_stage(mvdr_batch* batch, int stage, uint32_t reps, void* stream, float* avg_ms);
 
+/* ------------------------------------------------------ allocation-free solve
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): SolveNonRigidAlignment of one frame without a handle --
+ * index build and config->gn_iterations GN steps enqueued on `stream`, the
+ * solved warp field written to node_se3_out (device, mvdr_dq[num_nodes]).
+ * Every buffer of the solve lives in one library-owned device array of
+ * MVDR_NOALLOC_SCRATCH_BYTES per device (held until the process ends); a
+ * config whose buffers do not fit returns MVDR_E_CAPACITY before any GPU work
+ * (max_nodes about 2,000 at one 320x240 view in the mixed precision, 4,000 in
+ * FP32).  Nothing is allocated; the call does not synchronise; use_graph is
+ * ignored.  device_error (nullable, device u32) receives the solve's
+ * capacity error bits (0 = none; on an error the output is the input warp).
+ * Calls that overlap on one device share that array and race: serialise them
+ * (one stream, or events).  The results equal mvdr_solver_solve's for the
+ * same config and frame. */
+#define MVDR_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+mvdr_status mvdr_solve_noalloc(const mvdr_solver_config* config, const mvdr_frame_input* input, void* node_se3_out,
+                               uint32_t* device_error, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

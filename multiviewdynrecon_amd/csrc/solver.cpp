@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -953,3 +954,94 @@ mvdr_status mvdr_batch_solve(mvdr_batch* bt, void* stream) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------ allocation-free solve
+// mvdr_solve_noalloc: SolveNonRigidAlignment for callers that cannot allocate
+// device memory.  Every per-frame buffer of the config is carved from one
+// file-scope device array of MVDR_NOALLOC_SCRATCH_BYTES, by the same layout the
+// handle-based solver allocates; the kernel sequence is the handle's.  Each
+// device holds its own copy of the array (until the process ends); its address
+// there is taken once per device with hipGetSymbolAddress and used for every
+// launch.  Calls that overlap on one device share the array and race.
+namespace {
+
+constexpr size_t kScratchBytes = (size_t)MVDR_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kScratchDevices] = {};  // device address of `scratch` per device
+
+// The scratch array's device address on the current device (taken on the
+// first call on that device, with the device current).
+hipError_t scratch_address(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+// Device copies of the frame table and the buffer table (passed by value, so
+// the launch carries them), and the zeroed counters and partial sums the
+// handle-based solver clears at create.
+__global__ void __launch_bounds__(256) noalloc_setup_kernel(FrameDev f, Buffers b, FrameDev* fd, Buffers* bd) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *fd = f;
+        *bd = b;
+        *b.counts = Counts{};
+    }
+    const size_t n = (size_t)kPartialSlots * kPartialStride;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b.partial[i] = 0.0;
+}
+
+// The solved warp field into the caller's array; a frame whose solve hit a
+// capacity error keeps its input warp (its kernels skipped their work).
+__global__ void __launch_bounds__(256) noalloc_finish_kernel(const Buffers* __restrict__ bs, uint32_t slot,
+                                                             DQ* __restrict__ out, uint32_t* __restrict__ error) {
+    const Buffers b = *bs;
+    const FrameDev& f = *b.frame;
+    const uint32_t err = b.counts->error;
+    const DQ* src = err ? f.node_se3_in : b.se3[slot];
+    for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < f.N; n += gridDim.x * blockDim.x) out[n] = src[n];
+    if (error && blockIdx.x == 0 && threadIdx.x == 0) *error = err;
+}
+
+}  // namespace
+
+extern "C" mvdr_status mvdr_solve_noalloc(const mvdr_solver_config* config, const mvdr_frame_input* input,
+                                          void* node_se3_out, uint32_t* device_error, void* stream) {
+    if (!config || !input || !node_se3_out) return MVDR_E_INVALID;
+    const mvdr_solver_config& c = *config;
+    if (!config_shape_ok(c)) return MVDR_E_SHAPE;
+    Buffers b;
+    FrameDev* fd = nullptr;
+    Buffers* bd = nullptr;
+    uint32_t tiles = 0, chunks = 0;
+    if (layout_buffers(c, b, fd, bd, tiles, chunks, nullptr) > kScratchBytes) return MVDR_E_CAPACITY;
+    FrameDev f;
+    const char* err = nullptr;
+    const mvdr_status rc = frame_from_input(c, input, f, err);
+    if (rc != MVDR_OK) return rc;
+    void* base = nullptr;
+    if (scratch_address(&base) != hipSuccess) return MVDR_E_HIP;
+    layout_buffers(c, b, fd, bd, tiles, chunks, static_cast<char*>(base));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(noalloc_setup_kernel, dim3(64), dim3(256), 0, st, f, b, fd, bd);
+    const Batch B{bd, 1, tiles, live_nodes(f.N), chunks};
+    enqueue_batch(B, c, st, true, 0, c.gn_iterations);
+    hipLaunchKernelGGL(noalloc_finish_kernel, dim3(64), dim3(256), 0, st, bd, c.gn_iterations & 1,
+                       static_cast<DQ*>(node_se3_out), device_error);
+    return hipGetLastError() == hipSuccess ? MVDR_OK : MVDR_E_HIP;
+}
+

@@ -71,6 +71,8 @@ def lib():
         L.mvdr_batch_solve.argtypes = [C.c_void_p, C.c_void_p]
         L.mvdr_batch_bench_stage.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_void_p, C.POINTER(C.c_float)]
         L.mvdr_solver_bench_stage.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_void_p, C.POINTER(C.c_float)]
+        L.mvdr_solve_noalloc.argtypes = [C.POINTER(_abi.SolverConfig), C.POINTER(_abi.FrameInput), C.c_void_p,
+                                         C.c_void_p, C.c_void_p]
         _LIB = L
     return _LIB
 

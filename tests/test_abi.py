@@ -135,6 +135,20 @@ def test_create_rejects_bad_config_without_gpu(built_libs):
     assert lib().mvdr_solver_create(None, None) == 1  # MVDR_E_INVALID
 
 
+def test_noalloc_rejects_oversized_config_without_gpu(built_libs):
+    # the scratch-size check precedes any device work
+    from multiviewdynrecon_amd.frame import HostFrame
+    from multiviewdynrecon_amd.solver import lib
+    from tiny_scene import make_tiny_scene
+
+    frame = HostFrame(make_tiny_scene(1, views=1))
+    cfg = config.workload_config(1, precision=1, refine_steps=1)  # 3 x 640 x 480 x 8192 nodes: too large
+    assert lib().mvdr_solve_noalloc(C.byref(cfg), C.byref(frame.struct), C.c_void_p(8), None, None) == 3
+    assert lib().mvdr_solve_noalloc(None, C.byref(frame.struct), C.c_void_p(8), None, None) == 1
+    bad = config.default_config(pcg_iterations=65)
+    assert lib().mvdr_solve_noalloc(C.byref(bad), C.byref(frame.struct), C.c_void_p(8), None, None) == 2
+
+
 def test_status_strings(built_libs):
     from multiviewdynrecon_amd.solver import lib
 
