@@ -111,3 +111,25 @@ class CrossViewInput(C.Structure):
 
 
 CROSSVIEW_LISTS = {"candidates": 0, "unique": 1, "backtraced": 2}
+
+
+class RenderInput(C.Structure):
+    _fields_ = [("num_views", u32), ("width", u32), ("height", u32), ("num_surfels", u32),
+                ("intrinsic", Intrinsic * V), ("world2camera", Mat34 * V), ("reference_vertex", vp * V),
+                ("reference_normal", vp * V), ("warp_vertex", vp * V), ("warp_normal", vp * V), ("color_time", vp),
+                ("max_depth", f32), ("current_time", f32), ("confidence_threshold", f32), ("time_threshold", f32)]
+
+
+class RenderOutput(C.Structure):
+    _fields_ = [("reference_vertex", vp * V), ("reference_normal", vp * V), ("warp_vertex", vp * V),
+                ("warp_normal", vp * V), ("index_map", vp * V), ("normalized_rgb", vp * V), ("zbuffer", vp * V)]
+
+
+class RigidConfig(C.Structure):
+    _fields_ = [("max_views", u32), ("max_width", u32), ("max_height", u32), ("reference_reduction", u32)]
+
+
+class RigidInput(C.Structure):
+    _fields_ = [("num_views", u32), ("width", u32), ("height", u32), ("intrinsic", Intrinsic * V),
+                ("reference_vertex", vp * V), ("reference_normal", vp * V), ("conversion_vertex", vp * V),
+                ("conversion_normal", vp * V), ("world2camera", Mat34 * V), ("relative_pose", Mat34 * V)]

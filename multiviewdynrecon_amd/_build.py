@@ -23,6 +23,7 @@ ARCH = os.environ.get("MVDR_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-Wall", "-Wno-unused-function", "-I", INCLUDE]
 HIP_SOURCES = ["kernels_index.hip", "kernels_solve.hip", "kernels_pcg.hip", "kernels_mf.hip", "crossview.hip",
+               "warpfield.hip", "render.hip", "rigid.hip",
                "solver.cpp"]
 
 

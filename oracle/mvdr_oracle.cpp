@@ -27,6 +27,8 @@
 //   warp update ..................... Rodrigues closed form
 //   cross-view term ................. parity unpinned (reference implementation absent)
 #include "../include/mvdr/crossview.h"
+#include "../include/mvdr/render.h"
+#include "../include/mvdr/rigid.h"
 #include "../include/mvdr/solver.h"
 
 #include <algorithm>
@@ -41,6 +43,10 @@
 namespace {
 
 thread_local std::string g_err;
+
+struct float4_t {
+    float x, y, z, w;
+};
 
 template <typename R>
 struct V3 {
@@ -1508,6 +1514,249 @@ long long mvdr_oracle_crossview(const mvdr_crossview_config* cfg, const mvdr_cro
         ++n;
     }
     return n;
+}
+
+// ---------------------------------------------------------- warp field
+// BuildNodesGraphKernel (WarpField.cu:15-59): 8 slots, a closer node (squared
+// distance in (1e-6, worst)) replaces the worst slot, the new worst is the
+// first strict maximum; edges (i, slot) in slot order, unfilled = 0xFFFF.
+void mvdr_oracle_node_graph(const float* nodes, uint32_t n, uint16_t* graph) {
+    for (uint32_t i = 0; i < n; ++i) {
+        float dist[8];
+        int idx[8];
+        for (int k = 0; k < 8; ++k) { dist[k] = 1e5f; idx[k] = -1; }
+        int worst = 0;
+        const float* a = nodes + 4 * (size_t)i;
+        for (uint32_t k = 0; k < n; ++k) {
+            const float* b = nodes + 4 * (size_t)k;
+            const float d = (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]);
+            if (d > 1e-6f && d < dist[worst]) {
+                dist[worst] = d;
+                idx[worst] = (int)k;
+                worst = 0;
+                float mx = 0.f;
+                for (int j = 0; j < 8; ++j)
+                    if (dist[j] > mx) { worst = j; mx = dist[j]; }
+            }
+        }
+        for (int k = 0; k < 8; ++k) {
+            graph[2 * (8 * (size_t)i + k)] = (uint16_t)i;
+            graph[2 * (8 * (size_t)i + k) + 1] = (uint16_t)idx[k];
+        }
+    }
+}
+
+// skinningVertexAndNodeBruteForceKernel (CanonicalNodesSkinner.cu:356-437) with
+// the 4-slot heap of brute_foce_knn.cuh:17-39 over the real nodes; points are
+// the vertices, then the nodes.
+void mvdr_oracle_skin(const float* points, uint32_t num_points, const float* nodes, uint32_t n, float radius,
+                      uint16_t* knn, float* weight) {
+    for (uint32_t p = 0; p < num_points; ++p) {
+        const float* v = points + 4 * (size_t)p;
+        float d[4] = {1e6f, 1e6f, 1e6f, 1e6f};
+        uint32_t id[4] = {0, 0, 0, 0};
+        for (uint32_t k = 0; k < n; ++k) {
+            const float* q = nodes + 4 * (size_t)k;
+            const float dx = v[0] - q[0], dy = v[1] - q[1], dz = v[2] - q[2];
+            const float dist = std::fma(dz, dz, std::fma(dy, dy, dx * dx));
+            if (!(dist < d[0])) continue;
+            d[0] = dist; id[0] = k;
+            if (d[1] < d[2]) {
+                if (d[0] < d[2]) { std::swap(d[0], d[2]); std::swap(id[0], id[2]); }
+            } else if (d[0] < d[1]) {
+                std::swap(d[0], d[1]); std::swap(id[0], id[1]);
+                if (d[1] < d[3]) { std::swap(d[1], d[3]); std::swap(id[1], id[3]); }
+            }
+        }
+        float w[4];
+        const float den = (2.0f * radius) * radius;
+        for (int s = 0; s < 4; ++s) {
+            const float* q = nodes + 4 * (size_t)id[s];
+            const float dx = v[0] - q[0], dy = v[1] - q[1], dz = v[2] - q[2];
+            w[s] = std::exp(-(dx * dx + dy * dy + dz * dz) / den);
+        }
+        const float inv = 1.0f / (std::fabs(w[0]) + std::fabs(w[1]) + std::fabs(w[2]) + std::fabs(w[3]));
+        for (int s = 0; s < 4; ++s) {
+            knn[4 * (size_t)p + s] = (uint16_t)id[s];
+            weight[4 * (size_t)p + s] = w[s] * inv;
+        }
+    }
+}
+
+// forwardWarpVertexAndNodeKernel (SurfelNodeDeformer.cu:9-60) for one point
+// set: v' = T v (w kept), n' = R n, T = se3_matrix(blend(knn, w)) in FP32.
+// normals may be null (nodes).  knn naming a node >= n copies the point.
+void mvdr_oracle_warp(const float* field_dq, uint32_t n, const float* points, const float* normals,
+                      const uint16_t* knn, const float* weight, uint32_t count, int inverse, float* out_points,
+                      float* out_normals) {
+    std::vector<DQ<float>> field(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        mvdr_dq d;
+        std::memcpy(&d, field_dq + 8 * (size_t)i, sizeof(d));
+        field[i] = DQ<float>::from(d);
+    }
+    for (uint32_t p = 0; p < count; ++p) {
+        const float* v = points + 4 * (size_t)p;
+        const uint16_t* k = knn + 4 * (size_t)p;
+        float* ov = out_points + 4 * (size_t)p;
+        std::memcpy(ov, v, 16);
+        if (normals) std::memcpy(out_normals + 4 * (size_t)p, normals + 4 * (size_t)p, 16);
+        if (k[0] >= n || k[1] >= n || k[2] >= n || k[3] >= n) continue;
+        const M34<float> T = blend(field, k, weight + 4 * (size_t)p).se3_matrix();
+        V3<float> a = v3<float>(v[0], v[1], v[2]);
+        a = inverse ? T.rot_t(a - T.t) : T.apply(a);
+        ov[0] = a.x; ov[1] = a.y; ov[2] = a.z;
+        if (normals) {
+            const float* nn = normals + 4 * (size_t)p;
+            V3<float> b = v3<float>(nn[0], nn[1], nn[2]);
+            b = inverse ? T.rot_t(b) : T.rot(b);
+            float* on = out_normals + 4 * (size_t)p;
+            on[0] = b.x; on[1] = b.y; on[2] = b.z;
+        }
+    }
+}
+
+// ------------------------------------------------------- solver maps
+// The point program of render/shaders/SolverMap.vert/.frag as a z-buffer over
+// surfels in draw order (rules in include/mvdr/render.h): a surfel passing the
+// depth-range and confidence/time tests lands on pixel (floor(u), floor(v)) and
+// replaces the pixel's surfel when strictly nearer.  Host arrays; the output
+// struct's arrays are written (zbuffer unused).
+int mvdr_oracle_render(const mvdr_render_input* in, const mvdr_render_output* out) {
+    const uint32_t W = in->width, H = in->height, S = in->num_surfels;
+    const float4_t* ct = static_cast<const float4_t*>(in->color_time);
+    for (uint32_t v = 0; v < in->num_views; ++v) {
+        std::vector<float> depth((size_t)W * H, INFINITY);
+        std::vector<uint32_t> idx((size_t)W * H, 0xFFFFFFFFu);
+        const M34<float> T = M34<float>::from(in->world2camera[v]);
+        const mvdr_intrinsic K = in->intrinsic[v];
+        const float4_t* rv = static_cast<const float4_t*>(in->reference_vertex[v]);
+        const float4_t* wv = static_cast<const float4_t*>(in->warp_vertex[v]);
+        for (uint32_t s = 0; s < S; ++s) {
+            const V3<float> p = T.apply(v3<float>(wv[s].x, wv[s].y, wv[s].z));
+            if (p.z > in->max_depth + 0.05f || p.z < 0.f) continue;
+            if (rv[s].w < in->confidence_threshold && std::fabs(ct[s].z - in->current_time) >= in->time_threshold) continue;
+            const float u = (K.fx * p.x) / p.z + K.cx, w = (K.fy * p.y) / p.z + K.cy;
+            if (!(u >= 0.f && w >= 0.f && u < (float)W && w < (float)H)) continue;
+            const size_t o = (size_t)std::floor(w) * W + (size_t)std::floor(u);
+            if (p.z < depth[o]) { depth[o] = p.z; idx[o] = s; }
+        }
+        const float4_t* rn = static_cast<const float4_t*>(in->reference_normal[v]);
+        const float4_t* wn = static_cast<const float4_t*>(in->warp_normal[v]);
+        for (size_t o = 0; o < (size_t)W * H; ++o) {
+            float4_t a{0, 0, 0, 0}, b{0, 0, 0, 0}, c{0, 0, 0, 0}, d{0, 0, 0, 0}, rgb{0, 0, 0, 0};
+            const uint32_t s = idx[o];
+            if (s != 0xFFFFFFFFu) {
+                a = {rv[s].x, rv[s].y, rv[s].z, 1.f};
+                b = {rn[s].x, rn[s].y, rn[s].z, 0.f};
+                c = {wv[s].x, wv[s].y, wv[s].z, 1.f};
+                d = {wn[s].x, wn[s].y, wn[s].z, 0.f};
+                int32_t bits;
+                std::memcpy(&bits, &ct[s].x, 4);
+                rgb = {(float)((bits & 0x00FF0000) >> 16) / 255.f, (float)((bits & 0x0000FF00) >> 8) / 255.f,
+                       (float)(bits & 0x000000FF) / 255.f, ct[s].y};
+            }
+            static_cast<float4_t*>(out->reference_vertex[v])[o] = a;
+            static_cast<float4_t*>(out->reference_normal[v])[o] = b;
+            static_cast<float4_t*>(out->warp_vertex[v])[o] = c;
+            if (out->warp_normal[v]) static_cast<float4_t*>(out->warp_normal[v])[o] = d;
+            static_cast<uint32_t*>(out->index_map[v])[o] = s;
+            if (out->normalized_rgb[v]) static_cast<float4_t*>(out->normalized_rgb[v])[o] = rgb;
+        }
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------- rigid ICP
+// SolveRigidAlignment(solverMaps, observation, LiveField2ObservedField, n)
+// (RigidSolver.cpp:258-276, rows RigidSolver.cu:128-226, update
+// RigidSolver.cpp:389-446) with the rules of include/mvdr/rigid.h: FP64 sums
+// over all pixels (or every 32nd 256-pixel block, times 32, with
+// reference_reduction), Cholesky, twist -> Rodrigues transform on the left.
+// Host maps; `poses` holds the start poses on entry and the result on exit.
+// The optional `system` receives the last iteration's 27 sums of view 0.
+int mvdr_oracle_rigid(const mvdr_rigid_input* in, int reference_reduction, uint32_t iterations, mvdr_mat34* poses,
+                      double* system) {
+    const uint32_t W = in->width, H = in->height;
+    for (uint32_t v = 0; v < in->num_views; ++v) {
+        M34<float> T = M34<float>::from(poses[v]);
+        const float4_t* cv = static_cast<const float4_t*>(in->conversion_vertex[v]);
+        const float4_t* cn = static_cast<const float4_t*>(in->conversion_normal[v]);
+        const float4_t* rv = static_cast<const float4_t*>(in->reference_vertex[v]);
+        const float4_t* rn = static_cast<const float4_t*>(in->reference_normal[v]);
+        const mvdr_intrinsic K = in->intrinsic[v];
+        for (uint32_t it = 0; it < iterations; ++it) {
+            double S[27] = {0};
+            for (uint32_t pix = 0; pix < W * H; ++pix) {
+                if (reference_reduction && ((pix / 256) % 32) != 0) continue;
+                const V3<float> p = T.apply(v3<float>(cv[pix].x, cv[pix].y, cv[pix].z));
+                const V3<float> pn = T.rot(v3<float>(cn[pix].x, cn[pix].y, cn[pix].z));
+                const float u = (float)((double)p.x / ((double)p.z + 1e-10) * (double)K.fx + (double)K.cx);
+                const float w = (float)((double)p.y / ((double)p.z + 1e-10) * (double)K.fy + (double)K.cy);
+                auto px16 = [](float f) -> uint32_t {
+                    uint32_t r;
+                    if (!(f > 0.f)) r = 0u;
+                    else if (f >= 4294967040.f) r = 0xFFFFFFFFu;
+                    else r = (uint32_t)std::nearbyint(f);
+                    return r & 0xFFFFu;
+                };
+                const uint32_t x = px16(u), y = px16(w);
+                if (!(x < W && y < H)) continue;
+                const float4_t q = rv[y * W + x], qn = rn[y * W + x];
+                const float qa[4] = {q.x, q.y, q.z, q.w};
+                const float ndot = pn.x * qn.x + pn.y * qn.y + pn.z * qn.z;
+                const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                if (is_zero_vertex<float>(qa) || ndot < 0.8f || d2 > 0.03f * 0.03f) continue;
+                const V3<float> n = v3<float>(qn.x, qn.y, qn.z);
+                const V3<float> pxn = vcross(p, n);
+                const double b = -(double)(n.x * dx + n.y * dy + n.z * dz);
+                const double a[6] = {pxn.x, pxn.y, pxn.z, n.x, n.y, n.z};
+                int k = 0;
+                for (int i = 0; i < 6; ++i)
+                    for (int j = i; j < 6; ++j) S[k++] += a[i] * a[j];
+                for (int i = 0; i < 6; ++i) S[k++] += a[i] * b;
+            }
+            if (reference_reduction)
+                for (double& x : S) x *= 32.0;
+            if (system && v == 0) std::memcpy(system, S, sizeof(S));
+            double A[6][6], y[6];
+            int k = 0;
+            for (int i = 0; i < 6; ++i)
+                for (int j = i; j < 6; ++j) { A[i][j] = A[j][i] = S[k]; ++k; }
+            for (int i = 0; i < 6; ++i) y[i] = S[21 + i];
+            bool ok = true;
+            for (int j = 0; j < 6 && ok; ++j) {
+                double d = A[j][j];
+                for (int q2 = 0; q2 < j; ++q2) d -= A[j][q2] * A[j][q2];
+                if (!(d > 0.0)) { ok = false; break; }
+                A[j][j] = std::sqrt(d);
+                for (int i = j + 1; i < 6; ++i) {
+                    double t = A[i][j];
+                    for (int q2 = 0; q2 < j; ++q2) t -= A[i][q2] * A[j][q2];
+                    A[i][j] = t / A[j][j];
+                }
+            }
+            if (!ok) continue;
+            for (int i = 0; i < 6; ++i) {
+                double t = y[i];
+                for (int q2 = 0; q2 < i; ++q2) t -= A[i][q2] * y[q2];
+                y[i] = t / A[i][i];
+            }
+            for (int i = 5; i >= 0; --i) {
+                double t = y[i];
+                for (int q2 = i + 1; q2 < 6; ++q2) t -= A[q2][i] * y[q2];
+                y[i] = t / A[i][i];
+            }
+            const M34<float> U = twist_to_se3(v3<float>((float)y[0], (float)y[1], (float)y[2]),
+                                              v3<float>((float)y[3], (float)y[4], (float)y[5]));
+            T = U.mul(T);
+        }
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) poses[v].rot[3 * c + r] = T.m(r, c);
+        poses[v].trans[0] = T.t.x; poses[v].trans[1] = T.t.y; poses[v].trans[2] = T.t.z;
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -39,6 +39,47 @@ def test_crossview_exports_every_declared_symbol(built_libs):
     assert not missing, missing
 
 
+def test_warpfield_exports_every_declared_symbol(built_libs):
+    from multiviewdynrecon_amd.solver import lib
+
+    L = lib()
+    names = declared("warpfield.h")
+    assert len(names) == 4
+    assert not [n for n in names if not hasattr(L, n)]
+
+
+def test_warpfield_rejects_bad_arguments_without_gpu(built_libs):
+    from multiviewdynrecon_amd.warpfield import lib
+
+    p = C.c_void_p(16)
+    assert lib().mvdr_warp_skin(p, 10, None, 5, 0.025, p, p, p, p, None) == 1
+    assert lib().mvdr_warp_skin(p, 10, p, 70000, 0.025, p, p, p, p, None) == 2
+    assert lib().mvdr_warp_build_node_graph(p, 70000, p, None) == 2
+    assert lib().mvdr_warp_inverse(p, 0, p, p, p, p, 10, p, p, None) == 1
+
+
+def test_render_exports_and_rejects_bad_shapes_without_gpu(built_libs):
+    from multiviewdynrecon_amd.render import lib
+
+    assert declared("render.h") == ["mvdr_render_solver_maps"]
+    ri, ro = _abi.RenderInput(), _abi.RenderOutput()
+    ri.num_views, ri.width, ri.height = 4, 8, 8
+    assert lib().mvdr_render_solver_maps(C.byref(ri), C.byref(ro), None) == 2
+    ri.num_views = 1
+    assert lib().mvdr_render_solver_maps(C.byref(ri), C.byref(ro), None) == 1  # output maps missing
+
+
+def test_rigid_exports_and_rejects_bad_config_without_gpu(built_libs):
+    from multiviewdynrecon_amd.rigid import lib
+
+    L = lib()
+    names = declared("rigid.h")
+    assert len(names) == 4 and not [n for n in names if not hasattr(L, n)]
+    for bad in ((4, 64, 64, 0), (1, 0, 64, 0), (1, 64, 64, 2)):
+        h = C.c_void_p()
+        assert L.mvdr_rigid_create(C.byref(_abi.RigidConfig(*bad)), C.byref(h)) == 2 and not h.value
+
+
 def test_crossview_create_rejects_bad_config_without_gpu(built_libs):
     from multiviewdynrecon_amd.crossview import default_config, lib
 
@@ -63,11 +104,15 @@ LAYOUT_C = r"""
 #include "mvdr/solver.h"
 #include "mvdr/scene.h"
 #include "mvdr/crossview.h"
+#include "mvdr/render.h"
+#include "mvdr/rigid.h"
 #define P(T) printf(#T " %zu\n", sizeof(T));
 #define O(T, f) printf(#T "." #f " %zu\n", offsetof(T, f));
 int main(void) {
   P(mvdr_intrinsic) P(mvdr_mat34) P(mvdr_dq) P(mvdr_cross_pair) P(mvdr_solver_config)
   P(mvdr_frame_input) P(mvdr_solver_sizes) P(mvdr_scene_params) P(mvdr_scene_info)
+  P(mvdr_rigid_config) P(mvdr_rigid_input) O(mvdr_rigid_input, relative_pose)
+  P(mvdr_render_input) P(mvdr_render_output) O(mvdr_render_input, time_threshold) O(mvdr_render_output, zbuffer)
   P(mvdr_crossview_config) P(mvdr_crossview_input) O(mvdr_crossview_input, corr_map) O(mvdr_crossview_config, max_distance2)
   O(mvdr_frame_input, intrinsic) O(mvdr_frame_input, world2camera) O(mvdr_frame_input, vertex_confid)
   O(mvdr_frame_input, surfel_knn) O(mvdr_frame_input, node_se3) O(mvdr_frame_input, feature_pairs)
@@ -91,6 +136,8 @@ def test_ctypes_layout_matches_c_compiler():
         "mvdr_frame_input": C.sizeof(_abi.FrameInput), "mvdr_solver_sizes": C.sizeof(_abi.SolverSizes),
         "mvdr_scene_params": C.sizeof(_abi.SceneParams), "mvdr_scene_info": C.sizeof(_abi.SceneInfo),
         "mvdr_crossview_config": C.sizeof(_abi.CrossViewConfig), "mvdr_crossview_input": C.sizeof(_abi.CrossViewInput),
+        "mvdr_render_input": C.sizeof(_abi.RenderInput), "mvdr_render_output": C.sizeof(_abi.RenderOutput),
+        "mvdr_rigid_config": C.sizeof(_abi.RigidConfig), "mvdr_rigid_input": C.sizeof(_abi.RigidInput),
     }
     for k, v in py.items():
         assert int(c_vals[k]) == v, k
@@ -107,6 +154,9 @@ def test_ctypes_layout_matches_c_compiler():
         "mvdr_scene_params.feature_jitter_px": _abi.SceneParams.feature_jitter_px.offset,
         "mvdr_crossview_input.corr_map": _abi.CrossViewInput.corr_map.offset,
         "mvdr_crossview_config.max_distance2": _abi.CrossViewConfig.max_distance2.offset,
+        "mvdr_render_input.time_threshold": _abi.RenderInput.time_threshold.offset,
+        "mvdr_render_output.zbuffer": _abi.RenderOutput.zbuffer.offset,
+        "mvdr_rigid_input.relative_pose": _abi.RigidInput.relative_pose.offset,
     }
     for k, v in offs.items():
         assert int(c_vals[k]) == v, k
