@@ -2,13 +2,16 @@
 // indices and the block-row structure of J^T J.  Every kernel reads its sizes
 // from device memory (FrameDev / Counts), so the whole chain runs without a
 // host round trip and can be captured into a hipGraph once per solver.
+//
+// Selection decisions (window searches, distance thresholds, the camera
+// transforms feeding them) are evaluated without FMA contraction so that they
+// round exactly like the same expressions on the host.  The pragma precedes
+// the includes: the transforms are inline functions of common.h, and a
+// contraction state set after their definitions would not reach them.
+#pragma clang fp contract(off)
+
 #include "common.h"
 #include "device_util.h"
-
-// Selection decisions (window searches, distance thresholds) are evaluated
-// without FMA contraction so that they round exactly like the same
-// expressions on the host.
-#pragma clang fp contract(off)
 
 namespace mvdr {
 
