@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage-reps", type=int, default=50)
     ap.add_argument("--batch", type=int, default=32, help="frames solved per step (one launch per stage)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the extra legs of the default run: the uneven 16-scene batch and the matrix-free "
+                         "8k-node workload")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first: the known-size kernel that "
                          "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -232,11 +235,11 @@ def measured_traffic(kernel, workload, frames):
     return None
 
 
-def bench_uneven(args):
+def uneven_measure(batches, warmup, graph=1, solo_reps=5):
     """16 independent scenes, N ~ U[1000, 8000] nodes and 1-3 views each, solved
     as one batch (mixed precision, one hipGraph per batch): aggregate frames/s
-    over >= 100 batches, and the per-scene latency (a scene completes with its
-    batch) at p50 / p99; each scene solved alone is reported beside it."""
+    over `batches` batches, the per-scene latency (a scene completes with its
+    batch) at p50 / p99, and each scene solved alone."""
     import torch
 
     from multiviewdynrecon_amd.config import workload_config
@@ -246,10 +249,10 @@ def bench_uneven(args):
 
     count = 16
     params = uneven_scene_params(count)
-    nodes = np.array([p["target_nodes"] for p in params])
-    views = np.array([p["num_views"] for p in params])
+    nodes = [p["target_nodes"] for p in params]
+    views = [p["num_views"] for p in params]
     scenes = make_uneven_scenes(count)
-    cfg = workload_config(4, use_graph=args.graph)
+    cfg = workload_config(4, use_graph=graph)
     frames = [DeviceFrame(sc) for sc in scenes]
     solvers = []
     for fr in frames:
@@ -257,12 +260,10 @@ def bench_uneven(args):
         s.SetSolverInput(fr)
         solvers.append(s)
     batch = NonRigidBatch(solvers)
-    print(f"[bench] {count} scenes resident: nodes {sorted(int(n) for n in nodes)}, views {views.tolist()}",
-          file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
+    print(f"[bench] {count} uneven scenes resident: nodes {sorted(nodes)}, views {views}", file=sys.stderr, flush=True)
+    for _ in range(warmup):
         batch.SolveNonRigidAlignment()
     torch.cuda.synchronize()
-    batches = max(args.steps, 100)
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(batches)]
     t0 = time.perf_counter()
     for e0, e1 in ev:
@@ -272,31 +273,66 @@ def bench_uneven(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     lat = np.array([a.elapsed_time(b) for a, b in ev])
-    fps = count * batches / wall
-    solo = []
-    for s in solvers:
-        for _ in range(2):
-            s.SolveNonRigidAlignment()
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        for _ in range(5):
-            s.SolveNonRigidAlignment()
-        torch.cuda.synchronize()
-        solo.append((time.perf_counter() - t) * 1e3 / 5)
-    print(f"[bench] {batches} batches of {count}: {wall * 1e3 / batches:.3f} ms/batch, {fps:.1f} frames/s; "
-          f"latency p50 {np.percentile(lat, 50):.3f} p99 {np.percentile(lat, 99):.3f} ms", file=sys.stderr, flush=True)
+    solo = [time_solves(s, solo_reps, 2) for s in solvers]
+    out = {"frames_per_s": round(count * batches / wall, 2), "batches": batches, "ms_per_batch": round(wall * 1e3 / batches, 4),
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
+           "solo_latency_ms": {"p50": round(float(np.percentile(solo, 50)), 4),
+                               "p99": round(float(np.percentile(solo, 99)), 4)},
+           "nodes": nodes, "views": views, "gn_iterations": cfg.gn_iterations, "pcg_iterations": cfg.pcg_iterations}
+    print(f"[bench] uneven batch: {out['frames_per_s']} frames/s, latency p50 {out['latency_ms']['p50']} "
+          f"p99 {out['latency_ms']['p99']} ms", file=sys.stderr, flush=True)
+    del batch, solvers, frames
+    return out
+
+
+def bench_uneven(args):
+    """The uneven 16-scene batch as the benchmark line (aggregate frames/s)."""
+    m = uneven_measure(max(args.steps, 100), args.warmup, args.graph)
     out = {
         "metric": "aggregate frames/s, 16 uneven scenes per batch (1k-8k nodes, 1-3 views)",
-        "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1, "steps": batches, "warmup": args.warmup,
-        "ms_per_step": round(wall * 1e3 / batches, 4), "higher_is_better": True, "scaling": "weak",
+        "value": m["frames_per_s"], "unit": "frames/s", "n_gpus": 1, "steps": m["batches"], "warmup": args.warmup,
+        "ms_per_step": m["ms_per_batch"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 (f64 reductions / 6x6 factors)", "data": "synthetic (seeded scene generator)",
-        "config": {"workload": WORKLOAD_NAMES[4], "scenes": count, "nodes": [int(n) for n in nodes],
-                   "views": views.tolist(), "gn_iterations": cfg.gn_iterations, "pcg_iterations": cfg.pcg_iterations,
-                   "precision": "mixed", "hip_graph": bool(args.graph)},
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
-        "solo_latency_ms": {"p50": round(float(np.percentile(solo, 50)), 4), "p99": round(float(np.percentile(solo, 99)), 4)},
+        "config": {"workload": WORKLOAD_NAMES[4], "scenes": 16, "nodes": m["nodes"], "views": m["views"],
+                   "gn_iterations": m["gn_iterations"], "pcg_iterations": m["pcg_iterations"], "precision": "mixed",
+                   "hip_graph": bool(args.graph)},
+        "latency_ms": m["latency_ms"], "solo_latency_ms": m["solo_latency_ms"],
     }
     print(json.dumps(out), flush=True)
+
+
+def matrix_free_measure(frames, steps):
+    """The ~8k-node, 15-PCG matrix-free workload, mixed precision: ms per frame of
+    a batched step of `frames` frames, and one frame alone."""
+    import torch
+
+    from multiviewdynrecon_amd.config import workload_config
+    from multiviewdynrecon_amd.frame import DeviceFrame
+    from multiviewdynrecon_amd.scene import make_scene
+    from multiviewdynrecon_amd.solver import NonRigidBatch, NonRigidSolver
+
+    cfg = workload_config(3, precision=1, refine_steps=1, use_graph=1)
+    scenes = [make_scene(3)] + [make_scene(3, seed=2000 + k) for k in range(1, frames)]
+    solvers = []
+    for sc in scenes:
+        s = NonRigidSolver(cfg)
+        s.SetSolverInput(DeviceFrame(sc))
+        solvers.append(s)
+    batch = NonRigidBatch(solvers)
+    batch.SolveNonRigidAlignment()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.SolveNonRigidAlignment()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    single = time_solves(solvers[0], steps, 1)
+    out = {"workload": WORKLOAD_NAMES[3], "frames_per_step": frames, "ms_per_frame": round(ms / frames, 4),
+           "latency_ms_single_frame": round(single, 4), "nodes": scenes[0].num_nodes,
+           "surfels": scenes[0].num_surfels, "gn_iterations": cfg.gn_iterations, "pcg_iterations": cfg.pcg_iterations}
+    print(f"[bench] matrix-free 8k: {out['ms_per_frame']} ms/frame in batches of {frames}, "
+          f"{out['latency_ms_single_frame']} ms alone", file=sys.stderr, flush=True)
+    return out
 
 
 def main():
@@ -460,6 +496,11 @@ def main():
         "plain_fp32": {"ms_per_frame": round(plain_ms, 4), "speedup": round(plain_ms / ms_per_frame, 3),
                        "twist_rel_err": max(plain_errs) if plain_errs else None},
     }
+    if args.workload == 1 and not args.no_extras:
+        # the other two solve workloads at their real sizes, beside the headline
+        del batch, solvers, frames
+        out["uneven_batch"] = uneven_measure(30, 3, args.graph, solo_reps=3)
+        out["matrix_free_8k"] = matrix_free_measure(4, 5)
     print(json.dumps(out), flush=True)
 
 
