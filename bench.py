@@ -408,7 +408,7 @@ def main():
     print(f"[bench] single-frame latency {latency_ms:.3f} ms", file=sys.stderr, flush=True)
 
     # per-stage kernel timing of the batched launches (hipEvents on the stream)
-    mf = cfg.jtj_mode == 1
+    mf = cfg.jtj_mode in (1, 2)
     if mf:
         per_solve = {"eval_terms": cfg.gn_iterations, "assemble": cfg.gn_iterations,
                      "spmv": cfg.gn_iterations * (cfg.pcg_iterations + 2 * cfg.refine_steps),

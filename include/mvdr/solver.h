@@ -67,7 +67,11 @@ typedef enum {
 
 typedef enum {
     MVDR_JTJ_MATERIALIZED = 0, /* USE_MATERIALIZED_JTJ (solver_constants.h:12)   */
-    MVDR_JTJ_MATRIX_FREE = 1   /* J^T(J x) with per-term adjoints                */
+    MVDR_JTJ_MATRIX_FREE = 1,  /* J^T(J x) with per-term adjoints                */
+    MVDR_JTJ_RECOMPUTE = 2     /* matrix-free with J re-evaluated from the maps and
+                                  the current warp inside every PCG apply (the
+                                  stored J feeds only the block-Jacobi
+                                  preconditioner and J^T r)                        */
 } mvdr_jtj_mode;
 
 typedef struct {
@@ -228,6 +232,22 @@ mvdr_status mvdr_solver_get_sizes(mvdr_solver* solver, mvdr_solver_sizes* sizes)
  * bytes is too small, *needed receives the required byte count and the call
  * returns MVDR_E_CAPACITY without copying (needed may be NULL). */
 mvdr_status mvdr_solver_debug_get(mvdr_solver* solver, int tap, void* dst, size_t bytes, size_t* needed);
+
+/* Debug, MVDR_JTJ_RECOMPUTE only: the term Jacobian at the current warp (the
+ * field after the GN iterations run so far), re-evaluated from the maps by the
+ * kernels of the recomputing matrix-free apply, with unit term weights (no
+ * penalty).  transpose 0: out = J in, with in = device float[6 * num_nodes]
+ * (reference twists: rotation xyz, translation xyz per node) and out = device
+ * float4[num_terms] (term t's rows: dense depth then density when
+ * density_in_jtj; the three components of an edge / feature / cross-view
+ * term; foreground one row; unused rows 0).  transpose 1: out = J^T in with
+ * the two shapes swapped -- the exact adjoint of transpose 0.  Terms in the
+ * global order of the term taps (dense, smooth, foreground, feature, cross).
+ * Overwrites the solve's PCG work vectors (SPMV taps); synchronises `stream`.
+ * No counterpart in the reference (its matrix-free path has no such hook);
+ * it exposes the J x and J^T y of ApplyJtJMatrixFreeIndexed.cu:29-349 for
+ * adjoint and finite-difference checks. */
+mvdr_status mvdr_solver_debug_jacobian(mvdr_solver* solver, int transpose, const void* in, void* out, void* stream);
 
 typedef enum {
     MVDR_STAGE_EVAL_TERMS = 0,     /* term residuals / Jacobians                      */

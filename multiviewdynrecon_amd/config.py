@@ -9,7 +9,7 @@ off, cross view on; USE_MATERIALIZED_JTJ) and the benchmark schedule (5 GN x
 from . import _abi
 
 PRECISION = {"fp32": 0, "mixed": 1}
-JTJ_MODE = {"materialized": 0, "matrix_free": 1}
+JTJ_MODE = {"materialized": 0, "matrix_free": 1, "recompute": 2}
 
 
 def default_config(**overrides):
@@ -49,7 +49,7 @@ def workload_config(workload, **overrides):
     base = {0: dict(gn_iterations=3, pcg_iterations=10, max_width=320, max_height=240, max_views=1),
             1: dict(gn_iterations=5, pcg_iterations=10),
             2: dict(gn_iterations=5, pcg_iterations=10, precision=1, refine_steps=1),
-            3: dict(gn_iterations=5, pcg_iterations=15, jtj_mode=1),
+            3: dict(gn_iterations=5, pcg_iterations=15, jtj_mode=2),
             4: dict(gn_iterations=5, pcg_iterations=10, precision=1, refine_steps=1, max_nodes=12288,
                     max_graph_edges=8 * 12288)}[workload]
     base.update(overrides)

@@ -338,6 +338,7 @@ struct Buffers {
     uint32_t* pair_edge;     // kPairEdges x cap_pairs: their edge indices
     uint16_t* row_part;      // N x kRowPartners: each row's partners ascending, upper then lower
     float4* mf_u;            // terms: u_t of the matrix-free apply
+    float4* mf_g;            // 2 per term: J_t^T u_t of the recomputing matrix-free apply
     // assembly partials of runs crossing 64-entry chunk boundaries
     float* part_diag;        // (kv1 / 64 + 1) x 2 x 27
     float* part_pair;        // (kv2 / 64 + 1) x 2 x 36

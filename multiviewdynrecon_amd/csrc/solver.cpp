@@ -22,8 +22,10 @@ void launch_pair_blocks(const Batch& B, const Penalty& pen, hipStream_t stream);
 void launch_diag_blocks(const Batch& B, hipStream_t stream);
 void launch_assemble(const Batch& B, const Penalty& pen, bool gram, bool diag_only, hipStream_t stream);
 void launch_pcg_matrix_free(const Batch& B, uint32_t iters, uint32_t refine, bool fp64, const Penalty& pen,
-                            hipStream_t stream);
-void launch_mf_apply_once(const Batch& B, bool fp64, const Penalty& pen, hipStream_t stream);
+                            bool recompute, uint32_t which, hipStream_t stream);
+void launch_mf_apply_once(const Batch& B, bool fp64, const Penalty& pen, bool recompute, uint32_t which,
+                          hipStream_t stream);
+void launch_debug_jacobian(const Batch& B, bool fp64, bool transpose, uint32_t which, hipStream_t stream);
 void launch_spmv_once(const Batch& B, bool fp64, hipStream_t stream);
 void launch_pcg_update_once(const Batch& B, bool fp64, hipStream_t stream);
 uint32_t launch_cost(const Batch& B, const Penalty& pen, bool density_in_jtj, hipStream_t stream);
@@ -135,12 +137,15 @@ const char* error_text(uint32_t bits) {
     return "capacity exceeded";
 }
 
+// J^T J applied matrix-free (stored or recomputed Jacobians).
+bool matrix_free(const mvdr_solver_config& c) { return c.jtj_mode != MVDR_JTJ_MATERIALIZED; }
+
 // Index build and/or GN iterations [first_iter, first_iter + iters) of every
 // frame of the batch B, all sharing the schedule of config c.
 void enqueue_batch(const Batch& B, const mvdr_solver_config& c, hipStream_t stream, bool index, uint32_t first_iter,
                    uint32_t iters) {
     const bool fp64 = c.precision == MVDR_PRECISION_MIXED;
-    const bool mf = c.jtj_mode == MVDR_JTJ_MATRIX_FREE;
+    const bool mf = matrix_free(c);
     if (index) {
         launch_copy_se3(B, stream);
         launch_index_build(B, c.use_foreground != 0, c.use_cross_view != 0,
@@ -152,7 +157,8 @@ void enqueue_batch(const Batch& B, const mvdr_solver_config& c, hipStream_t stre
         if (mf) {
             // diagonal blocks and J^T r from the row Gram assembly; J^T J applied matrix-free
             launch_assemble(B, pen, true, true, stream);
-            launch_pcg_matrix_free(B, c.pcg_iterations, c.refine_steps, fp64, pen, stream);
+            launch_pcg_matrix_free(B, c.pcg_iterations, c.refine_steps, fp64, pen, c.jtj_mode == MVDR_JTJ_RECOMPUTE,
+                                   k & 1, stream);
             launch_apply_update(B, k, stream);
             continue;
         }
@@ -172,12 +178,12 @@ uint32_t live_nodes(uint32_t n) { return std::max<uint32_t>(64, (n + 63) / 64 * 
 // After an assembly: the mixed mode's fused CG leaves S_ij in offdiag and the
 // Gram sums in diag (no FP32 inverses).
 bool blocks_after_assembly(const mvdr_solver_config& c) {
-    return !(c.precision == MVDR_PRECISION_MIXED && c.jtj_mode != MVDR_JTJ_MATRIX_FREE);
+    return !(c.precision == MVDR_PRECISION_MIXED && !matrix_free(c));
 }
 
 void enqueue_solve(mvdr_solver* s, hipStream_t stream, bool index, uint32_t first_iter, uint32_t iters) {
     s->self.nodes = live_nodes(s->frame_host.N);
-    if (index) s->pair_terms = s->cfg.precision != MVDR_PRECISION_MIXED && s->cfg.jtj_mode != MVDR_JTJ_MATRIX_FREE;
+    if (index) s->pair_terms = s->cfg.precision != MVDR_PRECISION_MIXED && !matrix_free(s->cfg);
     enqueue_batch(s->self, s->cfg, stream, index, first_iter, iters);
 }
 
@@ -321,7 +327,7 @@ const char* mvdr_solver_last_error(const mvdr_solver* s) { return s ? s->error.c
 // Shape checks of a config that precede any device work.
 static bool config_shape_ok(const mvdr_solver_config& c) {
     if (c.max_views < 1 || c.max_views > MVDR_MAX_VIEWS || c.max_width < 1 || c.max_height < 1 || c.max_width > 65535 ||
-        c.max_height > 65535 || c.max_nodes < 1 || c.max_nodes > 65535 || c.precision > 1 || c.jtj_mode > 1 ||
+        c.max_height > 65535 || c.max_nodes < 1 || c.max_nodes > 65535 || c.precision > 1 || c.jtj_mode > 2 ||
         c.pcg_iterations > MVDR_MAX_PCG_ITERATIONS)
         return false;
     const uint64_t pixels = (uint64_t)c.max_views * c.max_width * c.max_height;
@@ -395,9 +401,10 @@ static size_t layout_buffers(const mvdr_solver_config& c, Buffers& b, FrameDev*&
     A(b.pair_range, b.cap_pairs);
     A(b.pair_terms, b.cap_kv2);
     A(b.kv2_pair, b.cap_kv2);
-    const bool mf = c.jtj_mode == MVDR_JTJ_MATRIX_FREE;
+    const bool mf = matrix_free(c);
     if (mf) {
         A(b.mf_u, knn_terms + b.cap_edges);
+        if (c.jtj_mode == MVDR_JTJ_RECOMPUTE) A(b.mf_g, 2 * (knn_terms + b.cap_edges));
     } else if (c.precision == MVDR_PRECISION_MIXED) {
         A(b.n2t_col, b.cap_kv1);
         A(b.row_part, (size_t)kRowPartners * c.max_nodes);
@@ -617,7 +624,7 @@ mvdr_status mvdr_solver_solve(mvdr_solver* s, void* stream) {
             if (e != hipSuccess) return fail(s, MVDR_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
         }
         HIP_TRY(s, hipGraphLaunch(s->graph_exec, st));
-        s->pair_terms = s->cfg.precision != MVDR_PRECISION_MIXED && s->cfg.jtj_mode != MVDR_JTJ_MATRIX_FREE;
+        s->pair_terms = s->cfg.precision != MVDR_PRECISION_MIXED && !matrix_free(s->cfg);
     } else {
         enqueue_solve(s, st, true, 0, gn);
         HIP_TRY(s, hipGetLastError());
@@ -695,7 +702,7 @@ mvdr_status mvdr_solver_debug_get(mvdr_solver* s, int tap, void* dst, size_t byt
     if (!s->has_input) return fail(s, MVDR_E_STATE, "no input");
     HIP_TRY(s, hipEventSynchronize(s->done));
     if ((tap == MVDR_TAP_NODEPAIR_RANGE || tap == MVDR_TAP_NODEPAIR_TERM_INDEX) && s->indexed && !s->pair_terms &&
-        s->cfg.jtj_mode != MVDR_JTJ_MATRIX_FREE) {
+        !matrix_free(s->cfg)) {
         // the row Gram assembly needs no NodePair2Term lists: build them for inspection
         s->self.nodes = live_nodes(s->frame_host.N);
         launch_pair_term_lists(s->self, nullptr);
@@ -771,13 +778,34 @@ mvdr_status mvdr_solver_debug_get(mvdr_solver* s, int tap, void* dst, size_t byt
     return MVDR_OK;
 }
 
+mvdr_status mvdr_solver_debug_jacobian(mvdr_solver* s, int transpose, const void* in, void* out, void* stream) {
+    if (!s || !in || !out || transpose < 0 || transpose > 1) return MVDR_E_INVALID;
+    if (s->cfg.jtj_mode != MVDR_JTJ_RECOMPUTE) return fail(s, MVDR_E_INVALID, "debug_jacobian needs jtj_mode RECOMPUTE");
+    if (!s->indexed) return fail(s, MVDR_E_STATE, "debug_jacobian needs a built index");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    HIP_TRY(s, hipEventSynchronize(s->done));
+    Counts c;
+    HIP_TRY(s, hipMemcpy(&c, s->b.counts, sizeof(Counts), hipMemcpyDeviceToHost));
+    if (c.error) return fail(s, MVDR_E_CAPACITY, error_text(c.error));
+    const size_t twists = 24ull * s->frame_host.N, rows = 16ull * c.off[4];
+    s->self.nodes = live_nodes(s->frame_host.N);
+    HIP_TRY(s, hipMemcpyAsync(transpose ? (void*)s->b.mf_u : (void*)s->b.d1, in, transpose ? rows : twists,
+                              hipMemcpyDeviceToDevice, st));
+    launch_debug_jacobian(s->self, s->cfg.precision == MVDR_PRECISION_MIXED, transpose != 0, s->iter & 1, st);
+    HIP_TRY(s, hipGetLastError());
+    HIP_TRY(s, hipMemcpyAsync(out, transpose ? (const void*)s->b.q : (const void*)s->b.mf_u, transpose ? twists : rows,
+                              hipMemcpyDeviceToDevice, st));
+    HIP_TRY(s, hipStreamSynchronize(st));
+    return MVDR_OK;
+}
+
 // Stage micro-benchmark over a batch (a solver is a batch of one): `reps`
 // launches of one stage on the data of GN iteration k, timed with hipEvents
 // on `st`.
 static mvdr_status bench_stage_impl(const Batch& B, const mvdr_solver_config& c, uint32_t k, int stage, uint32_t reps,
                                     hipStream_t st, float* avg_ms, std::string& err, bool& pair_blocks) {
     const bool fp64 = c.precision == MVDR_PRECISION_MIXED;
-    const bool mf = c.jtj_mode == MVDR_JTJ_MATRIX_FREE;
+    const bool mf = matrix_free(c);
     if (stage < MVDR_STAGE_EVAL_TERMS || stage > MVDR_STAGE_CG_ITERATION) {
         err = "unknown stage";
         return MVDR_E_INVALID;
@@ -802,7 +830,7 @@ static mvdr_status bench_stage_impl(const Batch& B, const mvdr_solver_config& c,
             case MVDR_STAGE_ASSEMBLE: launch_assemble(B, pen, fp64 || mf, mf, st); break;
             case MVDR_STAGE_PCG_UPDATE: launch_pcg_update_once(B, fp64, st); break;
             case MVDR_STAGE_SPMV:
-                if (mf) launch_mf_apply_once(B, fp64, pen, st);
+                if (mf) launch_mf_apply_once(B, fp64, pen, c.jtj_mode == MVDR_JTJ_RECOMPUTE, k & 1, st);
                 else launch_spmv_once(B, fp64, st);
                 break;
             case MVDR_STAGE_INDEX: enqueue_batch(B, c, st, true, 0, 0); break;
@@ -942,7 +970,7 @@ mvdr_status mvdr_batch_solve(mvdr_batch* bt, void* stream) {
     for (mvdr_solver* m : bt->members) {
         if (e == hipSuccess) e = hipEventRecord(m->done, st);
         m->indexed = true;
-        m->pair_terms = bt->cfg.precision != MVDR_PRECISION_MIXED && bt->cfg.jtj_mode != MVDR_JTJ_MATRIX_FREE;
+        m->pair_terms = bt->cfg.precision != MVDR_PRECISION_MIXED && !matrix_free(bt->cfg);
         m->iter = gn;
         m->pair_blocks = gn == 0 || blocks_after_assembly(bt->cfg);
     }

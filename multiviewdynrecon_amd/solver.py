@@ -73,6 +73,7 @@ def lib():
         L.mvdr_solver_bench_stage.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_void_p, C.POINTER(C.c_float)]
         L.mvdr_solve_noalloc.argtypes = [C.POINTER(_abi.SolverConfig), C.POINTER(_abi.FrameInput), C.c_void_p,
                                          C.c_void_p, C.c_void_p]
+        L.mvdr_solver_debug_jacobian.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -161,6 +162,26 @@ class NonRigidSolver:
         s = _abi.SolverSizes()
         self._check(lib().mvdr_solver_get_sizes(self.h, C.byref(s)))
         return {name: getattr(s, name) for name, _ in _abi.SolverSizes._fields_}
+
+    def debug_jacobian(self, values, transpose=False):
+        """The term Jacobian at the current warp, re-evaluated from the maps with
+        unit term weights (MVDR_JTJ_RECOMPUTE): J x for x of shape (nodes, 6)
+        (reference twists) -> (terms, 4) rows, or J^T y for y of shape
+        (terms, 4) -> (nodes, 6) with transpose=True."""
+        import torch
+
+        sz = self.sizes()
+        terms = sum(sz[k] for k in ("num_dense_terms", "num_smooth_terms", "num_foreground_terms",
+                                    "num_feature_terms", "num_cross_terms"))
+        shape_in, shape_out = ((terms, 4), (sz["num_nodes"], 6)) if transpose else ((sz["num_nodes"], 6), (terms, 4))
+        host = np.ascontiguousarray(values, dtype=np.float32)
+        if host.shape != shape_in:
+            raise ValueError(f"expected shape {shape_in}, got {host.shape}")
+        src = torch.from_numpy(host).to("cuda")
+        out = torch.empty(shape_out, dtype=torch.float32, device="cuda")
+        self._check(lib().mvdr_solver_debug_jacobian(self.h, int(bool(transpose)), C.c_void_p(src.data_ptr()),
+                                                      C.c_void_p(out.data_ptr()), _stream_handle(None)))
+        return out.cpu().numpy()
 
     def debug_get(self, tap):
         """Copy a stage buffer to a host numpy array (synchronises)."""

@@ -101,12 +101,13 @@ def test_matrix_free_gn_iterations_match_fp64_oracle(scene1):
 
 
 def test_matrix_free_scaled_body():
-    """~8k nodes / ~300k surfels, 5 GN x 15 PCG, matrix-free (plain FP32)."""
+    """~8k nodes / ~300k surfels, 5 GN x 15 PCG, matrix-free with recomputed
+    Jacobians (plain FP32)."""
     from multiviewdynrecon_amd.scene import make_scene
 
     scene = make_scene(3)
     cfg = workload_config(3)
-    assert cfg.jtj_mode == 1 and cfg.pcg_iterations == 15
+    assert cfg.jtj_mode == 2 and cfg.pcg_iterations == 15
     solver = NonRigidSolver(cfg)
     solver.SetSolverInput(DeviceFrame(scene))
     solver.BuildSolverIndex()

@@ -5,7 +5,8 @@
   member is bit-identical to the same scene solved alone, and the two largest
   members -- the largest overall (3 views) and the largest single-view scene --
   have per-GN twists within 1e-4 of the FP64 oracle from the same warp.
-* Scaled body, matrix-free (~8k nodes, ~300k surfels, 3 views, 5 GN x 15 PCG):
+* Scaled body, matrix-free (~8k nodes, ~300k surfels, 3 views, 5 GN x 15 PCG),
+  with the Jacobians recomputed inside every apply and with stored ones:
   per-GN twists of the matrix-free mixed solve within 1e-4 of the FP64 oracle
   linearised at its warp, and within 1e-4 of the materialized-J^T J solve run
   side by side from the same input (the two warps stay within rounding).
@@ -82,18 +83,21 @@ def scaled_body():
     return scene, DeviceFrame(scene)
 
 
-def test_scaled_body_matrix_free_against_oracle(scaled_body):
+@pytest.mark.parametrize("jtj_mode", [2, 1])
+def test_scaled_body_matrix_free_against_oracle(scaled_body, jtj_mode):
+    # 2: Jacobians recomputed inside every apply (the workload's form); 1: stored J
     scene, frame = scaled_body
-    cfg = workload_config(3, precision=1, refine_steps=1)
-    assert cfg.jtj_mode == 1 and cfg.pcg_iterations == 15 and scene.num_nodes > 7000
+    cfg = workload_config(3, precision=1, refine_steps=1, jtj_mode=jtj_mode)
+    assert workload_config(3).jtj_mode == 2 and cfg.pcg_iterations == 15 and scene.num_nodes > 7000
     errs = twist_errors_vs_oracle(cfg, scene, frame, cfg.gn_iterations)
-    print(f"scaled body, matrix-free mixed: twist rel err {errs}")
+    print(f"scaled body, matrix-free ({jtj_mode}) mixed: twist rel err {errs}")
     assert max(errs) < 1e-4, errs
 
 
-def test_scaled_body_matrix_free_equals_materialized(scaled_body):
+@pytest.mark.parametrize("jtj_mode", [2, 1])
+def test_scaled_body_matrix_free_equals_materialized(scaled_body, jtj_mode):
     _, frame = scaled_body
-    mf = NonRigidSolver(workload_config(3, precision=1, refine_steps=1))
+    mf = NonRigidSolver(workload_config(3, precision=1, refine_steps=1, jtj_mode=jtj_mode))
     mat = NonRigidSolver(workload_config(3, precision=1, refine_steps=1, jtj_mode=0))
     for s in (mf, mat):
         s.SetSolverInput(frame)
