@@ -15,7 +15,7 @@ namespace mvdr {
 
 constexpr int kMaxViews = MVDR_MAX_VIEWS;
 constexpr uint32_t kInvalidIndex = 0xFFFFFFFFu;
-constexpr uint32_t kPartialStride = 1024;  // per-block partial sums per reduction slot
+constexpr uint32_t kPartialStride = 1024;  // partial sums per reduction slot, at least (per-block sums)
 constexpr uint32_t kPartialSlots = 160;
 constexpr uint32_t kSlotCost = 150;        // partials of the total residual
 
@@ -367,7 +367,8 @@ struct Buffers {
     // set read by iteration i (cg_rws[i & 1]) and of the set it writes
     float *cg_rws[2], *cg_p, *cg_x, *cg_b;
     double* cg_scal;         // alpha_i, gamma_i per iteration
-    double* partial;         // kPartialSlots x kPartialStride per-block partial sums
+    double* partial;         // kPartialSlots x partial_stride partial sums (per block, or per 32-row CG group)
+    uint32_t partial_stride; // max(kPartialStride, CG row groups of max_nodes)
     // warp field double buffer
     DQ* se3[2];
 };

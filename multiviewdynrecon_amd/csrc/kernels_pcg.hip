@@ -256,6 +256,60 @@ __global__ void __launch_bounds__(256) refine_step_kernel(const Buffers* __restr
 constexpr uint32_t kSlotCgGamma = 0;   // gamma_i partials: slot i
 constexpr uint32_t kSlotCgDelta = 72;  // delta_i partials: slot 72 + i
 
+// Partial sums of the fused CG are per row group: the kHatRowsPerBlock block
+// rows a workgroup covers in one pass (rows base .. base + 31) sum their
+// values (a butterfly per wave, the four waves in order) and store them as
+// partial base / kHatRowsPerBlock.  A frame of N nodes has cg_partials(N) of
+// them whatever the grid, so its inner products, and its iterates, do not
+// depend on the workgroup count, which may then follow the batch size (one
+// pass of rows per workgroup for a lone frame, several for a full batch).
+// The four waves of a pass combine without a workgroup barrier: each stores
+// its sums in the pass's LDS slot and counts itself in (workgroup-scope
+// acquire-release on an LDS counter); the fourth to arrive adds the four in
+// wave order and stores the partial.  Slots are not reused within a launch,
+// so waves may drift apart by any number of passes.
+__host__ __device__ inline uint32_t cg_partials(uint32_t nodes) {
+    return (nodes + kHatRowsPerBlock - 1) / kHatRowsPerBlock;
+}
+constexpr uint32_t kCgMaxPasses = 64;  // passes of one workgroup: ceil(65535 / 32 / kXcds)
+struct GroupSums {
+    double part[kCgMaxPasses][2][4];  // [pass][value][wave]
+    uint32_t count[kCgMaxPasses];
+};
+// Zeroes the arrival counters; the caller's next workgroup barrier publishes them.
+__device__ inline void group_sums_init(GroupSums& sh) {
+    for (uint32_t q = threadIdx.x; q < kCgMaxPasses; q += blockDim.x) sh.count[q] = 0;
+}
+__device__ inline void group_partials2(const Buffers& b, GroupSums& sh, uint32_t pass, uint32_t slot0, double v0,
+                                       uint32_t slot1, double v1, uint32_t base) {
+    v0 = wave_sum(v0);
+    v1 = wave_sum(v1);
+    if ((threadIdx.x & 63) == 0) {
+        const uint32_t w = threadIdx.x >> 6;
+        sh.part[pass][0][w] = v0;
+        sh.part[pass][1][w] = v1;
+        const uint32_t n = __hip_atomic_fetch_add(&sh.count[pass], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (n == blockDim.x / 64 - 1) {
+            const double* p0 = sh.part[pass][0];
+            const double* p1 = sh.part[pass][1];
+            slot_ptr(b, slot0)[base / kHatRowsPerBlock] = p0[0] + p0[1] + p0[2] + p0[3];
+            slot_ptr(b, slot1)[base / kHatRowsPerBlock] = p1[0] + p1[1] + p1[2] + p1[3];
+        }
+    }
+}
+__device__ inline void group_partial(const Buffers& b, GroupSums& sh, uint32_t pass, uint32_t slot, double v,
+                                     uint32_t base) {
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) {
+        sh.part[pass][0][threadIdx.x >> 6] = v;
+        const uint32_t n = __hip_atomic_fetch_add(&sh.count[pass], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (n == blockDim.x / 64 - 1) {
+            const double* p0 = sh.part[pass][0];
+            slot_ptr(b, slot)[base / kHatRowsPerBlock] = p0[0] + p0[1] + p0[2] + p0[3];
+        }
+    }
+}
+
 // The transformed system.  The row Gram assembly leaves
 // S_ij = sum w_i w_j lambda^2 u_i^T u_i (21 values) per pair; here
 // B_ij = S_ij M_ij (the change of centre g_i -> g_j: rows e_c x (g_j - g_i) add
@@ -454,7 +508,10 @@ __global__ void __launch_bounds__(256) cg_init_kernel(const Buffers* __restrict_
     const Buffers b = frame_buffers_at(bs, wi.frame);
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kHatLanes;
-    double g = 0.0, d = 0.0;
+    __shared__ GroupSums sums;
+    group_sums_init(sums);
+    __syncthreads();
+    uint32_t pass = 0;
     if (!b.counts->error) {
         float* RWS = b.cg_rws[0];
         auto load = [&](uint32_t j, float v[6]) {
@@ -465,6 +522,7 @@ __global__ void __launch_bounds__(256) cg_init_kernel(const Buffers* __restrict_
         const uint32_t groups = gridDim.x * kHatRowsPerBlock;
         for (uint32_t base = wi.block * kHatRowsPerBlock; base < f.N; base += groups) {
             const uint32_t i = base + threadIdx.x / kHatLanes;
+            double g = 0.0, d = 0.0;
             // component a = sub of the row (lanes 0..5), its r_0 loaded before
             // the row apply
             const bool own = i < f.N && sub < 6;
@@ -483,9 +541,9 @@ __global__ void __launch_bounds__(256) cg_init_kernel(const Buffers* __restrict_
                 g += r * r;
                 d += w * r;
             }
+            group_partials2(b, sums, pass++, kSlotCgGamma + 0, g, kSlotCgDelta + 0, d, base);
         }
     }
-    write_partials2<double>(b, kSlotCgGamma + 0, g, kSlotCgDelta + 0, d, wi.block);
 }
 
 // alpha_i, beta_i from the partials of gamma_i, delta_i and the stored
@@ -519,9 +577,11 @@ __global__ void __launch_bounds__(256) cg_iter_kernel(const Buffers* __restrict_
     const Buffers b = frame_buffers_at(bs, wi.frame);
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kHatLanes;
+    __shared__ GroupSums sums;
+    group_sums_init(sums);  // published by the barrier of the partial reduction
     double alpha, beta;
-    cg_scalars(b, i, gridDim.x, alpha, beta, wi.block);
-    double g = 0.0, d = 0.0;
+    cg_scalars(b, i, cg_partials(f.N), alpha, beta, wi.block);
+    uint32_t pass = 0;
     if (!b.counts->error) {
         // reads (r_i, w_i, s_{i-1}), writes (r_{i+1}, w_{i+1}, s_i)
         // (selects, not an index into the by-value struct: that would move it to scratch)
@@ -548,6 +608,7 @@ __global__ void __launch_bounds__(256) cg_iter_kernel(const Buffers* __restrict_
         const uint32_t groups = gridDim.x * kHatRowsPerBlock;
         for (uint32_t base = wi.block * kHatRowsPerBlock; base < f.N; base += groups) {
             const uint32_t n = base + threadIdx.x / kHatLanes;
+            double g = 0.0, d = 0.0;
             // component a = sub of the row's own vectors (lanes 0..5), loaded
             // before the row apply so that their latency overlaps it
             const bool own = n < f.N && sub < 6;
@@ -577,9 +638,9 @@ __global__ void __launch_bounds__(256) cg_iter_kernel(const Buffers* __restrict_
                 g += (double)r_next * (double)r_next;
                 d += w_next * (double)r_next;
             }
+            group_partials2(b, sums, pass++, kSlotCgGamma + i + 1, g, kSlotCgDelta + i + 1, d, base);
         }
     }
-    write_partials2<double>(b, kSlotCgGamma + i + 1, g, kSlotCgDelta + i + 1, d, wi.block);
 }
 
 // Residual refinement on the CG solution (preconditioned steepest descent on
@@ -590,7 +651,10 @@ __global__ void __launch_bounds__(256) cg_refine_residual_kernel(const Buffers* 
     const Buffers b = frame_buffers_at(bs, wi.frame);
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kHatLanes;
-    double acc = 0.0;
+    __shared__ GroupSums sums;
+    group_sums_init(sums);
+    __syncthreads();
+    uint32_t pass = 0;
     if (!b.counts->error) {
         const float* X = b.cg_x;
         auto load = [&](uint32_t j, float v[6]) {
@@ -601,6 +665,7 @@ __global__ void __launch_bounds__(256) cg_refine_residual_kernel(const Buffers* 
         const uint32_t groups = gridDim.x * kHatRowsPerBlock;
         for (uint32_t base = wi.block * kHatRowsPerBlock; base < f.N; base += groups) {
             const uint32_t n = base + threadIdx.x / kHatLanes;
+            double acc = 0.0;
             // component a = sub of the row (lanes 0..5), its inputs loaded
             // before the row apply
             const bool own = n < f.N && sub < 6;
@@ -614,9 +679,9 @@ __global__ void __launch_bounds__(256) cg_refine_residual_kernel(const Buffers* 
                 b.q[k] = rr;
                 acc += (double)rr * (double)rr;
             }
+            group_partial(b, sums, pass++, kSlotRefNum, acc, base);
         }
     }
-    write_partial<double>(b, kSlotRefNum, acc, wi.block);
 }
 
 // ... den = (r^, A^ r^) ...
@@ -625,7 +690,10 @@ __global__ void __launch_bounds__(256) cg_refine_curvature_kernel(const Buffers*
     const Buffers b = frame_buffers_at(bs, wi.frame);
     const FrameDev& f = *b.frame;
     const uint32_t sub = threadIdx.x % kHatLanes;
-    double acc = 0.0;
+    __shared__ GroupSums sums;
+    group_sums_init(sums);
+    __syncthreads();
+    uint32_t pass = 0;
     if (!b.counts->error) {
         const float* Q = b.q;
         auto load = [&](uint32_t j, float v[6]) {
@@ -636,6 +704,7 @@ __global__ void __launch_bounds__(256) cg_refine_curvature_kernel(const Buffers*
         const uint32_t groups = gridDim.x * kHatRowsPerBlock;
         for (uint32_t base = wi.block * kHatRowsPerBlock; base < f.N; base += groups) {
             const uint32_t n = base + threadIdx.x / kHatLanes;
+            double acc = 0.0;
             const bool own = n < f.N && sub < 6;
             const double q = own ? (double)Q[6 * (size_t)n + sub] : 0.0;
             double y[6];
@@ -644,18 +713,19 @@ __global__ void __launch_bounds__(256) cg_refine_curvature_kernel(const Buffers*
                 const double ya = sub == 0 ? y[0] : sub == 1 ? y[1] : sub == 2 ? y[2] : sub == 3 ? y[3] : sub == 4 ? y[4] : y[5];
                 acc += q * (ya + q);
             }
+            group_partial(b, sums, pass++, kSlotRefDen, acc, base);
         }
     }
-    write_partial<double>(b, kSlotRefDen, acc, wi.block);
 }
 
 // ... x^ += (num / den) r^ (all refinement steps but the last; the last is
 // folded into the warp update).
-__global__ void __launch_bounds__(256) cg_refine_step_kernel(const Buffers* __restrict__ bs, uint32_t spmv_blocks) {
+__global__ void __launch_bounds__(256) cg_refine_step_kernel(const Buffers* __restrict__ bs) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     double num, den;
-    reduce_partials2<double>(slot_ptr(b, kSlotRefNum), spmv_blocks, slot_ptr(b, kSlotRefDen), spmv_blocks, num, den);
+    const uint32_t parts = cg_partials(f.N);
+    reduce_partials2<double>(slot_ptr(b, kSlotRefNum), parts, slot_ptr(b, kSlotRefDen), parts, num, den);
     if (b.counts->error) return;
     const double a = den > 0.0 ? num / den : 0.0;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -713,18 +783,19 @@ __device__ inline Mat34 mat34_of(DQ d) {
 // applyWarpFieldUpdateKernel (SolverIterationData.cu:9-57): Rodrigues twist
 // (MatUtils.cpp:10-40) left-composed with the node's SE3.
 // kHat: the solution is x = L^-T (x^ + a r^) of the fused CG, a the last
-// refinement step (0 without refinement).
+// refinement step (0 without refinement, refined = 0).
 template <bool kHat>
-__global__ void __launch_bounds__(256) apply_update_kernel(const Buffers* __restrict__ bs, uint32_t k, uint32_t refine_blocks) {
+__global__ void __launch_bounds__(256) apply_update_kernel(const Buffers* __restrict__ bs, uint32_t k, uint32_t refined) {
     const Buffers b = frame_buffers(bs);
     const DQ* __restrict__ se3_in = (k & 1) ? b.se3[1] : b.se3[0];
     DQ* __restrict__ se3_out = (k & 1) ? b.se3[0] : b.se3[1];
     const FrameDev& f = *b.frame;
     const uint32_t stride = gridDim.x * blockDim.x;
     double step = 0.0;
-    if (kHat && refine_blocks) {
+    if (kHat && refined) {
         double num, den;
-        reduce_partials2<double>(slot_ptr(b, kSlotRefNum), refine_blocks, slot_ptr(b, kSlotRefDen), refine_blocks, num, den);
+        const uint32_t parts = cg_partials(f.N);
+        reduce_partials2<double>(slot_ptr(b, kSlotRefNum), parts, slot_ptr(b, kSlotRefDen), parts, num, den);
         step = den > 0.0 ? num / den : 0.0;
     }
     const bool bad = b.counts->error != 0;
@@ -739,7 +810,7 @@ __global__ void __launch_bounds__(256) apply_update_kernel(const Buffers* __rest
 #pragma unroll
             for (int a = 0; a < 6; ++a) {
                 const size_t k = 6 * (size_t)n + a;
-                xh[a] = (double)b.cg_x[k] + (refine_blocks ? step * (double)b.q[k] : 0.0);
+                xh[a] = (double)b.cg_x[k] + (refined ? step * (double)b.q[k] : 0.0);
             }
             const double* L = b.linv64 + kTri * (size_t)n;
 #pragma unroll
@@ -841,19 +912,24 @@ void launch_apply_update(const Batch& B, uint32_t k, hipStream_t stream) {
 
 // Fused CG of the mixed precision mode: transform, init, one launch per
 // iteration, refinement, warp update of GN iteration k.
-// Workgroups per frame of the fused CG's row kernels: at most 64, each
-// looping over its rows (two passes of 32 rows at ~4k nodes), so a launch of
-// 32 frames is ~2k workgroups and every workgroup reduces 64 partials at its
-// start.  The count depends on the node count alone: a frame's partial sums,
-// and so its results, do not depend on the batch it is solved in.
-static uint32_t cg_grid(uint32_t nodes) {
+// Workgroups per frame of the fused CG's row kernels.  Their partial sums are
+// per wave (cg_partials), so the count changes no result and follows the
+// launch: a lone frame or a small batch takes one workgroup per 32 block rows
+// (one pass, the shortest chain of dependent loads); a larger batch, whose
+// launches are bandwidth bound, at most 64 per frame, each looping over its
+// rows (two passes at ~4k nodes), which halves the workgroups that reduce the
+// previous iteration's partials.
+constexpr uint32_t kCgOnePassWorkgroups = 2048;  // batch total up to which every row group has its own workgroup
+static uint32_t cg_grid(uint32_t nodes, uint32_t frames) {
     const uint32_t g = (nodes + kHatRowsPerBlock - 1) / kHatRowsPerBlock;
-    return std::max<uint32_t>(kXcds, std::min<uint32_t>((g + kXcds - 1) / kXcds * kXcds, 64u));
+    uint32_t cap = (uint64_t)g * frames <= kCgOnePassWorkgroups ? g : 64u;
+    cap = std::max(cap, (g + kCgMaxPasses - 1) / kCgMaxPasses);  // a workgroup's passes fit its LDS slots
+    return std::max<uint32_t>(kXcds, (std::min<uint32_t>(g, cap) + kXcds - 1) / kXcds * kXcds);
 }
 
 void launch_cg_solve(const Batch& B, uint32_t iters, uint32_t refine, uint32_t k, const Penalty& pen,
                      hipStream_t stream) {
-    const uint32_t sg = cg_grid(B.nodes);
+    const uint32_t sg = cg_grid(B.nodes, B.frames);
     iters = std::min(iters, kMaxIters);
     const uint32_t tgrid = std::min<uint32_t>((8 * B.nodes + 255) / 256, 2048u);  // grid-stride over P + N
     hipLaunchKernelGGL(cg_transform_kernel, dim3(tgrid, B.frames), dim3(256), 0, stream, B.dev, pen);
@@ -864,9 +940,9 @@ void launch_cg_solve(const Batch& B, uint32_t iters, uint32_t refine, uint32_t k
         hipLaunchKernelGGL(cg_refine_residual_kernel, dim3(sg, B.frames), dim3(256), 0, stream, B.dev);
         hipLaunchKernelGGL(cg_refine_curvature_kernel, dim3(sg, B.frames), dim3(256), 0, stream, B.dev);
         if (r + 1 < refine)
-            hipLaunchKernelGGL(cg_refine_step_kernel, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev, sg);
+            hipLaunchKernelGGL(cg_refine_step_kernel, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev);
     }
-    hipLaunchKernelGGL(apply_update_kernel<true>, dim3(64, B.frames), dim3(256), 0, stream, B.dev, k, refine ? sg : 0u);
+    hipLaunchKernelGGL(apply_update_kernel<true>, dim3(64, B.frames), dim3(256), 0, stream, B.dev, k, refine ? 1u : 0u);
 }
 
 uint32_t row_group_grid(uint32_t nodes) { return spmv_grid(nodes); }
@@ -899,7 +975,7 @@ void launch_refine_step(const Batch& B, bool fp64, uint32_t spmv_blocks, hipStre
 
 // One fused CG iteration (stage timing).
 void launch_cg_iter_once(const Batch& B, hipStream_t stream) {
-    hipLaunchKernelGGL(cg_iter_kernel, dim3(cg_grid(B.nodes), B.frames), dim3(256), 0, stream, B.dev, 1u);
+    hipLaunchKernelGGL(cg_iter_kernel, dim3(cg_grid(B.nodes, B.frames), B.frames), dim3(256), 0, stream, B.dev, 1u);
 }
 
 void launch_copy_se3(const Batch& B, hipStream_t stream) {

@@ -822,7 +822,7 @@ __global__ void __launch_bounds__(256) cost_kernel(const Buffers* __restrict__ b
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
     __syncthreads();
     // one partial per block; the host sums them in block order
-    if (threadIdx.x == 0) b.partial[(size_t)kSlotCost * kPartialStride + blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+    if (threadIdx.x == 0) b.partial[(size_t)kSlotCost * b.partial_stride + blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
 // ------------------------------------------------------------- host launch

@@ -16,7 +16,7 @@ static_assert(kHatLanes >= 6 && (kHatLanes & (kHatLanes - 1)) == 0,
               "lanes 0..5 of a row own its six components; the row sum is a butterfly");
 constexpr uint32_t kVecBlocks = 64;   // grid of the per-node vector kernels
 
-// partial-sum slots (kPartialStride doubles each)
+// partial-sum slots (partial_stride doubles each)
 constexpr uint32_t kSlotDelta = 0;     // delta_k   : slot k, k <= kMaxIters
 constexpr uint32_t kSlotDQ = 72;       // dot(d, q)_k: slot 72 + k
 constexpr uint32_t kSlotRefNum = 144;  // refinement numerator / denominator
@@ -24,7 +24,7 @@ constexpr uint32_t kSlotRefDen = 145;
 constexpr uint32_t kMaxIters = MVDR_MAX_PCG_ITERATIONS;
 static_assert(kMaxIters < kSlotDQ, "delta_k slots 0..kMaxIters precede the dot(d, q) slots");
 
-__device__ inline double* slot_ptr(const Buffers& b, uint32_t slot) { return b.partial + (size_t)slot * kPartialStride; }
+__device__ inline double* slot_ptr(const Buffers& b, uint32_t slot) { return b.partial + (size_t)slot * b.partial_stride; }
 
 // Sums of the partials of two slots (n0 and n1 of them), computed identically
 // by every block (fixed order: strided per-thread sums, a wave butterfly, the

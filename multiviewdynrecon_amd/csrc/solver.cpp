@@ -435,7 +435,9 @@ static size_t layout_buffers(const mvdr_solver_config& c, Buffers& b, FrameDev*&
         for (float** v : {&b.cg_p, &b.cg_x, &b.cg_b}) A(*v, 6 * (size_t)c.max_nodes);
         A(b.cg_scal, 2 * 72);
     }
-    A(b.partial, (size_t)kPartialSlots * kPartialStride);
+    // per-block sums, or one per 32-row group in the fused CG
+    b.partial_stride = std::max<uint32_t>(kPartialStride, (c.max_nodes + 31) / 32);
+    A(b.partial, (size_t)kPartialSlots * b.partial_stride);
     A(b.se3[0], c.max_nodes);
     A(b.se3[1], c.max_nodes);
     b.frame = frame_dev;
@@ -458,7 +460,7 @@ mvdr_status mvdr_solver_create(const mvdr_solver_config* config, mvdr_solver** o
         layout_buffers(c, b, s->frame_dev, s->b_dev, s->cap_tiles, s->cap_chunks, static_cast<char*>(block));
         e = hipMemset(b.counts, 0, sizeof(Counts));
     }
-    if (e == hipSuccess) e = hipMemset(b.partial, 0, (size_t)kPartialSlots * kPartialStride * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(b.partial, 0, (size_t)kPartialSlots * b.partial_stride * sizeof(double));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMemcpy(s->b_dev, &b, sizeof(Buffers), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -669,7 +671,7 @@ mvdr_status mvdr_solver_total_residual(mvdr_solver* s, void* stream, double* cos
     mvdr_status rc = check_counts(s, cnt);
     if (rc != MVDR_OK) return rc;
     std::vector<double> partials(parts);
-    HIP_TRY(s, hipMemcpy(partials.data(), s->b.partial + (size_t)kSlotCost * kPartialStride, parts * sizeof(double),
+    HIP_TRY(s, hipMemcpy(partials.data(), s->b.partial + (size_t)kSlotCost * s->b.partial_stride, parts * sizeof(double),
                          hipMemcpyDeviceToHost));
     double total = 0.0;
     for (double p : partials) total += p;
@@ -1028,7 +1030,7 @@ __global__ void __launch_bounds__(256) noalloc_setup_kernel(FrameDev f, Buffers 
         *bd = b;
         *b.counts = Counts{};
     }
-    const size_t n = (size_t)kPartialSlots * kPartialStride;
+    const size_t n = (size_t)kPartialSlots * b.partial_stride;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         b.partial[i] = 0.0;
 }
