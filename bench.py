@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage-reps", type=int, default=50)
     ap.add_argument("--batch", type=int, default=32, help="frames solved per step (one launch per stage)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in launch shapes (the autotuner changes no result, only workgroup counts)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the extra legs of the default run: the uneven 16-scene batch and the matrix-free "
                          "8k-node workload")
@@ -235,7 +237,7 @@ def measured_traffic(kernel, workload, frames):
     return None
 
 
-def uneven_measure(batches, warmup, graph=1, solo_reps=5):
+def uneven_measure(batches, warmup, graph=1, solo_reps=5, autotune=True):
     """16 independent scenes, N ~ U[1000, 8000] nodes and 1-3 views each, solved
     as one batch (mixed precision, one hipGraph per batch): aggregate frames/s
     over `batches` batches, the per-scene latency (a scene completes with its
@@ -261,6 +263,8 @@ def uneven_measure(batches, warmup, graph=1, solo_reps=5):
         solvers.append(s)
     batch = NonRigidBatch(solvers)
     print(f"[bench] {count} uneven scenes resident: nodes {sorted(nodes)}, views {views}", file=sys.stderr, flush=True)
+    batch.SolveNonRigidAlignment()
+    tuned = batch.autotune(reps=10) if autotune else None
     for _ in range(warmup):
         batch.SolveNonRigidAlignment()
     torch.cuda.synchronize()
@@ -278,7 +282,8 @@ def uneven_measure(batches, warmup, graph=1, solo_reps=5):
            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
            "solo_latency_ms": {"p50": round(float(np.percentile(solo, 50)), 4),
                                "p99": round(float(np.percentile(solo, 99)), 4)},
-           "nodes": nodes, "views": views, "gn_iterations": cfg.gn_iterations, "pcg_iterations": cfg.pcg_iterations}
+           "nodes": nodes, "views": views, "gn_iterations": cfg.gn_iterations, "pcg_iterations": cfg.pcg_iterations,
+           "launch_params": tuned}
     print(f"[bench] uneven batch: {out['frames_per_s']} frames/s, latency p50 {out['latency_ms']['p50']} "
           f"p99 {out['latency_ms']['p99']} ms", file=sys.stderr, flush=True)
     del batch, solvers, frames
@@ -287,7 +292,7 @@ def uneven_measure(batches, warmup, graph=1, solo_reps=5):
 
 def bench_uneven(args):
     """The uneven 16-scene batch as the benchmark line (aggregate frames/s)."""
-    m = uneven_measure(max(args.steps, 100), args.warmup, args.graph)
+    m = uneven_measure(max(args.steps, 100), args.warmup, args.graph, autotune=not args.no_autotune)
     out = {
         "metric": "aggregate frames/s, 16 uneven scenes per batch (1k-8k nodes, 1-3 views)",
         "value": m["frames_per_s"], "unit": "frames/s", "n_gpus": 1, "steps": m["batches"], "warmup": args.warmup,
@@ -301,7 +306,7 @@ def bench_uneven(args):
     print(json.dumps(out), flush=True)
 
 
-def matrix_free_measure(frames, steps):
+def matrix_free_measure(frames, steps, autotune=True):
     """The ~8k-node, 15-PCG matrix-free workload, mixed precision: ms per frame of
     a batched step of `frames` frames, and one frame alone."""
     import torch
@@ -320,6 +325,8 @@ def matrix_free_measure(frames, steps):
         solvers.append(s)
     batch = NonRigidBatch(solvers)
     batch.SolveNonRigidAlignment()
+    tuned = batch.autotune(reps=5) if autotune else None
+    batch.SolveNonRigidAlignment()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -329,7 +336,8 @@ def matrix_free_measure(frames, steps):
     single = time_solves(solvers[0], steps, 1)
     out = {"workload": WORKLOAD_NAMES[3], "frames_per_step": frames, "ms_per_frame": round(ms / frames, 4),
            "latency_ms_single_frame": round(single, 4), "nodes": scenes[0].num_nodes,
-           "surfels": scenes[0].num_surfels, "gn_iterations": cfg.gn_iterations, "pcg_iterations": cfg.pcg_iterations}
+           "surfels": scenes[0].num_surfels, "gn_iterations": cfg.gn_iterations, "pcg_iterations": cfg.pcg_iterations,
+           "launch_params": tuned}
     print(f"[bench] matrix-free 8k: {out['ms_per_frame']} ms/frame in batches of {frames}, "
           f"{out['latency_ms_single_frame']} ms alone", file=sys.stderr, flush=True)
     return out
@@ -380,6 +388,14 @@ def main():
         solvers.append(s)
     batch = NonRigidBatch(solvers)
     print(f"[bench] {B} frames resident", file=sys.stderr, flush=True)
+    tuned = {}
+    if not args.no_autotune:
+        # launch shapes for this batch and for a lone frame, before any timing
+        batch.SolveNonRigidAlignment()
+        tuned["batch"] = batch.autotune(reps=10)
+        solvers[0].SolveNonRigidAlignment()
+        tuned["single"] = solvers[0].autotune(reps=10)
+        print(f"[bench] autotuned launch shapes: {json.dumps(tuned)}", file=sys.stderr, flush=True)
 
     # timed region: K steps, one step = one batched solve of the B frames
     for _ in range(args.warmup):
@@ -493,14 +509,15 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baseline_threads": cpu_mt,
         "twist_rel_err": max(errs) if errs else None,
+        "launch_params": {"autotuned": not args.no_autotune, **tuned},
         "plain_fp32": {"ms_per_frame": round(plain_ms, 4), "speedup": round(plain_ms / ms_per_frame, 3),
                        "twist_rel_err": max(plain_errs) if plain_errs else None},
     }
     if args.workload == 1 and not args.no_extras:
         # the other two solve workloads at their real sizes, beside the headline
         del batch, solvers, frames
-        out["uneven_batch"] = uneven_measure(30, 3, args.graph, solo_reps=3)
-        out["matrix_free_8k"] = matrix_free_measure(4, 5)
+        out["uneven_batch"] = uneven_measure(30, 3, args.graph, solo_reps=3, autotune=not args.no_autotune)
+        out["matrix_free_8k"] = matrix_free_measure(4, 5, autotune=not args.no_autotune)
     print(json.dumps(out), flush=True)
 
 

@@ -288,6 +288,37 @@ mvdr_status mvdr_batch_solve(mvdr_batch* batch, void* stream);
 /* mvdr_solver_bench_stage over all members at once (one launch per rep). */
 mvdr_status mvdr_batch_bench_stage(mvdr_batch* batch, int stage, uint32_t reps, void* stream, float* avg_ms);
 
+/* ------------------------------------------------------- launch-shape tuning
+ * No counterpart in the reference (its launches use fixed block sizes).  The
+ * workgroup counts below spread work over the device and change no result
+ * (the fused CG's inner products are summed per 32-row group, the term
+ * evaluation and the CG transform are grid-stride loops over independent
+ * items).  A process-wide table holds them per (node count in 1024-node
+ * buckets, frames per launch); launches without an entry use the built-in
+ * rule.  Captured graphs are re-captured after the table changes.  The table
+ * is read from the file named by MVDR_LAUNCH_TABLE, if set, at the first
+ * launch. */
+typedef struct {
+    uint32_t cg_workgroups;         /* fused CG row kernels, per frame (0 in a set: the rule) */
+    uint32_t eval_workgroups;       /* term evaluation, per frame                             */
+    uint32_t transform_workgroups;  /* CG transform, per frame                                */
+} mvdr_launch_params;
+
+/* Times each tunable launch over its candidates (reps launches each, on the
+ * data of the last GN iteration of a completed solve, as bench_stage) and
+ * keeps the fastest in the table for this solver's / batch's shape; `chosen`
+ * (nullable) receives the resolved counts (0 for launches the config does not
+ * use).  Synchronises `stream`. */
+mvdr_status mvdr_solver_autotune(mvdr_solver* solver, uint32_t reps, void* stream, mvdr_launch_params* chosen);
+mvdr_status mvdr_batch_autotune(mvdr_batch* batch, uint32_t reps, void* stream, mvdr_launch_params* chosen);
+/* The counts launches for `nodes` nodes and `frames` frames use. */
+mvdr_status mvdr_launch_params_get(uint32_t nodes, uint32_t frames, mvdr_launch_params* out);
+mvdr_status mvdr_launch_params_set(uint32_t nodes, uint32_t frames, const mvdr_launch_params* params);
+mvdr_status mvdr_launch_table_clear(void);
+/* Text file, one line per entry: node_bucket frames cg eval transform. */
+mvdr_status mvdr_launch_table_save(const char* path);
+mvdr_status mvdr_launch_table_load(const char* path);
+
 /* ------------------------------------------------------ allocation-free solve
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

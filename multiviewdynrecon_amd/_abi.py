@@ -59,6 +59,10 @@ class FrameInput(C.Structure):
     ]
 
 
+class LaunchParams(C.Structure):
+    _fields_ = [("cg_workgroups", u32), ("eval_workgroups", u32), ("transform_workgroups", u32)]
+
+
 class SolverSizes(C.Structure):
     _fields_ = [
         ("num_dense_terms", u32), ("num_smooth_terms", u32), ("num_foreground_terms", u32),

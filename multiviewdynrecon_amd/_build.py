@@ -24,7 +24,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", 
              "-packed-fp32-ops", "-Wall", "-Wno-unused-function", "-I", INCLUDE]
 HIP_SOURCES = ["kernels_index.hip", "kernels_solve.hip", "kernels_pcg.hip", "kernels_mf.hip", "crossview.hip",
                "warpfield.hip", "render.hip", "rigid.hip",
-               "solver.cpp"]
+               "solver.cpp", "tuning.cpp"]
 
 
 def _run(cmd):

@@ -15,6 +15,7 @@
 #include "common.h"
 #include "device_util.h"
 #include "pcg_util.h"
+#include "tuning.h"
 #include "terms.h"
 
 namespace mvdr {
@@ -920,19 +921,31 @@ void launch_apply_update(const Batch& B, uint32_t k, hipStream_t stream) {
 // rows (two passes at ~4k nodes), which halves the workgroups that reduce the
 // previous iteration's partials.
 constexpr uint32_t kCgOnePassWorkgroups = 2048;  // batch total up to which every row group has its own workgroup
+// A tuned count (the launch table) replaces the rule; either is clamped to the
+// frame's row groups and to the passes a workgroup's LDS slots hold.
 static uint32_t cg_grid(uint32_t nodes, uint32_t frames) {
     const uint32_t g = (nodes + kHatRowsPerBlock - 1) / kHatRowsPerBlock;
-    uint32_t cap = (uint64_t)g * frames <= kCgOnePassWorkgroups ? g : 64u;
+    const uint32_t tuned = launch_params(nodes, frames).cg_workgroups;
+    uint32_t cap = tuned ? tuned : ((uint64_t)g * frames <= kCgOnePassWorkgroups ? g : 64u);
     cap = std::max(cap, (g + kCgMaxPasses - 1) / kCgMaxPasses);  // a workgroup's passes fit its LDS slots
     return std::max<uint32_t>(kXcds, (std::min<uint32_t>(g, cap) + kXcds - 1) / kXcds * kXcds);
 }
+
+// The CG transform: a grid-stride loop over pairs and nodes.
+static uint32_t transform_grid(uint32_t nodes, uint32_t frames) {
+    const uint32_t tuned = launch_params(nodes, frames).transform_workgroups;
+    return tuned ? tuned : std::min<uint32_t>((8 * nodes + 255) / 256, 2048u);
+}
+
+uint32_t cg_workgroups_for(uint32_t nodes, uint32_t frames) { return cg_grid(nodes, frames); }
+uint32_t transform_workgroups_for(uint32_t nodes, uint32_t frames) { return transform_grid(nodes, frames); }
 
 void launch_cg_solve(const Batch& B, uint32_t iters, uint32_t refine, uint32_t k, const Penalty& pen,
                      hipStream_t stream) {
     const uint32_t sg = cg_grid(B.nodes, B.frames);
     iters = std::min(iters, kMaxIters);
-    const uint32_t tgrid = std::min<uint32_t>((8 * B.nodes + 255) / 256, 2048u);  // grid-stride over P + N
-    hipLaunchKernelGGL(cg_transform_kernel, dim3(tgrid, B.frames), dim3(256), 0, stream, B.dev, pen);
+    hipLaunchKernelGGL(cg_transform_kernel, dim3(transform_grid(B.nodes, B.frames), B.frames), dim3(256), 0, stream,
+                       B.dev, pen);
     hipLaunchKernelGGL(cg_init_kernel, dim3(sg, B.frames), dim3(256), 0, stream, B.dev);
     for (uint32_t i = 0; i < iters; ++i)
         hipLaunchKernelGGL(cg_iter_kernel, dim3(sg, B.frames), dim3(256), 0, stream, B.dev, i);
@@ -971,6 +984,12 @@ void launch_refine_step(const Batch& B, bool fp64, uint32_t spmv_blocks, hipStre
         hipLaunchKernelGGL(refine_step_kernel<double>, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev, spmv_blocks);
     else
         hipLaunchKernelGGL(refine_step_kernel<float>, dim3(kVecBlocks, B.frames), dim3(256), 0, stream, B.dev, spmv_blocks);
+}
+
+// The CG transform alone (launch tuning).
+void launch_cg_transform_once(const Batch& B, const Penalty& pen, hipStream_t stream) {
+    hipLaunchKernelGGL(cg_transform_kernel, dim3(transform_grid(B.nodes, B.frames), B.frames), dim3(256), 0, stream,
+                       B.dev, pen);
 }
 
 // One fused CG iteration (stage timing).

@@ -5,6 +5,7 @@
 #include "device_util.h"
 #include "eval.h"
 #include "terms.h"
+#include "tuning.h"
 
 namespace mvdr {
 
@@ -827,11 +828,19 @@ __global__ void __launch_bounds__(256) cost_kernel(const Buffers* __restrict__ b
 
 // ------------------------------------------------------------- host launch
 // Terms at the warp field se3[which].
+// Workgroups per frame: a grid-stride loop over the terms (a tuned count from
+// the launch table, else 1024).
+uint32_t eval_workgroups_for(uint32_t nodes, uint32_t frames) {
+    const uint32_t tuned = launch_params(nodes, frames).eval_workgroups;
+    return tuned ? tuned : 1024u;
+}
+
 void launch_eval_terms(const Batch& B, uint32_t which, bool fp64_geometry, hipStream_t stream) {
+    const uint32_t grid = eval_workgroups_for(B.nodes, B.frames);
     if (fp64_geometry)
-        hipLaunchKernelGGL(eval_terms_kernel<double>, dim3(1024, B.frames), dim3(256), 0, stream, B.dev, which);
+        hipLaunchKernelGGL(eval_terms_kernel<double>, dim3(grid, B.frames), dim3(256), 0, stream, B.dev, which);
     else
-        hipLaunchKernelGGL(eval_terms_kernel<float>, dim3(1024, B.frames), dim3(256), 0, stream, B.dev, which);
+        hipLaunchKernelGGL(eval_terms_kernel<float>, dim3(grid, B.frames), dim3(256), 0, stream, B.dev, which);
 }
 
 // gram: the row Gram assembly with FP64 factors (mixed precision mode);

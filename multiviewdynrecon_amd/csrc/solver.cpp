@@ -11,6 +11,8 @@
 #include <type_traits>
 #include <vector>
 
+#include "tuning.h"
+
 #include "common.h"
 
 namespace mvdr {
@@ -34,6 +36,10 @@ void launch_apply_update(const Batch& B, uint32_t k, hipStream_t stream);
 void launch_cg_solve(const Batch& B, uint32_t iters, uint32_t refine, uint32_t k, const Penalty& pen,
                      hipStream_t stream);
 void launch_cg_iter_once(const Batch& B, hipStream_t stream);
+void launch_cg_transform_once(const Batch& B, const Penalty& pen, hipStream_t stream);
+uint32_t cg_workgroups_for(uint32_t nodes, uint32_t frames);
+uint32_t transform_workgroups_for(uint32_t nodes, uint32_t frames);
+uint32_t eval_workgroups_for(uint32_t nodes, uint32_t frames);
 void launch_copy_se3(const Batch& B, hipStream_t stream);
 }  // namespace mvdr
 
@@ -56,6 +62,7 @@ struct mvdr_solver {
     // (mixed mode: S_ij and the Gram sums until formed on demand)
     bool pair_blocks = true;
     uint32_t graph_nodes = 0;           // Batch::nodes the captured graph was sized for
+    uint64_t graph_table = 0;           // launch-table version the graph was captured with
     uint32_t iter = 0;                  // GN iterations done on the current frame
     hipEvent_t done = nullptr;          // end of the last enqueued work
     hipGraph_t graph = nullptr;
@@ -77,6 +84,7 @@ struct mvdr_batch {
     Buffers* dev = nullptr;             // the members' Buffers, one per frame
     Batch batch{};
     uint32_t graph_nodes = 0;
+    uint64_t graph_table = 0;
     hipEvent_t done = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -605,8 +613,10 @@ mvdr_status mvdr_solver_solve(mvdr_solver* s, void* stream) {
     if (s->cfg.use_graph) {
         // The pointer table and every buffer are device-resident, so one
         // captured graph serves every frame of this solver whose node count
-        // rounds to the same grid size; another is captured when it changes.
-        if (s->graph_exec && s->graph_nodes != live_nodes(s->frame_host.N)) {
+        // rounds to the same grid size; another is captured when it, or the
+        // launch table, changes.
+        if (s->graph_exec &&
+            (s->graph_nodes != live_nodes(s->frame_host.N) || s->graph_table != launch_table_version())) {
             (void)hipGraphExecDestroy(s->graph_exec);
             (void)hipGraphDestroy(s->graph);
             s->graph_exec = nullptr;
@@ -614,6 +624,7 @@ mvdr_status mvdr_solver_solve(mvdr_solver* s, void* stream) {
         }
         if (!s->graph_exec) {
             s->graph_nodes = live_nodes(s->frame_host.N);
+            s->graph_table = launch_table_version();
             hipStream_t cap = nullptr;
             HIP_TRY(s, hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
             hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
@@ -929,6 +940,159 @@ mvdr_status mvdr_batch_bench_stage(mvdr_batch* bt, int stage, uint32_t reps, voi
     return rc;
 }
 
+// Launch-shape tuning (tuning.h): for the batch's shape each tunable launch is
+// timed over its candidates on the data of GN iteration k (the stages of a
+// completed solve, as bench_stage) and the fastest is kept in the process-wide
+// launch table.  The parameters spread work over workgroups and change no
+// result; the timed launches rewrite only what they would write in a solve.
+static mvdr_status autotune_impl(const Batch& B, const mvdr_solver_config& c, uint32_t k, uint32_t reps, hipStream_t st,
+                                 mvdr_launch_params* chosen, std::string& err) {
+    const bool fp64 = c.precision == MVDR_PRECISION_MIXED;
+    const bool fused = fp64 && !matrix_free(c);
+    const Penalty pen = kernel_penalty(c, k);
+    LaunchParams p = launch_params(B.nodes, B.frames);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    auto timed = [&](auto&& launch) {
+        float ms = 0.f;
+        if (e != hipSuccess) return ms;
+        launch();  // warm-up
+        e = hipEventRecord(e0, st);
+        for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) launch();
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess) e = hipGetLastError();
+        return ms;
+    };
+    // term evaluation: workgroups per frame of its grid-stride loop
+    {
+        float best = 0.f;
+        uint32_t arg = 0;
+        for (uint32_t cand : {256u, 512u, 1024u, 2048u}) {
+            p.eval_workgroups = cand;
+            set_launch_params(B.nodes, B.frames, p);
+            const float t = timed([&] { launch_eval_terms(B, k & 1, fp64, st); });
+            if (arg == 0 || t < best) { best = t; arg = cand; }
+        }
+        p.eval_workgroups = arg;
+    }
+    if (fused) {
+        // fused CG row kernels: one pass of rows per workgroup, or at most
+        // 128 / 64 / 32 workgroups per frame (the resolved counts, each once)
+        const uint32_t g = (B.nodes + 31) / 32;
+        float best = 0.f;
+        uint32_t arg = 0, seen[4] = {0, 0, 0, 0}, n_seen = 0;
+        for (uint32_t cand : {g, 128u, 64u, 32u}) {
+            p.cg_workgroups = cand;
+            set_launch_params(B.nodes, B.frames, p);
+            const uint32_t resolved = cg_workgroups_for(B.nodes, B.frames);
+            bool dup = false;
+            for (uint32_t q = 0; q < n_seen; ++q) dup = dup || seen[q] == resolved;
+            if (dup) continue;
+            seen[n_seen++] = resolved;
+            const float t = timed([&] { launch_cg_iter_once(B, st); });
+            if (arg == 0 || t < best) { best = t; arg = cand; }
+        }
+        p.cg_workgroups = arg;
+        // CG transform: workgroups per frame of its grid-stride loop
+        best = 0.f;
+        arg = 0;
+        for (uint32_t cand : {256u, 512u, 1024u, 2048u}) {
+            p.transform_workgroups = cand;
+            set_launch_params(B.nodes, B.frames, p);
+            const float t = timed([&] { launch_cg_transform_once(B, pen, st); });
+            if (arg == 0 || t < best) { best = t; arg = cand; }
+        }
+        p.transform_workgroups = arg;
+    }
+    set_launch_params(B.nodes, B.frames, p);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) {
+        err = std::string("HIP: ") + hipGetErrorString(e);
+        return MVDR_E_HIP;
+    }
+    if (chosen) {
+        chosen->cg_workgroups = fused ? cg_workgroups_for(B.nodes, B.frames) : 0;
+        chosen->eval_workgroups = eval_workgroups_for(B.nodes, B.frames);
+        chosen->transform_workgroups = fused ? transform_workgroups_for(B.nodes, B.frames) : 0;
+    }
+    return MVDR_OK;
+}
+
+mvdr_status mvdr_solver_autotune(mvdr_solver* s, uint32_t reps, void* stream, mvdr_launch_params* chosen) {
+    if (!s || reps == 0) return MVDR_E_INVALID;
+    if (!s->indexed || s->iter == 0) return fail(s, MVDR_E_STATE, "autotune needs a completed solve");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    s->self.nodes = live_nodes(s->frame_host.N);
+    const mvdr_status rc = autotune_impl(s->self, s->cfg, s->iter - 1, reps, st, chosen, s->error);
+    if (rc == MVDR_OK) HIP_TRY(s, hipEventRecord(s->done, st));
+    return rc;
+}
+
+mvdr_status mvdr_batch_autotune(mvdr_batch* bt, uint32_t reps, void* stream, mvdr_launch_params* chosen) {
+    if (!bt || reps == 0) return MVDR_E_INVALID;
+    const mvdr_solver* m0 = bt->members[0];
+    for (const mvdr_solver* m : bt->members)
+        if (!m->indexed || m->iter != m0->iter || m->iter == 0) {
+            bt->error = "autotune needs a completed batch solve";
+            return MVDR_E_STATE;
+        }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const mvdr_status rc = autotune_impl(bt->batch, bt->cfg, m0->iter - 1, reps, st, chosen, bt->error);
+    if (rc == MVDR_OK)
+        for (mvdr_solver* m : bt->members) (void)hipEventRecord(m->done, st);
+    return rc;
+}
+
+mvdr_status mvdr_launch_params_get(uint32_t nodes, uint32_t frames, mvdr_launch_params* out) {
+    if (!out || nodes == 0 || nodes > 65535 || frames == 0) return MVDR_E_INVALID;
+    const uint32_t n = live_nodes(nodes);
+    out->cg_workgroups = cg_workgroups_for(n, frames);
+    out->eval_workgroups = eval_workgroups_for(n, frames);
+    out->transform_workgroups = transform_workgroups_for(n, frames);
+    return MVDR_OK;
+}
+
+mvdr_status mvdr_launch_params_set(uint32_t nodes, uint32_t frames, const mvdr_launch_params* p) {
+    if (!p || nodes == 0 || nodes > 65535 || frames == 0) return MVDR_E_INVALID;
+    set_launch_params(live_nodes(nodes), frames, LaunchParams{p->cg_workgroups, p->eval_workgroups, p->transform_workgroups});
+    return MVDR_OK;
+}
+
+mvdr_status mvdr_launch_table_clear(void) {
+    clear_launch_params();
+    return MVDR_OK;
+}
+
+mvdr_status mvdr_launch_table_save(const char* path) {
+    if (!path) return MVDR_E_INVALID;
+    FILE* f = std::fopen(path, "w");
+    if (!f) return MVDR_E_INVALID;
+    const std::string text = launch_table_text();
+    const bool ok = std::fwrite(text.data(), 1, text.size(), f) == text.size();
+    return (std::fclose(f) == 0 && ok) ? MVDR_OK : MVDR_E_INVALID;
+}
+
+mvdr_status mvdr_launch_table_load(const char* path) {
+    if (!path) return MVDR_E_INVALID;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return MVDR_E_INVALID;
+    std::string text;
+    char buf[4096];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, n);
+    std::fclose(f);
+    std::string err;
+    if (!load_launch_table_text(text, err)) {
+        std::fprintf(stderr, "mvdr_launch_table_load: %s\n", err.c_str());
+        return MVDR_E_INVALID;
+    }
+    return MVDR_OK;
+}
+
 const char* mvdr_batch_last_error(const mvdr_batch* bt) { return bt ? bt->error.c_str() : "null batch"; }
 
 mvdr_status mvdr_batch_solve(mvdr_batch* bt, void* stream) {
@@ -945,7 +1109,7 @@ mvdr_status mvdr_batch_solve(mvdr_batch* bt, void* stream) {
     bt->batch.nodes = nodes;
     hipError_t e = hipSuccess;
     if (bt->cfg.use_graph) {
-        if (bt->graph_exec && bt->graph_nodes != nodes) {
+        if (bt->graph_exec && (bt->graph_nodes != nodes || bt->graph_table != launch_table_version())) {
             (void)hipGraphExecDestroy(bt->graph_exec);
             (void)hipGraphDestroy(bt->graph);
             bt->graph_exec = nullptr;
@@ -953,6 +1117,7 @@ mvdr_status mvdr_batch_solve(mvdr_batch* bt, void* stream) {
         }
         if (!bt->graph_exec) {
             bt->graph_nodes = nodes;
+            bt->graph_table = launch_table_version();
             hipStream_t cap = nullptr;
             e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
             if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);

@@ -74,6 +74,13 @@ def lib():
         L.mvdr_solve_noalloc.argtypes = [C.POINTER(_abi.SolverConfig), C.POINTER(_abi.FrameInput), C.c_void_p,
                                          C.c_void_p, C.c_void_p]
         L.mvdr_solver_debug_jacobian.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        LP = C.POINTER(_abi.LaunchParams)
+        L.mvdr_solver_autotune.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, LP]
+        L.mvdr_batch_autotune.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, LP]
+        L.mvdr_launch_params_get.argtypes = [C.c_uint32, C.c_uint32, LP]
+        L.mvdr_launch_params_set.argtypes = [C.c_uint32, C.c_uint32, LP]
+        L.mvdr_launch_table_save.argtypes = [C.c_char_p]
+        L.mvdr_launch_table_load.argtypes = [C.c_char_p]
         _LIB = L
     return _LIB
 
@@ -162,6 +169,14 @@ class NonRigidSolver:
         s = _abi.SolverSizes()
         self._check(lib().mvdr_solver_get_sizes(self.h, C.byref(s)))
         return {name: getattr(s, name) for name, _ in _abi.SolverSizes._fields_}
+
+    def autotune(self, reps=10, stream=None):
+        """Times the tunable launch shapes on this solver's last solve and keeps
+        the fastest in the process-wide launch table; returns the chosen
+        workgroup counts (no result changes)."""
+        out = _abi.LaunchParams()
+        self._check(lib().mvdr_solver_autotune(self.h, reps, _stream_handle(stream), C.byref(out)))
+        return launch_params_dict(out)
 
     def debug_jacobian(self, values, transpose=False):
         """The term Jacobian at the current warp, re-evaluated from the maps with
@@ -262,8 +277,51 @@ class NonRigidBatch:
     def SolveNonRigidAlignment(self, stream=None):
         self._check(lib().mvdr_batch_solve(self.h, _stream_handle(stream)))
 
+    def autotune(self, reps=10, stream=None):
+        """NonRigidSolver.autotune for the batch's shape."""
+        out = _abi.LaunchParams()
+        self._check(lib().mvdr_batch_autotune(self.h, reps, _stream_handle(stream), C.byref(out)))
+        return launch_params_dict(out)
+
     def bench_stage(self, stage, reps, stream=None):
         """Mean ms per launch of one stage over all members (hipEvents on the stream)."""
         out = C.c_float()
         self._check(lib().mvdr_batch_bench_stage(self.h, stage, reps, _stream_handle(stream), C.byref(out)))
         return out.value
+
+
+# ------------------------------------------------------------- launch table
+def launch_params_dict(p):
+    return {name: getattr(p, name) for name, _ in _abi.LaunchParams._fields_}
+
+
+def launch_params(nodes, frames):
+    """Workgroup counts the launches for this shape use (table entry or rule)."""
+    out = _abi.LaunchParams()
+    rc = lib().mvdr_launch_params_get(nodes, frames, C.byref(out))
+    if rc != 0:
+        raise SolverError(rc, "mvdr_launch_params_get")
+    return launch_params_dict(out)
+
+
+def set_launch_params(nodes, frames, cg_workgroups=0, eval_workgroups=0, transform_workgroups=0):
+    p = _abi.LaunchParams(cg_workgroups, eval_workgroups, transform_workgroups)
+    rc = lib().mvdr_launch_params_set(nodes, frames, C.byref(p))
+    if rc != 0:
+        raise SolverError(rc, "mvdr_launch_params_set")
+
+
+def clear_launch_table():
+    lib().mvdr_launch_table_clear()
+
+
+def save_launch_table(path):
+    rc = lib().mvdr_launch_table_save(str(path).encode())
+    if rc != 0:
+        raise SolverError(rc, f"cannot write {path}")
+
+
+def load_launch_table(path):
+    rc = lib().mvdr_launch_table_load(str(path).encode())
+    if rc != 0:
+        raise SolverError(rc, f"cannot load {path}")

@@ -109,7 +109,7 @@ LAYOUT_C = r"""
 #define P(T) printf(#T " %zu\n", sizeof(T));
 #define O(T, f) printf(#T "." #f " %zu\n", offsetof(T, f));
 int main(void) {
-  P(mvdr_intrinsic) P(mvdr_mat34) P(mvdr_dq) P(mvdr_cross_pair) P(mvdr_solver_config)
+  P(mvdr_intrinsic) P(mvdr_mat34) P(mvdr_dq) P(mvdr_cross_pair) P(mvdr_solver_config) P(mvdr_launch_params)
   P(mvdr_frame_input) P(mvdr_solver_sizes) P(mvdr_scene_params) P(mvdr_scene_info)
   P(mvdr_rigid_config) P(mvdr_rigid_input) O(mvdr_rigid_input, relative_pose)
   P(mvdr_render_input) P(mvdr_render_output) O(mvdr_render_input, time_threshold) O(mvdr_render_output, zbuffer)
@@ -134,6 +134,7 @@ def test_ctypes_layout_matches_c_compiler():
         "mvdr_intrinsic": C.sizeof(_abi.Intrinsic), "mvdr_mat34": C.sizeof(_abi.Mat34), "mvdr_dq": C.sizeof(_abi.DQ),
         "mvdr_cross_pair": C.sizeof(_abi.CrossPair), "mvdr_solver_config": C.sizeof(_abi.SolverConfig),
         "mvdr_frame_input": C.sizeof(_abi.FrameInput), "mvdr_solver_sizes": C.sizeof(_abi.SolverSizes),
+        "mvdr_launch_params": C.sizeof(_abi.LaunchParams),
         "mvdr_scene_params": C.sizeof(_abi.SceneParams), "mvdr_scene_info": C.sizeof(_abi.SceneInfo),
         "mvdr_crossview_config": C.sizeof(_abi.CrossViewConfig), "mvdr_crossview_input": C.sizeof(_abi.CrossViewInput),
         "mvdr_render_input": C.sizeof(_abi.RenderInput), "mvdr_render_output": C.sizeof(_abi.RenderOutput),
