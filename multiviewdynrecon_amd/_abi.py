@@ -137,3 +137,12 @@ class RigidInput(C.Structure):
     _fields_ = [("num_views", u32), ("width", u32), ("height", u32), ("intrinsic", Intrinsic * V),
                 ("reference_vertex", vp * V), ("reference_normal", vp * V), ("conversion_vertex", vp * V),
                 ("conversion_normal", vp * V), ("world2camera", Mat34 * V), ("relative_pose", Mat34 * V)]
+
+
+class NodeCorrection(C.Structure):
+    _fields_ = [("num_views", u32), ("width", u32), ("height", u32), ("intrinsic", Intrinsic * V),
+                ("initial_camera_se3", Mat34 * V), ("world2camera", Mat34 * V), ("previous_vertex", vp * V),
+                ("guided_se3", vp * V), ("guided_valid", vp * V), ("num_nodes", u32), ("node_error", vp),
+                ("live_nodes", vp), ("node_se3", vp), ("node_knn", vp), ("node_knn_weight", vp),
+                ("num_surfels", u32), ("live_vertex", vp), ("live_normal", vp), ("surfel_knn", vp),
+                ("surfel_knn_weight", vp), ("correction", vp)]

@@ -735,33 +735,6 @@ __global__ void __launch_bounds__(256) cg_refine_step_kernel(const Buffers* __re
 }
 
 // ---------------------------------------------------------- warp update
-// DualQuaternion(const mat34&) (DualQuaternion.h:123-130) with Quaternion(mat33)
-// (Quaternion.h:46-77).
-__device__ inline DQ dq_from_mat34(const Mat34& T) {
-    const float m00 = T.c0.x, m10 = T.c0.y, m20 = T.c0.z;
-    const float m01 = T.c1.x, m11 = T.c1.y, m21 = T.c1.z;
-    const float m02 = T.c2.x, m12 = T.c2.y, m22 = T.c2.z;
-    float4 q;
-    const float tr = m00 + m11 + m22;
-    if (tr > 0) {
-        const float s = sqrtf(tr + 1.0f) * 2;
-        q.w = s * 0.25f; q.x = (m21 - m12) / s; q.y = (m02 - m20) / s; q.z = (m10 - m01) / s;
-    } else if ((m00 > m11) && (m00 > m22)) {
-        const float s = sqrtf(1.0f + m00 - m11 - m22) * 2;
-        q.w = (m21 - m12) / s; q.x = 0.25f * s; q.y = (m01 + m10) / s; q.z = (m02 + m20) / s;
-    } else if (m11 > m22) {
-        const float s = sqrtf(1.0f + m11 - m00 - m22) * 2;
-        q.w = (m02 - m20) / s; q.x = (m01 + m10) / s; q.y = 0.25f * s; q.z = (m12 + m21) / s;
-    } else {
-        const float s = sqrtf(1.0f + m22 - m00 - m11) * 2;
-        q.w = (m10 - m01) / s; q.x = (m02 + m20) / s; q.y = (m12 + m21) / s; q.z = 0.25f * s;
-    }
-    DQ o;
-    o.q0 = q;
-    o.q1 = qmul(make_float4(0.5f * T.t.x, 0.5f * T.t.y, 0.5f * T.t.z, 0.f), q);
-    return o;
-}
-
 // DualQuaternion::operator mat34 (DualQuaternion.h:309-319): normalised, with
 // Quaternion::matrix() normalising the rotation part once more.
 __device__ inline Mat34 mat34_of(DQ d) {

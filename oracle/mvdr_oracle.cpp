@@ -30,6 +30,7 @@
 #include "../include/mvdr/render.h"
 #include "../include/mvdr/rigid.h"
 #include "../include/mvdr/solver.h"
+#include "../include/mvdr/warpfield.h"
 
 #include <algorithm>
 #include <cmath>
@@ -1614,6 +1615,113 @@ void mvdr_oracle_warp(const float* field_dq, uint32_t n, const float* points, co
             on[0] = b.x; on[1] = b.y; on[2] = b.z;
         }
     }
+}
+
+// CorrectLargeErrorNode, correspondence-guided form (NonRigidSolver.cu:100-138,
+// :155-216; rules in include/mvdr/warpfield.h), on host arrays in place, FP32.
+namespace {
+Q<float> qpow_ref(Q<float> q, float t) {  // Quaternion::pow (Quaternion.h:235-249)
+    const float n = std::sqrt(q.x * q.x + q.y * q.y + q.z * q.z);
+    const float nt = std::atan2(n, q.w) * t;
+    if (std::fabs(n) < 1e-6f) return {0.f, 0.f, 0.f, std::cos(nt)};
+    const float sc = std::sin(nt) / n;
+    return {q.x * sc, q.y * sc, q.z * sc, std::cos(nt)};
+}
+DQ<float> dq_scale_ref(const DQ<float>& d, float ratio) {  // DualQuaternion::operator*(float) (:165-178)
+    float t = ratio;
+    if (t > 1.0f) t = 1.0f;
+    else if (t < 1e-10f) t = 1e-10f;
+    const Q<float> r = qmul(d.q1, conj(d.q0));
+    const Q<float> q0s = qpow_ref(d.q0, t);
+    return {q0s, qmul(Q<float>{t * r.x, t * r.y, t * r.z, t * r.w}, q0s)};
+}
+DQ<float> dq_mul_ref(const DQ<float>& a, const DQ<float>& b) {  // DualQuaternion::operator* (:152-157)
+    const Q<float> u = qmul(a.q1, b.q0), v = qmul(a.q0, b.q1);
+    return {qmul(a.q0, b.q0), {u.x + v.x, u.y + v.y, u.z + v.z, u.w + v.w}};
+}
+uint32_t pixel_ref(float x, float z, float f, float c) {
+    const float p = (float)(((double)x / ((double)z + 1e-10)) * (double)f + (double)c);
+    if (!(p > 0.f)) return 0u;
+    if (p >= 4294967040.f) return 0xFFFFFFFFu;
+    return (uint32_t)std::nearbyint(p);
+}
+}  // namespace
+
+int mvdr_oracle_correct_nodes(const mvdr_node_correction* in) {
+    if (!in || in->num_views < 1 || in->num_views > MVDR_MAX_VIEWS) return -1;
+    const uint32_t V = in->num_views, W = in->width, H = in->height, N = in->num_nodes;
+    const int hw = 2;                    // CorrSearchWindowHalfSize (NonRigidSolver.h:41)
+    const float r2 = 0.025f * 0.025f;    // NODE_RADIUS^2 (GlobalConfigs.h:85-86)
+    float4_t* nodes = static_cast<float4_t*>(in->live_nodes);
+    const float* err = static_cast<const float*>(in->node_error);
+    mvdr_dq* corr = static_cast<mvdr_dq*>(in->correction);
+    for (uint32_t n = 0; n < N; ++n) {
+        DQ<float> out{{0.f, 0.f, 0.f, 1.f}, {0.f, 0.f, 0.f, 0.f}};
+        const float4_t node = nodes[n];
+        uint32_t v = 0xFFFFFFFFu;  // decodeNodesCameraView (CommonUtils.h:110-117)
+        for (uint32_t i = 0; i < (uint32_t)MVDR_MAX_VIEWS; ++i)
+            if ((float)i < node.w && node.w <= (float)(i + 1)) { v = i; break; }
+        if (err[n] > 0.2f && v < V) {
+            const M34<float> w2c = M34<float>::from(in->world2camera[v]);
+            const M34<float> init = M34<float>::from(in->initial_camera_se3[v]);
+            const M34<float> w2c_inv = w2c.inverse(), init_inv = init.inverse();
+            const V3<float> c = init_inv.apply(w2c.apply(v3<float>(node.x, node.y, node.z)));
+            const mvdr_intrinsic K = in->intrinsic[v];
+            const uint32_t u = pixel_ref(c.x, c.z, K.fx, K.cx), y = pixel_ref(c.y, c.z, K.fy, K.cy);
+            M34<float> sel = M34<float>::identity();
+            const bool room = W > 2u * hw && H > 2u * hw;
+            if (room && !(u >= W - hw || u < (uint32_t)hw || y >= H - hw || y < (uint32_t)hw)) {
+                float best = 1e6f;
+                const float4_t* prev = static_cast<const float4_t*>(in->previous_vertex[v]);
+                const float* guided = static_cast<const float*>(in->guided_se3[v]);
+                const uint8_t* valid = static_cast<const uint8_t*>(in->guided_valid[v]);
+                for (uint32_t i = u - hw; i <= u + hw; ++i)
+                    for (uint32_t j = y - hw; j <= y + hw; ++j) {
+                        const size_t o = (size_t)j * W + i;
+                        const float pv[4] = {prev[o].x, prev[o].y, prev[o].z, prev[o].w};
+                        if (valid[o] != 1 || is_zero_vertex<float>(pv)) continue;
+                        M34<float> G;
+                        const float* g = guided + 12 * o;
+                        for (int col = 0; col < 3; ++col) G.c[col] = v3<float>(g[3 * col], g[3 * col + 1], g[3 * col + 2]);
+                        G.t = v3<float>(g[9], g[10], g[11]);
+                        const V3<float> d = c - v3<float>(pv[0], pv[1], pv[2]);
+                        const float d2 = d.x * d.x + d.y * d.y + d.z * d.z;
+                        const V3<float> m = G.apply(c) - c;
+                        if (best > d2 && d2 < 1e-4f && m.x * m.x + m.y * m.y + m.z * m.z < 4 * r2) {
+                            best = d2;
+                            sel = G;
+                        }
+                    }
+                sel = w2c_inv.mul(init).mul(sel).mul(init_inv).mul(w2c);
+            }
+            out = dq_scale_ref(DQ<float>::from_mat34(sel), err[n]);
+        }
+        out.to(corr[n]);
+    }
+    std::vector<DQ<float>> field(N);
+    for (uint32_t n = 0; n < N; ++n) field[n] = DQ<float>::from(corr[n]);
+    auto apply = [&](float4_t* pts, float4_t* normals, const uint16_t* knn, const float* w, uint32_t count,
+                     mvdr_dq* se3) {
+        for (uint32_t p = 0; p < count; ++p) {
+            const uint16_t* k = knn + 4 * (size_t)p;
+            if (k[0] >= N || k[1] >= N || k[2] >= N || k[3] >= N) continue;
+            const DQ<float> avg = blend(field, k, w + 4 * (size_t)p);
+            const M34<float> T = avg.se3_matrix();
+            const V3<float> a = T.apply(v3<float>(pts[p].x, pts[p].y, pts[p].z));
+            pts[p].x = a.x; pts[p].y = a.y; pts[p].z = a.z;
+            if (normals) {
+                const V3<float> b = T.rot(v3<float>(normals[p].x, normals[p].y, normals[p].z));
+                normals[p].x = b.x; normals[p].y = b.y; normals[p].z = b.z;
+            }
+            if (se3) dq_mul_ref(avg, DQ<float>::from(se3[p])).to(se3[p]);
+        }
+    };
+    apply(static_cast<float4_t*>(in->live_vertex), static_cast<float4_t*>(in->live_normal),
+          static_cast<const uint16_t*>(in->surfel_knn), static_cast<const float*>(in->surfel_knn_weight), in->num_surfels,
+          nullptr);
+    apply(nodes, nullptr, static_cast<const uint16_t*>(in->node_knn), static_cast<const float*>(in->node_knn_weight), N,
+          static_cast<mvdr_dq*>(in->node_se3));
+    return 0;
 }
 
 // ------------------------------------------------------- solver maps

@@ -44,7 +44,7 @@ def test_warpfield_exports_every_declared_symbol(built_libs):
 
     L = lib()
     names = declared("warpfield.h")
-    assert len(names) == 4
+    assert len(names) == 5
     assert not [n for n in names if not hasattr(L, n)]
 
 
@@ -56,6 +56,13 @@ def test_warpfield_rejects_bad_arguments_without_gpu(built_libs):
     assert lib().mvdr_warp_skin(p, 10, p, 70000, 0.025, p, p, p, p, None) == 2
     assert lib().mvdr_warp_build_node_graph(p, 70000, p, None) == 2
     assert lib().mvdr_warp_inverse(p, 0, p, p, p, p, 10, p, p, None) == 1
+    nc = _abi.NodeCorrection()
+    nc.num_views, nc.width, nc.height, nc.num_nodes = 4, 8, 8, 1
+    assert lib().mvdr_warp_correct_nodes(C.byref(nc), None) == 2  # more views than MVDR_MAX_VIEWS
+    nc.num_views = 1
+    assert lib().mvdr_warp_correct_nodes(C.byref(nc), None) == 1  # node arrays missing
+    nc.num_nodes = 0
+    assert lib().mvdr_warp_correct_nodes(C.byref(nc), None) == 0  # nothing to correct
 
 
 def test_render_exports_and_rejects_bad_shapes_without_gpu(built_libs):
@@ -106,12 +113,14 @@ LAYOUT_C = r"""
 #include "mvdr/crossview.h"
 #include "mvdr/render.h"
 #include "mvdr/rigid.h"
+#include "mvdr/warpfield.h"
 #define P(T) printf(#T " %zu\n", sizeof(T));
 #define O(T, f) printf(#T "." #f " %zu\n", offsetof(T, f));
 int main(void) {
   P(mvdr_intrinsic) P(mvdr_mat34) P(mvdr_dq) P(mvdr_cross_pair) P(mvdr_solver_config) P(mvdr_launch_params)
   P(mvdr_frame_input) P(mvdr_solver_sizes) P(mvdr_scene_params) P(mvdr_scene_info)
   P(mvdr_rigid_config) P(mvdr_rigid_input) O(mvdr_rigid_input, relative_pose)
+  P(mvdr_node_correction) O(mvdr_node_correction, num_nodes) O(mvdr_node_correction, correction)
   P(mvdr_render_input) P(mvdr_render_output) O(mvdr_render_input, time_threshold) O(mvdr_render_output, zbuffer)
   P(mvdr_crossview_config) P(mvdr_crossview_input) O(mvdr_crossview_input, corr_map) O(mvdr_crossview_config, max_distance2)
   O(mvdr_frame_input, intrinsic) O(mvdr_frame_input, world2camera) O(mvdr_frame_input, vertex_confid)
@@ -139,6 +148,7 @@ def test_ctypes_layout_matches_c_compiler():
         "mvdr_crossview_config": C.sizeof(_abi.CrossViewConfig), "mvdr_crossview_input": C.sizeof(_abi.CrossViewInput),
         "mvdr_render_input": C.sizeof(_abi.RenderInput), "mvdr_render_output": C.sizeof(_abi.RenderOutput),
         "mvdr_rigid_config": C.sizeof(_abi.RigidConfig), "mvdr_rigid_input": C.sizeof(_abi.RigidInput),
+        "mvdr_node_correction": C.sizeof(_abi.NodeCorrection),
     }
     for k, v in py.items():
         assert int(c_vals[k]) == v, k
@@ -158,6 +168,8 @@ def test_ctypes_layout_matches_c_compiler():
         "mvdr_render_input.time_threshold": _abi.RenderInput.time_threshold.offset,
         "mvdr_render_output.zbuffer": _abi.RenderOutput.zbuffer.offset,
         "mvdr_rigid_input.relative_pose": _abi.RigidInput.relative_pose.offset,
+        "mvdr_node_correction.num_nodes": _abi.NodeCorrection.num_nodes.offset,
+        "mvdr_node_correction.correction": _abi.NodeCorrection.correction.offset,
     }
     for k, v in offs.items():
         assert int(c_vals[k]) == v, k
