@@ -366,6 +366,7 @@ struct Buffers {
     uint16_t* row_part;      // N x kRowPartners: each row's partners ascending, upper then lower
     float4* mf_u;            // terms: u_t of the matrix-free apply
     float4* mf_g;            // 2 per term: J_t^T u_t of the recomputing matrix-free apply
+    float* n2t_w;            // kv1: slot weight of each Node2Term entry's node (sign: several slots)
     // assembly partials of runs crossing 64-entry chunk boundaries
     float* part_diag;        // (kv1 / 64 + 1) x 2 x 27
     float* part_pair;        // (kv2 / 64 + 1) x 2 x 36

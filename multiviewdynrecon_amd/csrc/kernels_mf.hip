@@ -351,6 +351,36 @@ __device__ inline int knn_term_rows(const Buffers& b, const FrameDev& f, const D
     return 3;
 }
 
+// The weight of each Node2Term entry's node, fixed by the index build: the
+// slot weight of node i in the term's knn, its sign bit set when the term
+// names i in more than one slot (the rare entries whose apply needs the
+// multiplicity correction).  Graph-edge entries are not read through it.
+__global__ void __launch_bounds__(256) mf_entry_weights_kernel(const Buffers* __restrict__ bs) {
+    const Buffers b = frame_buffers(bs);
+    const FrameDev& f = *b.frame;
+    const Counts& c = *b.counts;
+    if (c.error) return;
+    const TermBounds tb(c);
+    const uint32_t sub = threadIdx.x % kRowLanes;
+    const uint32_t groups = gridDim.x * kRowsPerBlock;
+    for (uint32_t base = blockIdx.x * kRowsPerBlock; base < f.N; base += groups) {
+        const uint32_t i = base + threadIdx.x / kRowLanes;
+        if (i >= f.N) continue;
+        const uint32_t beg = b.n2t_offset[i], end = b.n2t_offset[i + 1];
+        for (uint32_t e = beg + sub; e < end; e += kRowLanes) {
+            uint32_t kk;
+            const int type = type_of(tb, b.n2t_index[e], kk);
+            if (type == kSmooth) continue;
+            ushort4 knn;
+            float4 wk;
+            term_knn(b, type, kk, knn, wk);
+            const uint32_t mult = (knn.x == i) + (knn.y == i) + (knn.z == i) + (knn.w == i);
+            const float wi = slot_weight(knn, wk, i);
+            b.n2t_w[e] = mult > 1 ? copysignf(fabsf(wi), -1.0f) : fabsf(wi);
+        }
+    }
+}
+
 template <typename Acc, typename R, int kSrc, int kOut>
 __global__ void __launch_bounds__(256) mfr_terms_kernel(const Buffers* __restrict__ bs, uint32_t k, uint32_t cur, Penalty pen,
                                                         uint32_t which) {
@@ -510,15 +540,16 @@ __global__ void __launch_bounds__(256) mfr_gather_kernel(const Buffers* __restri
                         }
                     } else {
                         const float4 g1 = b.mf_g[2 * (size_t)t + 1];
-                        ushort4 knn;
-                        float4 wk;
-                        term_knn(b, type, kk, knn, wk);
-                        const float wi = slot_weight(knn, wk, i);
-                        const uint32_t mult = (knn.x == i) + (knn.y == i) + (knn.z == i) + (knn.w == i);
+                        const float ws = b.n2t_w[e];
+                        const float wi = fabsf(ws);
                         const float gv[6] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y};
 #pragma unroll
                         for (int a = 0; a < 6; ++a) y[a] += (Acc)(wi * gv[a]);
-                        if (correct && mult > 1) {
+                        if (correct && signbit(ws)) {
+                            ushort4 knn;
+                            float4 wk;
+                            term_knn(b, type, kk, knn, wk);
+                            const uint32_t mult = (knn.x == i) + (knn.y == i) + (knn.z == i) + (knn.w == i);
                             float J[3][6], lam[3];
                             const int rows = knn_term_rows<R>(b, f, se3, type, kk, l2, density, J, lam);
                             for (int r = 0; r < rows; ++r) {
@@ -692,6 +723,10 @@ void launch_pcg_matrix_free(const Batch& B, uint32_t iters, uint32_t refine, boo
 void launch_mf_apply_once(const Batch& B, bool fp64, const Penalty& pen, bool recompute, uint32_t which,
                           hipStream_t stream) {
     apply_any(B, recompute, fp64, 0, 0, 0, pen, which, stream);
+}
+
+void launch_mf_entry_weights(const Batch& B, hipStream_t stream) {
+    hipLaunchKernelGGL(mf_entry_weights_kernel, dim3(row_group_grid(B.nodes), B.frames), dim3(256), 0, stream, B.dev);
 }
 
 // The term Jacobian at se3[which] with unit weights, recomputed from the maps:

@@ -28,6 +28,7 @@ void launch_pcg_matrix_free(const Batch& B, uint32_t iters, uint32_t refine, boo
 void launch_mf_apply_once(const Batch& B, bool fp64, const Penalty& pen, bool recompute, uint32_t which,
                           hipStream_t stream);
 void launch_debug_jacobian(const Batch& B, bool fp64, bool transpose, uint32_t which, hipStream_t stream);
+void launch_mf_entry_weights(const Batch& B, hipStream_t stream);
 void launch_spmv_once(const Batch& B, bool fp64, hipStream_t stream);
 void launch_pcg_update_once(const Batch& B, bool fp64, hipStream_t stream);
 uint32_t launch_cost(const Batch& B, const Penalty& pen, bool density_in_jtj, hipStream_t stream);
@@ -158,6 +159,7 @@ void enqueue_batch(const Batch& B, const mvdr_solver_config& c, hipStream_t stre
         launch_copy_se3(B, stream);
         launch_index_build(B, c.use_foreground != 0, c.use_cross_view != 0,
                            mf ? kIndexNodes : (fp64 ? kIndexGram : kIndexLists), stream);
+        if (c.jtj_mode == MVDR_JTJ_RECOMPUTE) launch_mf_entry_weights(B, stream);
     }
     for (uint32_t k = first_iter; k < first_iter + iters; ++k) {
         const Penalty pen = kernel_penalty(c, k);
@@ -412,7 +414,10 @@ static size_t layout_buffers(const mvdr_solver_config& c, Buffers& b, FrameDev*&
     const bool mf = matrix_free(c);
     if (mf) {
         A(b.mf_u, knn_terms + b.cap_edges);
-        if (c.jtj_mode == MVDR_JTJ_RECOMPUTE) A(b.mf_g, 2 * (knn_terms + b.cap_edges));
+        if (c.jtj_mode == MVDR_JTJ_RECOMPUTE) {
+            A(b.mf_g, 2 * (knn_terms + b.cap_edges));
+            A(b.n2t_w, b.cap_kv1);
+        }
     } else if (c.precision == MVDR_PRECISION_MIXED) {
         A(b.n2t_col, b.cap_kv1);
         A(b.row_part, (size_t)kRowPartners * c.max_nodes);
