@@ -433,21 +433,21 @@ struct Lambda2 {
           feature(p.feature * p.feature), cross(p.cross * p.cross) {}
 };
 
-// LDS rows of one staged entry, 16 words each: G_t values 0..15 (lo), values
-// 16..26 and zeros (hi), the weight row.  Word c of row r sits at
-// 16 r + (c ^ swz(r)), swz(r) = 2 ((r >> 1) & 7): a lane's float2 stores of
-// its own row and the product's column reads (4 rows x 16 columns per step)
-// both fall on distinct banks.
+// LDS rows of one staged entry, 16 words each: a G_t row (values 0..15 for
+// the first product, then values 16..26 and zeros for the second) and the
+// weight row.  Word c of row r sits at 16 r + (c ^ swz(r)),
+// swz(r) = 2 ((r >> 1) & 7): a lane's float2 stores of its own row and the
+// product's column reads (4 rows x 16 columns per step) both fall on distinct
+// banks.  The two halves of G_t share one buffer (8 KB of LDS per row wave).
 constexpr uint32_t kRowWords = 16;
 __device__ inline uint32_t swz(uint32_t r) { return 2 * ((r >> 1) & 7); }
 
 // G_t of `rows` Jacobian rows (world frame; moved to the node frame of g here)
-// with their lambda^2 and residuals, stored to the lane's LDS rows.  Row 0
-// always exists; rows 1, 2 are added when present.
+// with their lambda^2 and residuals.  Row 0 always exists; rows 1, 2 are added
+// when present.
 template <int kMaxRows>
-__device__ inline void store_gram(float J[3][6], const float res[3], const float lam2[3], int rows, const float4 g,
-                                  float* gl, float* gh) {
-    float G[kGramVals];
+__device__ inline void rows_gram(float J[3][6], const float res[3], const float lam2[3], int rows, const float4 g,
+                                 float (&G)[kGramVals]) {
     to_node_frame(J[0], g);
     {
         int o = 0;
@@ -473,14 +473,65 @@ __device__ inline void store_gram(float J[3][6], const float res[3], const float
             }
         }
     }
+}
+
+// G_t of a point term (graph edge, feature, cross view) in closed form.  Its
+// rows are h_r (q x e_r, e_r) in node-centred coordinates, q = p - g_i the
+// term's point relative to the node, with residuals rho_r.  With
+// H_r = lambda^2 h_r^2 and V_r = lambda^2 h_r rho_r:
+//   rotation block    [q]x H [q]x^T   (sum_r H_r (q x e_r)(q x e_r)^T)
+//   rotation / trans  [q]x H
+//   translation       diag(H)
+//   -J^T r            -(q x V, V)
+// q is formed as one difference of nearby points, so the lever arm keeps its
+// FP32 precision.
+__device__ inline void point_gram(float qx, float qy, float qz, float H0, float H1, float H2, float V0, float V1,
+                                  float V2, float (&G)[kGramVals]) {
+    const float xx = qx * qx, yy = qy * qy, zz = qz * qz;
+    G[0] = zz * H1 + yy * H2;  // M00
+    G[1] = -qx * qy * H2;      // M01
+    G[2] = -qx * qz * H1;      // M02
+    G[3] = 0.f;                // C00
+    G[4] = -qz * H1;           // C01
+    G[5] = qy * H2;            // C02
+    G[6] = zz * H0 + xx * H2;  // M11
+    G[7] = -qy * qz * H0;      // M12
+    G[8] = qz * H0;            // C10
+    G[9] = 0.f;                // C11
+    G[10] = -qx * H2;          // C12
+    G[11] = yy * H0 + xx * H1; // M22
+    G[12] = -qy * H0;          // C20
+    G[13] = qx * H1;           // C21
+    G[14] = 0.f;               // C22
+    G[15] = H0;
+    G[16] = 0.f;
+    G[17] = 0.f;
+    G[18] = H1;
+    G[19] = 0.f;
+    G[20] = H2;
+    G[21] = -(qy * V2 - qz * V1);
+    G[22] = -(qz * V0 - qx * V2);
+    G[23] = -(qx * V1 - qy * V0);
+    G[24] = -V0;
+    G[25] = -V1;
+    G[26] = -V2;
+}
+
+// One half of G_t into the lane's LDS row: values 0..15, or 16..26 and zeros.
+template <int kHalf>
+__device__ inline void store_half(const float (&G)[kGramVals], float* gl) {
     const uint32_t f = swz(threadIdx.x);
+    if (kHalf == 0) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) *reinterpret_cast<float2*>(gl + ((2 * q) ^ f)) = make_float2(G[2 * q], G[2 * q + 1]);
+        for (int q = 0; q < 8; ++q) *reinterpret_cast<float2*>(gl + ((2 * q) ^ f)) = make_float2(G[2 * q], G[2 * q + 1]);
+    } else {
 #pragma unroll
-    for (int q = 0; q < 5; ++q) *reinterpret_cast<float2*>(gh + ((2 * q) ^ f)) = make_float2(G[16 + 2 * q], G[17 + 2 * q]);
-    *reinterpret_cast<float2*>(gh + (10 ^ f)) = make_float2(G[26], 0.f);
+        for (int q = 0; q < 5; ++q)
+            *reinterpret_cast<float2*>(gl + ((2 * q) ^ f)) = make_float2(G[16 + 2 * q], G[17 + 2 * q]);
+        *reinterpret_cast<float2*>(gl + (10 ^ f)) = make_float2(G[26], 0.f);
 #pragma unroll
-    for (int q = 6; q < 8; ++q) *reinterpret_cast<float2*>(gh + ((2 * q) ^ f)) = make_float2(0.f, 0.f);
+        for (int q = 6; q < 8; ++q) *reinterpret_cast<float2*>(gl + ((2 * q) ^ f)) = make_float2(0.f, 0.f);
+    }
 }
 
 // Partner columns of row i held by the index or searched in the row's pair keys.
@@ -550,32 +601,32 @@ __device__ inline void store_knn_weights(const Buffers& b, const GramColumns& gc
     }
 }
 
-// One Node2Term entry of row i (term t, record raw) staged in LDS: its G_t
-// row and its weight row (zeros for an empty lane).
+// One Node2Term entry of row i (term t, record raw): its weight row stored to
+// LDS, its G_t returned in registers (zeros for an empty lane).
 __device__ inline void stage_entry(const Buffers& b, const TermBounds& c, const Lambda2& l2, bool density,
                                    const GramColumns& gc, uint32_t t, const RawRec& raw, uint32_t pc, uint32_t i,
-                                   const float4 g, float* gl, float* gh, float* wl, float* wdummy) {
-    // zero rows (swizzled positions: the lanes' stores stay on distinct banks)
+                                   const float4 g, float (&G)[kGramVals], float* wl, float* wdummy) {
+    // zero row (swizzled positions: the lanes' stores stay on distinct banks)
     const uint32_t fz = swz(threadIdx.x);
 #pragma unroll
     for (int q = 0; q < 8; ++q) *reinterpret_cast<float2*>(wl + ((2 * q) ^ fz)) = make_float2(0.f, 0.f);
-    if (t == 0xFFFFFFFFu) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) *reinterpret_cast<float2*>(gl + ((2 * q) ^ fz)) = make_float2(0.f, 0.f);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) *reinterpret_cast<float2*>(gh + ((2 * q) ^ fz)) = make_float2(0.f, 0.f);
-        return;
-    }
+    for (int q = 0; q < (int)kGramVals; ++q) G[q] = 0.f;
+    if (t == 0xFFFFFFFFu) return;
     uint32_t k;
     const int type = type_of(c, t, k);
-    float J[3][6], res[3], lam2[3];
     if (type == kSmooth) {
-        // graph edge: rows of node i or j, unit weight, no partner columns here
+        // graph edge (computeSmoothTermJacobian, geometry_icp_jacobian.cuh:323-343):
+        // rows of node i or j, unit weight, no partner columns here
         SmoothRec sm;
         __builtin_memcpy(&sm, &raw, sizeof(SmoothRec));
-        smooth_rows(sm, i == (sm.ij & 0xFFFF), J, res);
-        lam2[0] = lam2[1] = lam2[2] = l2.smooth;
-        store_gram<3>(J, res, lam2, 3, g, gl, gh);
+        const bool node_i = i == (sm.ij & 0xFFFF);
+        const float on = sm.valid ? l2.smooth : 0.f;
+        const float sv = node_i ? on : -on;
+        const float qx = (node_i ? sm.Ti_xj[0] : sm.Ti_xj[0] - sm.r[0]) - g.x;
+        const float qy = (node_i ? sm.Ti_xj[1] : sm.Ti_xj[1] - sm.r[1]) - g.y;
+        const float qz = (node_i ? sm.Ti_xj[2] : sm.Ti_xj[2] - sm.r[2]) - g.z;
+        point_gram(qx, qy, qz, on, on, on, sv * sm.r[0], sv * sm.r[1], sv * sm.r[2], G);
         if (gc.c0 == 0) {
             const uint32_t f = swz(threadIdx.x);
             wl[0 ^ f] = 1.f;
@@ -590,6 +641,7 @@ __device__ inline void stage_entry(const Buffers& b, const TermBounds& c, const 
         __builtin_memcpy(&rec, &raw, sizeof(ScalarRec));
         knn = rec.knn;
         w = rec.w;
+        float J[3][6], res[3], lam2[3];
 #pragma unroll
         for (int q = 0; q < 6; ++q) J[0][q] = rec.J[q];
         res[0] = rec.r;
@@ -603,25 +655,64 @@ __device__ inline void stage_entry(const Buffers& b, const TermBounds& c, const 
             lam2[1] = l2.density;
             rows = 2;
         }
-        store_gram<2>(J, res, lam2, rows, g, gl, gh);
+        rows_gram<2>(J, res, lam2, rows, g, G);
     } else {
+        // point-to-point rows (computePointToPointICPTermJacobian,
+        // geometry_icp_jacobian.cuh:91-117): h_r = huber(r_r), rho_r = h_r r_r
         P2PRec rec;
         __builtin_memcpy(&rec, &raw, sizeof(P2PRec));
         knn = rec.knn;
         w = rec.w;
-        p2p_rows(rec, J, res);
         // bit select: a select of two fields would become a load through a
         // selected address (and the struct a scratch array)
         const uint32_t mask = type == kFeature ? 0xFFFFFFFFu : 0u;
         const float l = __uint_as_float((__float_as_uint(l2.feature) & mask) | (__float_as_uint(l2.cross) & ~mask));
-        lam2[0] = lam2[1] = lam2[2] = l;
-        store_gram<3>(J, res, lam2, 3, g, gl, gh);
+        const float4 p = rec.warped;
+        const float r0 = p.x - rec.target.x, r1 = p.y - rec.target.y, r2 = p.z - rec.target.z;
+        const float h0 = huber(r0), h1 = huber(r1), h2 = huber(r2);
+        const float H0 = l * (h0 * h0), H1 = l * (h1 * h1), H2 = l * (h2 * h2);
+        point_gram(p.x - g.x, p.y - g.y, p.z - g.z, H0, H1, H2, H0 * r0, H1 * r1, H2 * r2, G);
     }
     store_knn_weights(b, gc, knn, w, i, pc, wl, wdummy);
 }
 
+// acc += G^T W over the staged entries: `steps` products of 4 entries, the
+// operands of lane (m, kq) at 64 u + roff[u & 3].
+__device__ inline void gram_product(const float* Gs, const float* Ws, const uint32_t (&roff)[4], uint32_t steps,
+                                    gram_f4& acc) {
+    uint32_t st = 0;
+    if (steps == 16) {  // a full step: every operand read is issued before the products
+        float a[16], bv[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const uint32_t o = 4 * kRowWords * u + roff[u & 3];
+            a[u] = Gs[o];
+            bv[u] = Ws[o];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], bv[u], acc, 0, 0, 0);
+        return;
+    }
+    for (; st + 4 <= steps; st += 4) {  // four steps' operands in flight at once
+        float a[4], bv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t o = 4 * kRowWords * (st + u) + roff[u];  // st is a multiple of 4
+            a[u] = Gs[o];
+            bv[u] = Ws[o];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], bv[u], acc, 0, 0, 0);
+    }
+    for (; st < steps; ++st) {
+        const uint32_t j = st & 3;
+        const uint32_t o = 4 * kRowWords * st + (j == 0 ? roff[0] : j == 1 ? roff[1] : j == 2 ? roff[2] : roff[3]);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(Gs[o], Ws[o], acc, 0, 0, 0);
+    }
+}
+
 // diag_only: the diagonal blocks and J^T r alone (the matrix-free solve).
-__global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict__ bs, Penalty pen, int diag_only) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) gram_rows_kernel(const Buffers* __restrict__ bs, Penalty pen, int diag_only) {
     const Buffers b = frame_buffers(bs);
     // The grid covers the batch's largest node count in one sweep, so this
     // workgroup's row is known before its frame's sizes arrive: the row's
@@ -634,14 +725,12 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error || i >= f.N) return;
-    __shared__ float Glo[64 * kRowWords];
-    __shared__ float Ghi[64 * kRowWords];
+    __shared__ float Gs[64 * kRowWords];
     __shared__ float Wl[64 * kRowWords];
     __shared__ float Wd[64];  // weight stores of slots outside the pass
     const uint32_t lane = threadIdx.x;
     const uint32_t m = lane & 15, kq = lane >> 4;
-    float* gl = Glo + kRowWords * lane;
-    float* gh = Ghi + kRowWords * lane;
+    float* gl = Gs + kRowWords * lane;
     float* wl = Wl + kRowWords * lane;
     float* wd = Wd + lane;
     // read offsets of lane (m, kq): row 4 u + kq, word m ^ swz(row); swz
@@ -670,53 +759,20 @@ __global__ void __launch_bounds__(64) gram_rows_kernel(const Buffers* __restrict
             uint32_t t_nxt = beg + 64 + lane < end ? b.n2t_index[beg + 64 + lane] : 0xFFFFFFFFu;
             uint32_t pc_nxt = cols_in && beg + 64 + lane < end ? b.n2t_col[beg + 64 + lane] : 0xFFFFFFFFu;
             for (uint32_t e0 = beg; e0 < end; e0 += 64) {
-                stage_entry(b, tb, l2, density, gc, t_cur, rec_cur, pc_cur, i, g, gl, gh, wl, wd);
+                float G[kGramVals];
+                stage_entry(b, tb, l2, density, gc, t_cur, rec_cur, pc_cur, i, g, G, wl, wd);
+                store_half<0>(G, gl);
                 const RawRec rec_nxt = load_record(b, tb, t_nxt);
                 const uint32_t enn = e0 + 128 + lane;
                 const uint32_t t_nn = enn < end ? b.n2t_index[enn] : 0xFFFFFFFFu;
                 const uint32_t pc_nn = cols_in && enn < end ? b.n2t_col[enn] : 0xFFFFFFFFu;
-                wave_lds_sync();
-                // acc += G^T W over 4 entries per step
                 const uint32_t steps = (min(end - e0, 64u) + 3) / 4;
-                uint32_t st = 0;
-                if (steps == 16) {  // a full step: every operand read is issued before the products
-                    float a0[16], a1[16], bv[16];
-#pragma unroll
-                    for (int u = 0; u < 16; ++u) {
-                        const uint32_t o = 4 * kRowWords * u + roff[u & 3];
-                        a0[u] = Glo[o];
-                        a1[u] = Ghi[o];
-                        bv[u] = Wl[o];
-                    }
-#pragma unroll
-                    for (int u = 0; u < 16; ++u) {
-                        lo = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], bv[u], lo, 0, 0, 0);
-                        hi = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], bv[u], hi, 0, 0, 0);
-                    }
-                    st = 16;
-                }
-                for (; st + 4 <= steps; st += 4) {  // four steps' operands in flight at once
-                    float a0[4], a1[4], bv[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const uint32_t o = 4 * kRowWords * (st + u) + roff[u];  // st is a multiple of 4
-                        a0[u] = Glo[o];
-                        a1[u] = Ghi[o];
-                        bv[u] = Wl[o];
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        lo = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], bv[u], lo, 0, 0, 0);
-                        hi = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], bv[u], hi, 0, 0, 0);
-                    }
-                }
-                for (; st < steps; ++st) {
-                    const uint32_t j = st & 3;
-                    const uint32_t o = 4 * kRowWords * st + (j == 0 ? roff[0] : j == 1 ? roff[1] : j == 2 ? roff[2] : roff[3]);
-                    const float a0 = Glo[o], a1 = Ghi[o], bv = Wl[o];
-                    lo = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bv, lo, 0, 0, 0);
-                    hi = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bv, hi, 0, 0, 0);
-                }
+                wave_lds_sync();
+                gram_product(Gs, Wl, roff, steps, lo);
+                wave_lds_sync();  // the G rows are rewritten with the second half
+                store_half<1>(G, gl);
+                wave_lds_sync();
+                gram_product(Gs, Wl, roff, steps, hi);
                 wave_lds_sync();  // LDS is rewritten by the next step
                 t_cur = t_nxt;
                 pc_cur = pc_nxt;
