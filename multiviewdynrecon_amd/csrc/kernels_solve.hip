@@ -398,16 +398,16 @@ constexpr uint16_t kNoNode = 0xFFFF;
 typedef float gram_f4 __attribute__((ext_vector_type(4)));
 
 // A term record as loaded: 64 B (dense, foreground, feature, cross) or 32 B
-// (graph edge, upper half zero).  Loading it apart from its use lets a row
-// fetch its next 64 entries while the current ones are on the matrix cores.
+// (graph edge; the upper half is not loaded and not read).  Loading it apart
+// from its use lets a row fetch its next 64 entries while the current ones are
+// on the matrix cores.  t is always a valid term: lanes past a row's end load
+// the row's last entry again (and stage it with zero weights).
 struct RawRec {
     uint4 q[4];
 };
 
 __device__ inline RawRec load_record(const Buffers& b, const TermBounds& c, uint32_t t) {
     RawRec r;
-    r.q[0] = r.q[1] = r.q[2] = r.q[3] = make_uint4(0, 0, 0, 0);
-    if (t == 0xFFFFFFFFu) return r;
     uint32_t k;
     const int type = type_of(c, t, k);
     const uint4* p = type == kSmooth ? reinterpret_cast<const uint4*>(b.smooth + k)
@@ -604,15 +604,12 @@ __device__ inline void store_knn_weights(const Buffers& b, const GramColumns& gc
 // One Node2Term entry of row i (term t, record raw): its weight row stored to
 // LDS, its G_t returned in registers (zeros for an empty lane).
 __device__ inline void stage_entry(const Buffers& b, const TermBounds& c, const Lambda2& l2, bool density,
-                                   const GramColumns& gc, uint32_t t, const RawRec& raw, uint32_t pc, uint32_t i,
-                                   const float4 g, float (&G)[kGramVals], float* wl, float* wdummy) {
+                                   const GramColumns& gc, uint32_t t, bool live, const RawRec& raw, uint32_t pc,
+                                   uint32_t i, const float4 g, float (&G)[kGramVals], float* wl, float* wdummy) {
     // zero row (swizzled positions: the lanes' stores stay on distinct banks)
     const uint32_t fz = swz(threadIdx.x);
 #pragma unroll
     for (int q = 0; q < 8; ++q) *reinterpret_cast<float2*>(wl + ((2 * q) ^ fz)) = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int q = 0; q < (int)kGramVals; ++q) G[q] = 0.f;
-    if (t == 0xFFFFFFFFu) return;
     uint32_t k;
     const int type = type_of(c, t, k);
     if (type == kSmooth) {
@@ -629,8 +626,9 @@ __device__ inline void stage_entry(const Buffers& b, const TermBounds& c, const 
         point_gram(qx, qy, qz, on, on, on, sv * sm.r[0], sv * sm.r[1], sv * sm.r[2], G);
         if (gc.c0 == 0) {
             const uint32_t f = swz(threadIdx.x);
-            wl[0 ^ f] = 1.f;
-            wl[1 ^ f] = 1.f;
+            const float one = live ? 1.f : 0.f;
+            wl[0 ^ f] = one;
+            wl[1 ^ f] = one;
         }
         return;
     }
@@ -640,7 +638,7 @@ __device__ inline void stage_entry(const Buffers& b, const TermBounds& c, const 
         ScalarRec rec;
         __builtin_memcpy(&rec, &raw, sizeof(ScalarRec));
         knn = rec.knn;
-        w = rec.w;
+        w = live ? rec.w : make_float4(0.f, 0.f, 0.f, 0.f);
         float J[3][6], res[3], lam2[3];
 #pragma unroll
         for (int q = 0; q < 6; ++q) J[0][q] = rec.J[q];
@@ -662,7 +660,7 @@ __device__ inline void stage_entry(const Buffers& b, const TermBounds& c, const 
         P2PRec rec;
         __builtin_memcpy(&rec, &raw, sizeof(P2PRec));
         knn = rec.knn;
-        w = rec.w;
+        w = live ? rec.w : make_float4(0.f, 0.f, 0.f, 0.f);
         // bit select: a select of two fields would become a load through a
         // selected address (and the struct a scratch array)
         const uint32_t mask = type == kFeature ? 0xFFFFFFFFu : 0u;
@@ -750,35 +748,41 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) gr
         for (uint32_t c0 = 0; c0 < cols; c0 += kGramCols) {
             const GramColumns gc{c0, ubeg, U, mode};
             gram_f4 lo = {0.f, 0.f, 0.f, 0.f}, hi = {0.f, 0.f, 0.f, 0.f};
-            // two-stage fetch: the records of the next step and the entries of
-            // the step after it are in flight while this step computes
-            uint32_t t_cur = beg + lane < end ? b.n2t_index[beg + lane] : 0xFFFFFFFFu;
-            const bool cols_in = mode == 1;
-            uint32_t pc_cur = cols_in && beg + lane < end ? b.n2t_col[beg + lane] : 0xFFFFFFFFu;
-            RawRec rec_cur = load_record(b, tb, t_cur);
-            uint32_t t_nxt = beg + 64 + lane < end ? b.n2t_index[beg + 64 + lane] : 0xFFFFFFFFu;
-            uint32_t pc_nxt = cols_in && beg + 64 + lane < end ? b.n2t_col[beg + 64 + lane] : 0xFFFFFFFFu;
-            for (uint32_t e0 = beg; e0 < end; e0 += 64) {
-                float G[kGramVals];
-                stage_entry(b, tb, l2, density, gc, t_cur, rec_cur, pc_cur, i, g, G, wl, wd);
-                store_half<0>(G, gl);
-                const RawRec rec_nxt = load_record(b, tb, t_nxt);
-                const uint32_t enn = e0 + 128 + lane;
-                const uint32_t t_nn = enn < end ? b.n2t_index[enn] : 0xFFFFFFFFu;
-                const uint32_t pc_nn = cols_in && enn < end ? b.n2t_col[enn] : 0xFFFFFFFFu;
-                const uint32_t steps = (min(end - e0, 64u) + 3) / 4;
-                wave_lds_sync();
-                gram_product(Gs, Wl, roff, steps, lo);
-                wave_lds_sync();  // the G rows are rewritten with the second half
-                store_half<1>(G, gl);
-                wave_lds_sync();
-                gram_product(Gs, Wl, roff, steps, hi);
-                wave_lds_sync();  // LDS is rewritten by the next step
-                t_cur = t_nxt;
-                pc_cur = pc_nxt;
-                rec_cur = rec_nxt;
-                t_nxt = t_nn;
-                pc_nxt = pc_nn;
+            if (end > beg) {
+                // two-stage fetch: the records of the next step and the entries
+                // of the step after it are in flight while this step computes.
+                // Entry indices past the row's end are clamped to its last entry.
+                const uint32_t last = end - 1;
+                const bool cols_in = mode == 1;
+                uint32_t e = min(beg + lane, last);
+                uint32_t t_cur = b.n2t_index[e];
+                uint32_t pc_cur = cols_in ? b.n2t_col[e] : 0xFFFFFFFFu;
+                RawRec rec_cur = load_record(b, tb, t_cur);
+                e = min(beg + 64 + lane, last);
+                uint32_t t_nxt = b.n2t_index[e];
+                uint32_t pc_nxt = cols_in ? b.n2t_col[e] : 0xFFFFFFFFu;
+                for (uint32_t e0 = beg; e0 < end; e0 += 64) {
+                    float G[kGramVals];
+                    stage_entry(b, tb, l2, density, gc, t_cur, e0 + lane < end, rec_cur, pc_cur, i, g, G, wl, wd);
+                    store_half<0>(G, gl);
+                    const RawRec rec_nxt = load_record(b, tb, t_nxt);
+                    const uint32_t enn = min(e0 + 128 + lane, last);
+                    const uint32_t t_nn = b.n2t_index[enn];
+                    const uint32_t pc_nn = cols_in ? b.n2t_col[enn] : 0xFFFFFFFFu;
+                    const uint32_t steps = (min(end - e0, 64u) + 3) / 4;
+                    wave_lds_sync();
+                    gram_product(Gs, Wl, roff, steps, lo);
+                    wave_lds_sync();  // the G rows are rewritten with the second half
+                    store_half<1>(G, gl);
+                    wave_lds_sync();
+                    gram_product(Gs, Wl, roff, steps, hi);
+                    wave_lds_sync();  // LDS is rewritten by the next step
+                    t_cur = t_nxt;
+                    pc_cur = pc_nxt;
+                    rec_cur = rec_nxt;
+                    t_nxt = t_nn;
+                    pc_nxt = pc_nn;
+                }
             }
             // lane (m, kq) holds values 4 kq + r (lo) and 16 + 4 kq + r (hi) of column c0 + m
             const uint32_t n = c0 + m;

@@ -39,7 +39,10 @@ __device__ inline int type_of(const Counts& c, uint32_t t, uint32_t& local) {
 // the sum 1*(n==y) + 2*(n==z) + 3*(n==w).
 __device__ inline float slot_weight(ushort4 k, float4 w, uint32_t n) {
     const int s = 1 * (n == k.y) + 2 * (n == k.z) + 3 * (n == k.w);
-    return s == 0 ? w.x : (s == 1 ? w.y : (s == 2 ? w.z : w.w));
+    float r = w.x;  // selects, not branches
+    r = s == 1 ? w.y : r;
+    r = s == 2 ? w.z : r;
+    return s >= 3 ? w.w : r;
 }
 
 // Huber-weighted point-to-point Jacobian rows and residuals
