@@ -674,6 +674,45 @@ __device__ inline void stage_entry(const Buffers& b, const TermBounds& c, const 
     store_knn_weights(b, gc, knn, w, i, pc, wl, wdummy);
 }
 
+// A step whose entries are all dense terms (the bulk of every row), partner
+// columns from the index, no density rows: no type dispatch.  Same values as
+// stage_entry.
+__device__ inline void stage_dense(const Buffers& b, const TermBounds& c, const Lambda2& l2, const GramColumns& gc,
+                                   uint32_t t, bool live, const RawRec& raw, uint32_t pc, uint32_t i, const float4 g,
+                                   float (&G)[kGramVals], float* wl, float* wdummy) {
+    const uint32_t f = swz(threadIdx.x);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) *reinterpret_cast<float2*>(wl + ((2 * q) ^ f)) = make_float2(0.f, 0.f);
+    ScalarRec rec;
+    __builtin_memcpy(&rec, &raw, sizeof(ScalarRec));
+    float J[3][6], res[3], lam2[3] = {1.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) J[0][q] = rec.J[q];
+    res[0] = rec.r;
+    rows_gram<1>(J, res, lam2, 1, g, G);
+    const ushort4 knn = rec.knn;
+    const float4 w = live ? rec.w : make_float4(0.f, 0.f, 0.f, 0.f);
+    const bool distinct = knn.x != knn.y && knn.x != knn.z && knn.x != knn.w && knn.y != knn.z && knn.y != knn.w &&
+                          knn.z != knn.w;
+    if (!distinct) {  // repeated knn nodes: the general rule
+        store_knn_weights(b, gc, knn, w, i, pc, wl, wdummy);
+        return;
+    }
+    const float wi = slot_weight(knn, w, i);
+    if (gc.c0 == 0) {
+        wl[0 ^ f] = wi * wi;
+        wl[1 ^ f] = wi;
+    }
+    // distinct knn: slot q holds knn[q] with weight w[q]; its column is 0xFF
+    // unless knn[q] is an upper partner of i
+    const float wq[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t r = ((pc >> (8 * q)) & 0xFFu) - gc.c0;
+        if (r < kGramCols) wl[r ^ f] = wi * wq[q];
+    }
+}
+
 // acc += G^T W over the staged entries: `steps` products of 4 entries, the
 // operands of lane (m, kq) at 64 u + roff[u & 3].
 __device__ inline void gram_product(const float* Gs, const float* Ws, const uint32_t (&roff)[4], uint32_t steps,
@@ -709,24 +748,68 @@ __device__ inline void gram_product(const float* Gs, const float* Ws, const uint
     }
 }
 
+// The columns of one finished pass of row i: lane (m, kq) holds values
+// 4 kq + r (lo) and 16 + 4 kq + r (hi) of column c0 + m.
+__device__ inline void store_columns(const Buffers& b, uint32_t i, uint32_t c0, uint32_t cols, uint32_t ubeg,
+                                     const gram_f4& lo, const gram_f4& hi, uint32_t m, uint32_t kq) {
+    const uint32_t n = c0 + m;
+    if (n == 0) {
+        float* D = b.diag + 36 * (size_t)i;  // 32 values; the fix-up reads the first 21
+        *reinterpret_cast<float4*>(D + 4 * kq) = make_float4(lo[0], lo[1], lo[2], lo[3]);
+        *reinterpret_cast<float4*>(D + 16 + 4 * kq) = make_float4(hi[0], hi[1], hi[2], hi[3]);
+    } else if (n == 1) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint32_t q = 16 + 4 * kq + r;
+            if (q >= 21 && q < kGramVals) b.jtr[6 * (size_t)i + q - 21] = hi[r];
+        }
+    } else if (n < cols) {
+        float* S = b.offdiag + 36 * (size_t)(ubeg + n - 2);  // 24 values; the fix-up reads 21
+        *reinterpret_cast<float4*>(S + 4 * kq) = make_float4(lo[0], lo[1], lo[2], lo[3]);
+        if (kq < 2) *reinterpret_cast<float4*>(S + 16 + 4 * kq) = make_float4(hi[0], hi[1], hi[2], hi[3]);
+    }
+}
+
+__device__ inline uint32_t lane_u32(uint32_t v, uint32_t k) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)k);
+}
+__device__ inline float lane_f32(float v, uint32_t k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)k));
+}
+
+// Rows per row wave.  A wave runs the Node2Term entries of kGramRows
+// consecutive rows (and of each row's column passes) as one stream of
+// 64-entry steps: the entries and term records of a row's first step are
+// fetched while the previous row's last step computes, so a row start costs
+// no dependent loads of its own.
+constexpr uint32_t kGramRows = 2;
+
+// Position in a wave's stream: row k (of the wave), column pass c0, step e0.
+struct GramCursor {
+    uint32_t k, c0, e0;
+};
+
 // diag_only: the diagonal blocks and J^T r alone (the matrix-free solve).
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) gram_rows_kernel(const Buffers* __restrict__ bs, Penalty pen, int diag_only) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+gram_rows_kernel(const Buffers* __restrict__ bs, Penalty pen, int diag_only) {
     const Buffers b = frame_buffers(bs);
+    const uint32_t lane = threadIdx.x;
     // The grid covers the batch's largest node count in one sweep, so this
-    // workgroup's row is known before its frame's sizes arrive: the row's
-    // offsets load alongside them (clamped to the capacity; a row past the
-    // frame's node count exits below).
-    const uint32_t i = xcd_remap(blockIdx.x, gridDim.x, gridDim.x);
-    const uint32_t is = min(i, b.cap_nodes - 1);
-    const uint32_t beg = b.n2t_offset[is], end = b.n2t_offset[is + 1];
-    const uint32_t pbeg = diag_only ? 0 : b.pair_offset[is], pend = diag_only ? 0 : b.pair_offset[is + 1];
+    // wave's rows are known before its frame's sizes arrive: their offsets
+    // load alongside them, lane k holding row r0 + k's (clamped to the
+    // capacity; rows past the frame's node count are not used).
+    const uint32_t r0 = xcd_remap(blockIdx.x, gridDim.x, gridDim.x) * kGramRows;
+    const uint32_t ol = min(r0 + lane, b.cap_nodes);
+    const uint32_t off_l = b.n2t_offset[ol];
+    const uint32_t poff_l = diag_only ? 0u : b.pair_offset[ol];
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
-    if (c.error || i >= f.N) return;
+    if (c.error || r0 >= f.N) return;
+    const uint32_t nrows = min(kGramRows, f.N - r0);
+    const float4 g_l = f.nodes[min(r0 + lane, f.N - 1)];
     __shared__ float Gs[64 * kRowWords];
     __shared__ float Wl[64 * kRowWords];
     __shared__ float Wd[64];  // weight stores of slots outside the pass
-    const uint32_t lane = threadIdx.x;
     const uint32_t m = lane & 15, kq = lane >> 4;
     float* gl = Gs + kRowWords * lane;
     float* wl = Wl + kRowWords * lane;
@@ -739,69 +822,110 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) gr
     const Lambda2 l2(pen);
     const bool density = b.density && pen.density != 0.f;
     const TermBounds tb(c);
-    {
-        const uint32_t ubeg = pbeg, U = pend - pbeg;
-        const float4 g = f.nodes[i];
-        const uint32_t cols = 2 + U;
+    const gram_f4 zero4 = {0.f, 0.f, 0.f, 0.f};
+
+    // rows without entries: zero diagonal and J^T r (no pairs)
+    for (uint32_t k = 0; k < nrows; ++k)
+        if (lane_u32(off_l, k) == lane_u32(off_l, k + 1)) store_columns(b, r0 + k, 0, 2, 0, zero4, zero4, m, kq);
+
+    // the stream: every (row, pass) of the wave's rows that has entries
+    auto row_beg = [&](uint32_t k) { return lane_u32(off_l, k); };
+    auto row_end = [&](uint32_t k) { return lane_u32(off_l, k + 1); };
+    auto row_cols = [&](uint32_t k) { return 2 + lane_u32(poff_l, k + 1) - lane_u32(poff_l, k); };
+    auto skip_empty = [&](GramCursor& q) {
+        while (q.k < nrows && row_beg(q.k) == row_end(q.k)) ++q.k;
+        q.c0 = 0;
+        q.e0 = q.k < nrows ? row_beg(q.k) : 0;
+    };
+    auto advance = [&](GramCursor q) {
+        q.e0 += 64;
+        if (q.e0 < row_end(q.k)) return q;
+        q.c0 += kGramCols;
+        if (q.c0 < row_cols(q.k)) {
+            q.e0 = row_beg(q.k);
+            return q;
+        }
+        ++q.k;
+        skip_empty(q);
+        return q;
+    };
+    GramCursor cur{0, 0, 0};
+    skip_empty(cur);
+    if (cur.k >= nrows) return;
+    // two-stage fetch: the records of the next step and the entries of the step
+    // after it are in flight while a step computes.  Entry indices past a row's
+    // end are clamped to its last entry (staged with zero weights).
+    uint32_t e = min(cur.e0 + lane, row_end(cur.k) - 1);
+    // n2t_col exists only with partner columns (it is not allocated for the
+    // matrix-free solve, which assembles the diagonal alone)
+    const bool cols_in = !diag_only;
+    uint32_t t_cur = b.n2t_index[e];
+    uint32_t pc_cur = cols_in ? b.n2t_col[e] : 0xFFFFFFFFu;
+    RawRec rec_cur = load_record(b, tb, t_cur);
+    GramCursor nxt = advance(cur);
+    uint32_t t_nxt = t_cur, pc_nxt = pc_cur;
+    if (nxt.k < nrows) {
+        e = min(nxt.e0 + lane, row_end(nxt.k) - 1);
+        t_nxt = b.n2t_index[e];
+        pc_nxt = cols_in ? b.n2t_col[e] : 0xFFFFFFFFu;
+    }
+    gram_f4 lo = zero4, hi = zero4;
+    while (cur.k < nrows) {
+        const uint32_t k = cur.k, i = r0 + k;
+        const uint32_t end = row_end(k), cols = diag_only ? 2u : row_cols(k);
+        const uint32_t ubeg = lane_u32(poff_l, k);
         // partner columns held by n2t_col while they fit its u8 columns
         const int mode = diag_only ? 0 : (cols < 0xFF ? 1 : 2);
-        for (uint32_t c0 = 0; c0 < cols; c0 += kGramCols) {
-            const GramColumns gc{c0, ubeg, U, mode};
-            gram_f4 lo = {0.f, 0.f, 0.f, 0.f}, hi = {0.f, 0.f, 0.f, 0.f};
-            if (end > beg) {
-                // two-stage fetch: the records of the next step and the entries
-                // of the step after it are in flight while this step computes.
-                // Entry indices past the row's end are clamped to its last entry.
-                const uint32_t last = end - 1;
-                const bool cols_in = mode == 1;
-                uint32_t e = min(beg + lane, last);
-                uint32_t t_cur = b.n2t_index[e];
-                uint32_t pc_cur = cols_in ? b.n2t_col[e] : 0xFFFFFFFFu;
-                RawRec rec_cur = load_record(b, tb, t_cur);
-                e = min(beg + 64 + lane, last);
-                uint32_t t_nxt = b.n2t_index[e];
-                uint32_t pc_nxt = cols_in ? b.n2t_col[e] : 0xFFFFFFFFu;
-                for (uint32_t e0 = beg; e0 < end; e0 += 64) {
-                    float G[kGramVals];
-                    stage_entry(b, tb, l2, density, gc, t_cur, e0 + lane < end, rec_cur, pc_cur, i, g, G, wl, wd);
-                    store_half<0>(G, gl);
-                    const RawRec rec_nxt = load_record(b, tb, t_nxt);
-                    const uint32_t enn = min(e0 + 128 + lane, last);
-                    const uint32_t t_nn = b.n2t_index[enn];
-                    const uint32_t pc_nn = cols_in ? b.n2t_col[enn] : 0xFFFFFFFFu;
-                    const uint32_t steps = (min(end - e0, 64u) + 3) / 4;
-                    wave_lds_sync();
-                    gram_product(Gs, Wl, roff, steps, lo);
-                    wave_lds_sync();  // the G rows are rewritten with the second half
-                    store_half<1>(G, gl);
-                    wave_lds_sync();
-                    gram_product(Gs, Wl, roff, steps, hi);
-                    wave_lds_sync();  // LDS is rewritten by the next step
-                    t_cur = t_nxt;
-                    pc_cur = pc_nxt;
-                    rec_cur = rec_nxt;
-                    t_nxt = t_nn;
-                    pc_nxt = pc_nn;
-                }
-            }
-            // lane (m, kq) holds values 4 kq + r (lo) and 16 + 4 kq + r (hi) of column c0 + m
-            const uint32_t n = c0 + m;
-            if (n == 0) {
-                float* D = b.diag + 36 * (size_t)i;  // 32 values; the fix-up reads the first 21
-                *reinterpret_cast<float4*>(D + 4 * kq) = make_float4(lo[0], lo[1], lo[2], lo[3]);
-                *reinterpret_cast<float4*>(D + 16 + 4 * kq) = make_float4(hi[0], hi[1], hi[2], hi[3]);
-            } else if (n == 1) {
+        const GramColumns gc{cur.c0, ubeg, cols - 2, mode};
+        const float4 g = make_float4(lane_f32(g_l.x, k), lane_f32(g_l.y, k), lane_f32(g_l.z, k), 0.f);
+        const uint32_t pc = mode == 1 ? pc_cur : 0xFFFFFFFFu;
+        const bool live = cur.e0 + lane < end;
+        float G[kGramVals];
+        // entries ascend by term within a row and lane 63 holds the step's
+        // last one: the step is all dense when that one is
+        const bool dense_step = mode == 1 && !density && lane_u32(t_cur, 63) < tb.o0;
+        if (dense_step)
+            stage_dense(b, tb, l2, gc, t_cur, live, rec_cur, pc, i, g, G, wl, wd);
+        else
+            stage_entry(b, tb, l2, density, gc, t_cur, live, rec_cur, pc, i, g, G, wl, wd);
+        store_half<0>(G, gl);
+        const GramCursor nn = advance(nxt);
+        RawRec rec_nxt = rec_cur;
+        if (nxt.k < nrows) {
+            if (lane_u32(t_nxt, 63) < tb.o0) {
+                const uint4* rp = reinterpret_cast<const uint4*>(b.dense + t_nxt);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const uint32_t q = 16 + 4 * kq + r;
-                    if (q >= 21 && q < kGramVals) b.jtr[6 * (size_t)i + q - 21] = hi[r];
-                }
-            } else if (n < cols) {
-                float* S = b.offdiag + 36 * (size_t)(ubeg + n - 2);  // 24 values; the fix-up reads 21
-                *reinterpret_cast<float4*>(S + 4 * kq) = make_float4(lo[0], lo[1], lo[2], lo[3]);
-                if (kq < 2) *reinterpret_cast<float4*>(S + 16 + 4 * kq) = make_float4(hi[0], hi[1], hi[2], hi[3]);
+                for (int q = 0; q < 4; ++q) rec_nxt.q[q] = rp[q];
+            } else {
+                rec_nxt = load_record(b, tb, t_nxt);
             }
         }
+        uint32_t t_nn = t_nxt, pc_nn = pc_nxt;
+        if (nn.k < nrows) {
+            const uint32_t enn = min(nn.e0 + lane, row_end(nn.k) - 1);
+            t_nn = b.n2t_index[enn];
+            pc_nn = cols_in ? b.n2t_col[enn] : 0xFFFFFFFFu;
+        }
+        const uint32_t steps = (min(end - cur.e0, 64u) + 3) / 4;
+        wave_lds_sync();
+        gram_product(Gs, Wl, roff, steps, lo);
+        wave_lds_sync();  // the G rows are rewritten with the second half
+        store_half<1>(G, gl);
+        wave_lds_sync();
+        gram_product(Gs, Wl, roff, steps, hi);
+        wave_lds_sync();  // LDS is rewritten by the next step
+        if (nxt.k != k || nxt.c0 != cur.c0) {  // the pass ends with this step
+            store_columns(b, i, cur.c0, cols, ubeg, lo, hi, m, kq);
+            lo = zero4;
+            hi = zero4;
+        }
+        cur = nxt;
+        nxt = nn;
+        t_cur = t_nxt;
+        pc_cur = pc_nxt;
+        rec_cur = rec_nxt;
+        t_nxt = t_nn;
+        pc_nxt = pc_nn;
     }
 }
 
@@ -910,8 +1034,8 @@ void launch_eval_terms(const Batch& B, uint32_t which, bool fp64_geometry, hipSt
 void launch_assemble(const Batch& B, const Penalty& pen, bool gram, bool diag_only, hipStream_t stream) {
     const uint32_t fix_blocks = std::min<uint32_t>((8 * B.nodes + 255) / 256, 1024u);  // grid-stride over N + P
     if (gram) {
-        const uint32_t rows = (B.nodes + kXcds - 1) / kXcds * kXcds;
-        hipLaunchKernelGGL(gram_rows_kernel, dim3(rows, B.frames), dim3(64), 0, stream, B.dev, pen, diag_only ? 1 : 0);
+        const uint32_t waves = ((B.nodes + kGramRows - 1) / kGramRows + kXcds - 1) / kXcds * kXcds;
+        hipLaunchKernelGGL(gram_rows_kernel, dim3(waves, B.frames), dim3(64), 0, stream, B.dev, pen, diag_only ? 1 : 0);
         hipLaunchKernelGGL(gram_fixup_kernel, dim3((B.nodes + 63) / 64, B.frames), dim3(64), 0, stream, B.dev,
                            diag_only ? 1 : 0, diag_only ? 1 : 0);
     } else {
