@@ -287,6 +287,17 @@ const char* mvdr_batch_last_error(const mvdr_batch* batch);
 mvdr_status mvdr_batch_solve(mvdr_batch* batch, void* stream);
 /* mvdr_solver_bench_stage over all members at once (one launch per rep). */
 mvdr_status mvdr_batch_bench_stage(mvdr_batch* batch, int stage, uint32_t reps, void* stream, float* avg_ms);
+/* Solve lanes (no counterpart in the reference): the members are split into
+ * `lanes` contiguous runs of near-equal size, each solved by its own chain of
+ * launches (and captured graph) on its own stream, forked from and joined
+ * back to the stream given to mvdr_batch_solve, so that one lane's
+ * latency-bound CG launches overlap another's evaluation and assembly.  Every
+ * member's result is the same for any lane count.  A new batch has two lanes
+ * from 8 members on, else one.  1 <= lanes <= min(members,
+ * MVDR_MAX_BATCH_LANES); waits for the batch's last solve. */
+#define MVDR_MAX_BATCH_LANES 4
+mvdr_status mvdr_batch_set_lanes(mvdr_batch* batch, uint32_t lanes);
+uint32_t mvdr_batch_lanes(const mvdr_batch* batch);
 
 /* ------------------------------------------------------- launch-shape tuning
  * No counterpart in the reference (its launches use fixed block sizes).  The

@@ -79,22 +79,54 @@ struct mvdr_solver {
     }
 };
 
+// One solve lane of a batch: a contiguous run of members whose launches form
+// their own chain (and captured graph) on their own stream.  Lanes run
+// concurrently, so one lane's latency-bound CG launches overlap another's
+// evaluation and assembly.  Every kernel indexes its frame, so a member's
+// result does not depend on the lane it runs in.
+struct BatchLane {
+    Batch batch{};
+    uint32_t first = 0, count = 0;  // members [first, first + count)
+    uint32_t graph_nodes = 0;
+    uint64_t graph_table = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t stream = nullptr;  // lanes 1..: the lane's own stream (lane 0 uses the caller's)
+    hipEvent_t joined = nullptr;   // lanes 1..: recorded on the lane's stream after its solve
+};
+
 struct mvdr_batch {
     std::vector<mvdr_solver*> members;
     mvdr_solver_config cfg{};
     Buffers* dev = nullptr;             // the members' Buffers, one per frame
-    Batch batch{};
-    uint32_t graph_nodes = 0;
-    uint64_t graph_table = 0;
+    Batch batch{};                      // all members (stage timing)
+    std::vector<BatchLane> lanes;
+    hipEvent_t fork = nullptr;          // recorded on the caller's stream before the lanes start
     hipEvent_t done = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
     std::string error;
 
+    void drop_graphs() {
+        for (BatchLane& l : lanes) {
+            if (l.graph_exec) (void)hipGraphExecDestroy(l.graph_exec);
+            if (l.graph) (void)hipGraphDestroy(l.graph);
+            l.graph_exec = nullptr;
+            l.graph = nullptr;
+        }
+    }
+    void drop_lane_streams() {
+        for (BatchLane& l : lanes) {
+            if (l.stream) (void)hipStreamSynchronize(l.stream);
+            if (l.joined) (void)hipEventDestroy(l.joined);
+            if (l.stream) (void)hipStreamDestroy(l.stream);
+            l.joined = nullptr;
+            l.stream = nullptr;
+        }
+    }
     ~mvdr_batch() {
         if (done) (void)hipEventSynchronize(done);
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (graph) (void)hipGraphDestroy(graph);
+        drop_graphs();
+        drop_lane_streams();
+        if (fork) (void)hipEventDestroy(fork);
         if (done) (void)hipEventDestroy(done);
         if (dev) (void)hipFree(dev);
     }
@@ -891,6 +923,44 @@ static bool same_schedule(const mvdr_solver_config& a, const mvdr_solver_config&
            a.precision == b.precision && a.jtj_mode == b.jtj_mode && a.refine_steps == b.refine_steps;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Lanes of a new batch: two from 8 members on (each lane then still fills the
+// device in its evaluation and assembly launches), else one.
+uint32_t default_lanes(uint32_t members) { return members >= 8 ? 2u : 1u; }
+
+// Splits the members into `n` contiguous lanes of near-equal size, each with
+// its own stream and join event (lane 0 runs on the caller's stream).
+hipError_t make_lanes(mvdr_batch* bt, uint32_t n) {
+    bt->drop_graphs();
+    bt->drop_lane_streams();
+    const uint32_t count = (uint32_t)bt->members.size();
+    bt->lanes.assign(n, BatchLane{});
+    hipError_t e = hipSuccess;
+    for (uint32_t l = 0; l < n; ++l) {
+        BatchLane& lane = bt->lanes[l];
+        lane.first = (uint32_t)((uint64_t)count * l / n);
+        lane.count = (uint32_t)((uint64_t)count * (l + 1) / n) - lane.first;
+        Batch B{bt->dev + lane.first, lane.count, 0, 64, 0};
+        for (uint32_t k = lane.first; k < lane.first + lane.count; ++k) {
+            B.cap_tiles = std::max(B.cap_tiles, bt->members[k]->self.cap_tiles);
+            B.cap_chunks = std::max(B.cap_chunks, bt->members[k]->self.cap_chunks);
+        }
+        lane.batch = B;
+        if (l > 0 && e == hipSuccess) {
+            e = hipStreamCreateWithFlags(&lane.stream, hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&lane.joined, hipEventDisableTiming);
+        }
+    }
+    return e;
+}
+
+}  // namespace
+
+extern "C" {
+
 mvdr_status mvdr_batch_create(mvdr_solver* const* solvers, uint32_t count, mvdr_batch** out) {
     if (!out) return MVDR_E_INVALID;
     *out = nullptr;
@@ -915,16 +985,34 @@ mvdr_status mvdr_batch_create(mvdr_solver* const* solvers, uint32_t count, mvdr_
     hipError_t e = hipMalloc(&bt->dev, sizeof(Buffers) * count);
     if (e == hipSuccess) e = hipMemcpy(bt->dev, host.data(), sizeof(Buffers) * count, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->fork, hipEventDisableTiming);
+    if (e == hipSuccess) {
+        B.dev = bt->dev;
+        bt->batch = B;
+        e = make_lanes(bt, default_lanes(count));
+    }
     if (e != hipSuccess) {
         std::fprintf(stderr, "mvdr_batch_create: %s\n", hipGetErrorString(e));
         delete bt;
         return MVDR_E_HIP;
     }
-    B.dev = bt->dev;
-    bt->batch = B;
     *out = bt;
     return MVDR_OK;
 }
+
+mvdr_status mvdr_batch_set_lanes(mvdr_batch* bt, uint32_t lanes) {
+    if (!bt || lanes == 0 || lanes > MVDR_MAX_BATCH_LANES || lanes > bt->members.size()) return MVDR_E_INVALID;
+    if (lanes == bt->lanes.size()) return MVDR_OK;
+    (void)hipEventSynchronize(bt->done);
+    const hipError_t e = make_lanes(bt, lanes);
+    if (e != hipSuccess) {
+        bt->error = std::string("HIP: ") + hipGetErrorString(e);
+        return MVDR_E_HIP;
+    }
+    return MVDR_OK;
+}
+
+uint32_t mvdr_batch_lanes(const mvdr_batch* bt) { return bt ? (uint32_t)bt->lanes.size() : 0u; }
 
 void mvdr_batch_destroy(mvdr_batch* bt) { delete bt; }
 
@@ -1046,7 +1134,18 @@ mvdr_status mvdr_batch_autotune(mvdr_batch* bt, uint32_t reps, void* stream, mvd
             return MVDR_E_STATE;
         }
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const mvdr_status rc = autotune_impl(bt->batch, bt->cfg, m0->iter - 1, reps, st, chosen, bt->error);
+    // the shapes the solve launches: one per lane (frames per lane, the lane's
+    // node count)
+    mvdr_status rc = MVDR_OK;
+    for (size_t l = 0; l < bt->lanes.size() && rc == MVDR_OK; ++l) {
+        Batch B = bt->lanes[l].batch;
+        uint32_t nodes = 64;
+        for (uint32_t k = bt->lanes[l].first; k < bt->lanes[l].first + bt->lanes[l].count; ++k)
+            nodes = std::max(nodes, live_nodes(bt->members[k]->frame_host.N));
+        B.nodes = nodes;
+        mvdr_launch_params got{};
+        rc = autotune_impl(B, bt->cfg, m0->iter - 1, reps, st, l == 0 ? chosen : &got, bt->error);
+    }
     if (rc == MVDR_OK)
         for (mvdr_solver* m : bt->members) (void)hipEventRecord(m->done, st);
     return rc;
@@ -1109,34 +1208,47 @@ mvdr_status mvdr_batch_solve(mvdr_batch* bt, void* stream) {
         }
     hipStream_t st = static_cast<hipStream_t>(stream);
     const uint32_t gn = bt->cfg.gn_iterations;
-    uint32_t nodes = 64;
-    for (const mvdr_solver* m : bt->members) nodes = std::max(nodes, live_nodes(m->frame_host.N));
-    bt->batch.nodes = nodes;
-    hipError_t e = hipSuccess;
-    if (bt->cfg.use_graph) {
-        if (bt->graph_exec && (bt->graph_nodes != nodes || bt->graph_table != launch_table_version())) {
-            (void)hipGraphExecDestroy(bt->graph_exec);
-            (void)hipGraphDestroy(bt->graph);
-            bt->graph_exec = nullptr;
-            bt->graph = nullptr;
-        }
-        if (!bt->graph_exec) {
-            bt->graph_nodes = nodes;
-            bt->graph_table = launch_table_version();
-            hipStream_t cap = nullptr;
-            e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
-            if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-            if (e == hipSuccess) {
-                enqueue_batch(bt->batch, bt->cfg, cap, true, 0, gn);
-                e = hipStreamEndCapture(cap, &bt->graph);
+    uint32_t all_nodes = 64;
+    for (const mvdr_solver* m : bt->members) all_nodes = std::max(all_nodes, live_nodes(m->frame_host.N));
+    bt->batch.nodes = all_nodes;
+    const bool forked = bt->lanes.size() > 1;
+    hipError_t e = forked ? hipEventRecord(bt->fork, st) : hipSuccess;
+    for (BatchLane& lane : bt->lanes) {
+        if (e != hipSuccess) break;
+        hipStream_t ls = lane.stream ? lane.stream : st;
+        if (lane.stream) e = hipStreamWaitEvent(ls, bt->fork, 0);
+        if (e != hipSuccess) break;
+        uint32_t nodes = 64;
+        for (uint32_t k = lane.first; k < lane.first + lane.count; ++k)
+            nodes = std::max(nodes, live_nodes(bt->members[k]->frame_host.N));
+        lane.batch.nodes = nodes;
+        if (bt->cfg.use_graph) {
+            if (lane.graph_exec && (lane.graph_nodes != nodes || lane.graph_table != launch_table_version())) {
+                (void)hipGraphExecDestroy(lane.graph_exec);
+                (void)hipGraphDestroy(lane.graph);
+                lane.graph_exec = nullptr;
+                lane.graph = nullptr;
             }
-            if (e == hipSuccess) e = hipGraphInstantiate(&bt->graph_exec, bt->graph, nullptr, nullptr, 0);
-            if (cap) (void)hipStreamDestroy(cap);
+            if (!lane.graph_exec) {
+                lane.graph_nodes = nodes;
+                lane.graph_table = launch_table_version();
+                hipStream_t cap = nullptr;
+                e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+                if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+                if (e == hipSuccess) {
+                    enqueue_batch(lane.batch, bt->cfg, cap, true, 0, gn);
+                    e = hipStreamEndCapture(cap, &lane.graph);
+                }
+                if (e == hipSuccess) e = hipGraphInstantiate(&lane.graph_exec, lane.graph, nullptr, nullptr, 0);
+                if (cap) (void)hipStreamDestroy(cap);
+            }
+            if (e == hipSuccess) e = hipGraphLaunch(lane.graph_exec, ls);
+        } else {
+            enqueue_batch(lane.batch, bt->cfg, ls, true, 0, gn);
+            e = hipGetLastError();
         }
-        if (e == hipSuccess) e = hipGraphLaunch(bt->graph_exec, st);
-    } else {
-        enqueue_batch(bt->batch, bt->cfg, st, true, 0, gn);
-        e = hipGetLastError();
+        if (e == hipSuccess && lane.stream) e = hipEventRecord(lane.joined, ls);
+        if (e == hipSuccess && lane.stream) e = hipStreamWaitEvent(st, lane.joined, 0);
     }
     if (e == hipSuccess) e = hipEventRecord(bt->done, st);
     for (mvdr_solver* m : bt->members) {

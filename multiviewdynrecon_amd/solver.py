@@ -77,6 +77,9 @@ def lib():
         LP = C.POINTER(_abi.LaunchParams)
         L.mvdr_solver_autotune.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, LP]
         L.mvdr_batch_autotune.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, LP]
+        L.mvdr_batch_set_lanes.argtypes = [C.c_void_p, C.c_uint32]
+        L.mvdr_batch_lanes.argtypes = [C.c_void_p]
+        L.mvdr_batch_lanes.restype = C.c_uint32
         L.mvdr_launch_params_get.argtypes = [C.c_uint32, C.c_uint32, LP]
         L.mvdr_launch_params_set.argtypes = [C.c_uint32, C.c_uint32, LP]
         L.mvdr_launch_table_save.argtypes = [C.c_char_p]
@@ -288,6 +291,14 @@ class NonRigidBatch:
         out = C.c_float()
         self._check(lib().mvdr_batch_bench_stage(self.h, stage, reps, _stream_handle(stream), C.byref(out)))
         return out.value
+
+    @property
+    def lanes(self):
+        """Solve lanes: runs of members solved concurrently on their own streams."""
+        return lib().mvdr_batch_lanes(self.h)
+
+    def set_lanes(self, lanes):
+        self._check(lib().mvdr_batch_set_lanes(self.h, lanes))
 
 
 # ------------------------------------------------------------- launch table

@@ -58,6 +58,43 @@ def test_batch_members_equal_single_solves(frames, precision, count):
             assert s.sizes() == sizes
 
 
+@pytest.mark.parametrize("graph", [0, 1])
+def test_batch_lanes_equal_single_solves(frames, graph):
+    """Lanes solve runs of members concurrently on their own streams; every
+    lane count gives each member its single-solve result bit for bit."""
+    cfg = shared_config("mixed", use_graph=graph)
+    count = 8
+    single = []
+    for fr in frames:
+        s = NonRigidSolver(cfg)
+        s.SetSolverInput(fr)
+        s.SolveNonRigidAlignment()
+        single.append((s.solved_node_se3_host(), s.debug_get("twist")))
+    members = []
+    for k in range(count):
+        s = NonRigidSolver(cfg)
+        s.SetSolverInput(frames[k % len(frames)])
+        members.append(s)
+    batch = NonRigidBatch(members)
+    assert batch.lanes == 2  # the default from 8 members on
+    for lanes in (1, 3, 4, 2):
+        batch.set_lanes(lanes)
+        assert batch.lanes == lanes
+        for rep in range(2):
+            batch.SolveNonRigidAlignment()
+            for k, s in enumerate(members):
+                se3, twist = single[k % len(frames)]
+                np.testing.assert_array_equal(s.solved_node_se3_host(), se3, err_msg=f"lanes {lanes} member {k}")
+                np.testing.assert_array_equal(s.debug_get("twist"), twist, err_msg=f"lanes {lanes} member {k}")
+    for bad in (0, 5, 9):
+        with pytest.raises(SolverError):
+            batch.set_lanes(bad)
+    small = NonRigidBatch(members[:3])
+    assert small.lanes == 1
+    with pytest.raises(SolverError):
+        small.set_lanes(4)
+
+
 def test_batch_graph_replays_match(frames):
     cfg = shared_config("mixed", use_graph=1)
     members = []
