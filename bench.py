@@ -264,7 +264,7 @@ def uneven_measure(batches, warmup, graph=1, solo_reps=5, autotune=True):
     batch = NonRigidBatch(solvers)
     print(f"[bench] {count} uneven scenes resident: nodes {sorted(nodes)}, views {views}", file=sys.stderr, flush=True)
     batch.SolveNonRigidAlignment()
-    tuned = batch.autotune(reps=10) if autotune else None
+    tuned = dict(batch.autotune(reps=10), lanes=batch.lanes) if autotune else None
     for _ in range(warmup):
         batch.SolveNonRigidAlignment()
     torch.cuda.synchronize()
@@ -325,7 +325,7 @@ def matrix_free_measure(frames, steps, autotune=True):
         solvers.append(s)
     batch = NonRigidBatch(solvers)
     batch.SolveNonRigidAlignment()
-    tuned = batch.autotune(reps=5) if autotune else None
+    tuned = dict(batch.autotune(reps=5), lanes=batch.lanes) if autotune else None
     batch.SolveNonRigidAlignment()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -392,7 +392,7 @@ def main():
     if not args.no_autotune:
         # launch shapes for this batch and for a lone frame, before any timing
         batch.SolveNonRigidAlignment()
-        tuned["batch"] = batch.autotune(reps=10)
+        tuned["batch"] = dict(batch.autotune(reps=10), lanes=batch.lanes)
         solvers[0].SolveNonRigidAlignment()
         tuned["single"] = solvers[0].autotune(reps=10)
         print(f"[bench] autotuned launch shapes: {json.dumps(tuned)}", file=sys.stderr, flush=True)

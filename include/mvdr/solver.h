@@ -319,7 +319,11 @@ typedef struct {
  * data of the last GN iteration of a completed solve, as bench_stage) and
  * keeps the fastest in the table for this solver's / batch's shape; `chosen`
  * (nullable) receives the resolved counts (0 for launches the config does not
- * use).  Synchronises `stream`. */
+ * use).  A batch also times reps whole solves (its members' inputs solved
+ * again, with the same results) for each lane count 1 .. min(members,
+ * MVDR_MAX_BATCH_LANES), with the shapes tuned for its lanes, and keeps the
+ * fastest (mvdr_batch_lanes reads it); `chosen` then describes lane 0.
+ * Synchronises `stream`. */
 mvdr_status mvdr_solver_autotune(mvdr_solver* solver, uint32_t reps, void* stream, mvdr_launch_params* chosen);
 mvdr_status mvdr_batch_autotune(mvdr_batch* batch, uint32_t reps, void* stream, mvdr_launch_params* chosen);
 /* The counts launches for `nodes` nodes and `frames` frames use. */

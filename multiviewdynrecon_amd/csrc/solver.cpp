@@ -1134,18 +1134,63 @@ mvdr_status mvdr_batch_autotune(mvdr_batch* bt, uint32_t reps, void* stream, mvd
             return MVDR_E_STATE;
         }
     hipStream_t st = static_cast<hipStream_t>(stream);
-    // the shapes the solve launches: one per lane (frames per lane, the lane's
-    // node count)
-    mvdr_status rc = MVDR_OK;
-    for (size_t l = 0; l < bt->lanes.size() && rc == MVDR_OK; ++l) {
-        Batch B = bt->lanes[l].batch;
-        uint32_t nodes = 64;
-        for (uint32_t k = bt->lanes[l].first; k < bt->lanes[l].first + bt->lanes[l].count; ++k)
-            nodes = std::max(nodes, live_nodes(bt->members[k]->frame_host.N));
-        B.nodes = nodes;
-        mvdr_launch_params got{};
-        rc = autotune_impl(B, bt->cfg, m0->iter - 1, reps, st, l == 0 ? chosen : &got, bt->error);
+    const uint32_t k = m0->iter - 1;
+    // launch shapes of the current lanes: one per lane (frames per lane, the
+    // lane's node count)
+    auto tune_shapes = [&](mvdr_launch_params* out) {
+        mvdr_status rc = MVDR_OK;
+        for (size_t l = 0; l < bt->lanes.size() && rc == MVDR_OK; ++l) {
+            Batch B = bt->lanes[l].batch;
+            uint32_t nodes = 64;
+            for (uint32_t q = bt->lanes[l].first; q < bt->lanes[l].first + bt->lanes[l].count; ++q)
+                nodes = std::max(nodes, live_nodes(bt->members[q]->frame_host.N));
+            B.nodes = nodes;
+            mvdr_launch_params got{};
+            rc = autotune_impl(B, bt->cfg, k, reps, st, l == 0 && out ? out : &got, bt->error);
+        }
+        return rc;
+    };
+    // then the lane count: whole solves (same inputs, same results) timed for
+    // each candidate with its tuned shapes
+    const uint32_t members = (uint32_t)bt->members.size();
+    uint32_t best = (uint32_t)bt->lanes.size();
+    if (members > 1) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipError_t e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        float best_ms = 0.f;
+        for (uint32_t L = 1; L <= std::min<uint32_t>(members, MVDR_MAX_BATCH_LANES) && e == hipSuccess; ++L) {
+            (void)hipEventSynchronize(bt->done);
+            e = make_lanes(bt, L);
+            if (e != hipSuccess) break;
+            mvdr_status rc = tune_shapes(nullptr);
+            if (rc == MVDR_OK) rc = mvdr_batch_solve(bt, st);  // captures the lanes' graphs
+            if (rc != MVDR_OK) {
+                if (e0) (void)hipEventDestroy(e0);
+                if (e1) (void)hipEventDestroy(e1);
+                return rc;
+            }
+            e = hipEventRecord(e0, st);
+            for (uint32_t r = 0; r < reps && e == hipSuccess; ++r)
+                if (mvdr_batch_solve(bt, st) != MVDR_OK) e = hipErrorUnknown;
+            if (e == hipSuccess) e = hipEventRecord(e1, st);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e == hipSuccess && (L == 1 || ms < best_ms)) {
+                best_ms = ms;
+                best = L;
+            }
+        }
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (e == hipSuccess) e = make_lanes(bt, best);
+        if (e != hipSuccess) {
+            bt->error = std::string("HIP: ") + hipGetErrorString(e);
+            return MVDR_E_HIP;
+        }
     }
+    const mvdr_status rc = tune_shapes(chosen);
     if (rc == MVDR_OK)
         for (mvdr_solver* m : bt->members) (void)hipEventRecord(m->done, st);
     return rc;
