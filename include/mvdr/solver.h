@@ -313,6 +313,7 @@ typedef struct {
     uint32_t cg_workgroups;         /* fused CG row kernels, per frame (0 in a set: the rule) */
     uint32_t eval_workgroups;       /* term evaluation, per frame                             */
     uint32_t transform_workgroups;  /* CG transform, per frame                                */
+    uint32_t gram_rows;             /* rows per row wave of the Gram assembly: 1, 2 or 4      */
 } mvdr_launch_params;
 
 /* Times each tunable launch over its candidates (reps launches each, on the

@@ -60,7 +60,7 @@ class FrameInput(C.Structure):
 
 
 class LaunchParams(C.Structure):
-    _fields_ = [("cg_workgroups", u32), ("eval_workgroups", u32), ("transform_workgroups", u32)]
+    _fields_ = [("cg_workgroups", u32), ("eval_workgroups", u32), ("transform_workgroups", u32), ("gram_rows", u32)]
 
 
 class SolverSizes(C.Structure):

@@ -777,12 +777,12 @@ __device__ inline float lane_f32(float v, uint32_t k) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)k));
 }
 
-// Rows per row wave.  A wave runs the Node2Term entries of kGramRows
-// consecutive rows (and of each row's column passes) as one stream of
-// 64-entry steps: the entries and term records of a row's first step are
-// fetched while the previous row's last step computes, so a row start costs
-// no dependent loads of its own.
-constexpr uint32_t kGramRows = 2;
+// Rows per row wave (kRows: 1, 2 or 4, a launch shape).  A wave runs the
+// Node2Term entries of kRows consecutive rows (and of each row's column
+// passes) as one stream of 64-entry steps: the entries and term records of a
+// row's first step are fetched while the previous row's last step computes,
+// so a row start costs no dependent loads of its own.  Every row's products
+// run in the same order for any kRows.
 
 // Position in a wave's stream: row k (of the wave), column pass c0, step e0.
 struct GramCursor {
@@ -790,6 +790,7 @@ struct GramCursor {
 };
 
 // diag_only: the diagonal blocks and J^T r alone (the matrix-free solve).
+template <uint32_t kGramRows>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 gram_rows_kernel(const Buffers* __restrict__ bs, Penalty pen, int diag_only) {
     const Buffers b = frame_buffers(bs);
@@ -1014,6 +1015,28 @@ __global__ void __launch_bounds__(256) cost_kernel(const Buffers* __restrict__ b
 // Terms at the warp field se3[which].
 // Workgroups per frame: a grid-stride loop over the terms (a tuned count from
 // the launch table, else 1024).
+// Rows per Gram row wave: the table's choice, else two.
+uint32_t gram_rows_for(uint32_t nodes, uint32_t frames) {
+    const uint32_t tuned = launch_params(nodes, frames).gram_rows;
+    return tuned == 1 || tuned == 4 ? tuned : 2u;
+}
+
+template <uint32_t kRows>
+static void launch_gram_rows(const Batch& B, const Penalty& pen, bool diag_only, hipStream_t stream) {
+    const uint32_t waves = ((B.nodes + kRows - 1) / kRows + kXcds - 1) / kXcds * kXcds;
+    hipLaunchKernelGGL(gram_rows_kernel<kRows>, dim3(waves, B.frames), dim3(64), 0, stream, B.dev, pen,
+                       diag_only ? 1 : 0);
+}
+
+// The row Gram assembly alone (assemble and the autotuner).
+void launch_gram_rows_once(const Batch& B, const Penalty& pen, bool diag_only, hipStream_t stream) {
+    switch (gram_rows_for(B.nodes, B.frames)) {
+        case 1: launch_gram_rows<1>(B, pen, diag_only, stream); break;
+        case 4: launch_gram_rows<4>(B, pen, diag_only, stream); break;
+        default: launch_gram_rows<2>(B, pen, diag_only, stream); break;
+    }
+}
+
 uint32_t eval_workgroups_for(uint32_t nodes, uint32_t frames) {
     const uint32_t tuned = launch_params(nodes, frames).eval_workgroups;
     return tuned ? tuned : 1024u;
@@ -1034,8 +1057,7 @@ void launch_eval_terms(const Batch& B, uint32_t which, bool fp64_geometry, hipSt
 void launch_assemble(const Batch& B, const Penalty& pen, bool gram, bool diag_only, hipStream_t stream) {
     const uint32_t fix_blocks = std::min<uint32_t>((8 * B.nodes + 255) / 256, 1024u);  // grid-stride over N + P
     if (gram) {
-        const uint32_t waves = ((B.nodes + kGramRows - 1) / kGramRows + kXcds - 1) / kXcds * kXcds;
-        hipLaunchKernelGGL(gram_rows_kernel, dim3(waves, B.frames), dim3(64), 0, stream, B.dev, pen, diag_only ? 1 : 0);
+        launch_gram_rows_once(B, pen, diag_only, stream);
         hipLaunchKernelGGL(gram_fixup_kernel, dim3((B.nodes + 63) / 64, B.frames), dim3(64), 0, stream, B.dev,
                            diag_only ? 1 : 0, diag_only ? 1 : 0);
     } else {

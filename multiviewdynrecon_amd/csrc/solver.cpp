@@ -40,6 +40,8 @@ void launch_cg_iter_once(const Batch& B, hipStream_t stream);
 void launch_cg_transform_once(const Batch& B, const Penalty& pen, hipStream_t stream);
 uint32_t cg_workgroups_for(uint32_t nodes, uint32_t frames);
 uint32_t transform_workgroups_for(uint32_t nodes, uint32_t frames);
+uint32_t gram_rows_for(uint32_t nodes, uint32_t frames);
+void launch_gram_rows_once(const Batch& B, const Penalty& pen, bool diag_only, hipStream_t stream);
 uint32_t eval_workgroups_for(uint32_t nodes, uint32_t frames);
 void launch_copy_se3(const Batch& B, hipStream_t stream);
 }  // namespace mvdr
@@ -1100,6 +1102,19 @@ static mvdr_status autotune_impl(const Batch& B, const mvdr_solver_config& c, ui
         }
         p.transform_workgroups = arg;
     }
+    if (fp64) {
+        // row Gram assembly: rows per row wave (the diagonal-only form for the
+        // matrix-free solve)
+        float best = 0.f;
+        uint32_t arg = 0;
+        for (uint32_t cand : {1u, 2u, 4u}) {
+            p.gram_rows = cand;
+            set_launch_params(B.nodes, B.frames, p);
+            const float t = timed([&] { launch_gram_rows_once(B, pen, !fused, st); });
+            if (arg == 0 || t < best) { best = t; arg = cand; }
+        }
+        p.gram_rows = arg;
+    }
     set_launch_params(B.nodes, B.frames, p);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -1111,6 +1126,7 @@ static mvdr_status autotune_impl(const Batch& B, const mvdr_solver_config& c, ui
         chosen->cg_workgroups = fused ? cg_workgroups_for(B.nodes, B.frames) : 0;
         chosen->eval_workgroups = eval_workgroups_for(B.nodes, B.frames);
         chosen->transform_workgroups = fused ? transform_workgroups_for(B.nodes, B.frames) : 0;
+        chosen->gram_rows = fp64 ? gram_rows_for(B.nodes, B.frames) : 0;
     }
     return MVDR_OK;
 }
@@ -1202,12 +1218,14 @@ mvdr_status mvdr_launch_params_get(uint32_t nodes, uint32_t frames, mvdr_launch_
     out->cg_workgroups = cg_workgroups_for(n, frames);
     out->eval_workgroups = eval_workgroups_for(n, frames);
     out->transform_workgroups = transform_workgroups_for(n, frames);
+    out->gram_rows = gram_rows_for(n, frames);
     return MVDR_OK;
 }
 
 mvdr_status mvdr_launch_params_set(uint32_t nodes, uint32_t frames, const mvdr_launch_params* p) {
     if (!p || nodes == 0 || nodes > 65535 || frames == 0) return MVDR_E_INVALID;
-    set_launch_params(live_nodes(nodes), frames, LaunchParams{p->cg_workgroups, p->eval_workgroups, p->transform_workgroups});
+    set_launch_params(live_nodes(nodes), frames,
+                      LaunchParams{p->cg_workgroups, p->eval_workgroups, p->transform_workgroups, p->gram_rows});
     return MVDR_OK;
 }
 

@@ -27,14 +27,23 @@ bool store_table_text(const std::string& text, std::string& err) {
         ++number;
         if (line.empty() || line[0] == '#') continue;
         std::istringstream fields(line);
-        long long v[5];
-        for (long long& x : v) {
-            if (!(fields >> x) || x < 0 || x > 0xFFFFFFFFll) {
-                err = "launch table line " + std::to_string(number) + ": expected five unsigned integers";
+        long long v[6] = {0, 0, 0, 0, 0, 0};
+        for (int q = 0; q < 5; ++q) {
+            if (!(fields >> v[q]) || v[q] < 0 || v[q] > 0xFFFFFFFFll) {
+                err = "launch table line " + std::to_string(number) + ": expected five or six unsigned integers";
                 return false;
             }
         }
         std::string extra;
+        if (fields >> extra) {
+            char* stop = nullptr;
+            const long long g = std::strtoll(extra.c_str(), &stop, 10);
+            if (*stop != '\0' || g < 0 || g > 0xFFFFFFFFll) {
+                err = "launch table line " + std::to_string(number) + ": expected five or six unsigned integers";
+                return false;
+            }
+            v[5] = g;
+        }
         if (fields >> extra) {
             err = "launch table line " + std::to_string(number) + ": trailing text";
             return false;
@@ -43,7 +52,8 @@ bool store_table_text(const std::string& text, std::string& err) {
             err = "launch table line " + std::to_string(number) + ": zero frames";
             return false;
         }
-        table[{(uint32_t)v[0], (uint32_t)v[1]}] = LaunchParams{(uint32_t)v[2], (uint32_t)v[3], (uint32_t)v[4]};
+        table[{(uint32_t)v[0], (uint32_t)v[1]}] =
+            LaunchParams{(uint32_t)v[2], (uint32_t)v[3], (uint32_t)v[4], (uint32_t)v[5]};
     }
     std::lock_guard<std::mutex> lock(g_mutex);
     g_table = std::move(table);
@@ -74,7 +84,7 @@ LaunchParams launch_params(uint32_t nodes, uint32_t frames) {
     std::call_once(g_env_once, load_from_environment);
     std::lock_guard<std::mutex> lock(g_mutex);
     const auto it = g_table.find({node_bucket(nodes), frames});
-    return it == g_table.end() ? LaunchParams{0, 0, 0} : it->second;
+    return it == g_table.end() ? LaunchParams{0, 0, 0, 0} : it->second;
 }
 
 void set_launch_params(uint32_t nodes, uint32_t frames, const LaunchParams& p) {
@@ -101,10 +111,11 @@ std::string launch_table_text() {
     std::call_once(g_env_once, load_from_environment);
     std::lock_guard<std::mutex> lock(g_mutex);
     std::ostringstream out;
-    out << "# mvdr launch table: node_bucket(1024 nodes) frames cg_workgroups eval_workgroups transform_workgroups\n";
+    out << "# mvdr launch table: node_bucket(1024 nodes) frames cg_workgroups eval_workgroups transform_workgroups "
+           "gram_rows\n";
     for (const auto& kv : g_table)
         out << kv.first.first << ' ' << kv.first.second << ' ' << kv.second.cg_workgroups << ' '
-            << kv.second.eval_workgroups << ' ' << kv.second.transform_workgroups << '\n';
+            << kv.second.eval_workgroups << ' ' << kv.second.transform_workgroups << ' ' << kv.second.gram_rows << '\n';
     return out.str();
 }
 

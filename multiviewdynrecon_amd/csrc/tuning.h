@@ -14,6 +14,7 @@ struct LaunchParams {
     uint32_t cg_workgroups;         // fused CG row kernels, per frame (0: rule)
     uint32_t eval_workgroups;       // term evaluation, per frame (0: rule)
     uint32_t transform_workgroups;  // CG transform, per frame (0: rule)
+    uint32_t gram_rows;             // rows per Gram row wave: 1, 2 or 4 (0: rule)
 };
 
 // Shapes are keyed by the batch's node count in 1024-node buckets and its
@@ -28,7 +29,8 @@ void clear_launch_params();
 // re-captured.
 uint64_t launch_table_version();
 
-// Text form: one line per entry, "bucket frames cg eval transform".
+// Text form: one line per entry, "bucket frames cg eval transform gram"
+// (lines without the last field read it as 0).
 std::string launch_table_text();
 bool load_launch_table_text(const std::string& text, std::string& err);
 

@@ -315,8 +315,8 @@ def launch_params(nodes, frames):
     return launch_params_dict(out)
 
 
-def set_launch_params(nodes, frames, cg_workgroups=0, eval_workgroups=0, transform_workgroups=0):
-    p = _abi.LaunchParams(cg_workgroups, eval_workgroups, transform_workgroups)
+def set_launch_params(nodes, frames, cg_workgroups=0, eval_workgroups=0, transform_workgroups=0, gram_rows=0):
+    p = _abi.LaunchParams(cg_workgroups, eval_workgroups, transform_workgroups, gram_rows)
     rc = lib().mvdr_launch_params_set(nodes, frames, C.byref(p))
     if rc != 0:
         raise SolverError(rc, "mvdr_launch_params_set")

@@ -56,10 +56,11 @@ def test_tuning_changes_no_result(scene1, precision, use_graph):
     assert_same(solved(members), ref, "after autotune")
     batch.set_lanes(1)  # the forced shapes below are for launches of both frames
     nodes = max(sc.num_nodes for sc in scenes)
-    for cg, ev, tr in ((8, 256, 2048), (32, 2048, 256), (10000, 512, 512)):
-        sv.set_launch_params(nodes, len(scenes), cg_workgroups=cg, eval_workgroups=ev, transform_workgroups=tr)
+    for cg, ev, tr, gr in ((8, 256, 2048, 1), (32, 2048, 256, 4), (10000, 512, 512, 2)):
+        sv.set_launch_params(nodes, len(scenes), cg_workgroups=cg, eval_workgroups=ev, transform_workgroups=tr,
+                             gram_rows=gr)
         batch.SolveNonRigidAlignment()
-        assert_same(solved(members), ref, f"forced cg {cg} eval {ev} transform {tr}")
+        assert_same(solved(members), ref, f"forced cg {cg} eval {ev} transform {tr} gram rows {gr}")
     # a lone frame: its own shape, the same results as in the batch
     single = NonRigidSolver(cfg)
     single.SetSolverInput(DeviceFrame(scene1))
