@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage-reps", type=int, default=50)
     ap.add_argument("--batch", type=int, default=32, help="frames solved per step (one launch per stage)")
+    ap.add_argument("--lanes", type=int, default=0,
+                    help="solve lanes of the batch (0: the library's default, or the autotuner's choice)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in launch shapes (the autotuner changes no result, only workgroup counts)")
     ap.add_argument("--no-extras", action="store_true",
@@ -387,12 +389,17 @@ def main():
         s.SetSolverInput(fr)
         solvers.append(s)
     batch = NonRigidBatch(solvers)
+    if args.lanes:
+        batch.set_lanes(args.lanes)
     print(f"[bench] {B} frames resident", file=sys.stderr, flush=True)
     tuned = {}
     if not args.no_autotune:
         # launch shapes for this batch and for a lone frame, before any timing
         batch.SolveNonRigidAlignment()
         tuned["batch"] = dict(batch.autotune(reps=10), lanes=batch.lanes)
+        if args.lanes:
+            batch.set_lanes(args.lanes)
+            tuned["batch"]["lanes"] = args.lanes
         solvers[0].SolveNonRigidAlignment()
         tuned["single"] = solvers[0].autotune(reps=10)
         print(f"[bench] autotuned launch shapes: {json.dumps(tuned)}", file=sys.stderr, flush=True)

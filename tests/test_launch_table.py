@@ -17,7 +17,7 @@ def test_rules_without_entries():
     # ~4k nodes: 127 row groups of 32 -> one pass each for a lone frame (rounded
     # to the 8 XCDs), at most 64 workgroups per frame in a 32-frame launch
     assert sv.launch_params(4048, 1) == {"cg_workgroups": 128, "eval_workgroups": 1024, "transform_workgroups": 128,
-                                         "gram_rows": 2}
+                                         "gram_rows": 1}
     assert sv.launch_params(4048, 32)["cg_workgroups"] == 64
     # 16 frames of ~4k nodes: 127 x 16 <= 2048 workgroups, still one pass
     assert sv.launch_params(4048, 16)["cg_workgroups"] == 128
@@ -37,7 +37,7 @@ def test_entries_override_and_clamp():
     sv.set_launch_params(4048, 32, cg_workgroups=20)
     assert sv.launch_params(4048, 32)["cg_workgroups"] == 24  # rounded up to the XCD count
     sv.set_launch_params(4048, 8, gram_rows=3)
-    assert sv.launch_params(4048, 8)["gram_rows"] == 2  # 1, 2 or 4; anything else is the rule
+    assert sv.launch_params(4048, 8)["gram_rows"] == 1  # 1, 2 or 4; anything else is the rule
 
 
 def test_save_load_round_trip(tmp_path):
@@ -60,7 +60,7 @@ def test_five_field_lines_read_gram_rows_as_rule(tmp_path):
     path.write_text("# header\n4 32 64 1024 128\n")
     sv.load_launch_table(path)
     assert sv.launch_params(4048, 32) == {"cg_workgroups": 64, "eval_workgroups": 1024, "transform_workgroups": 128,
-                                          "gram_rows": 2}
+                                          "gram_rows": 1}
 
 
 @pytest.mark.parametrize("text", ["4 32 64 1024\n", "4 0 64 1024 128\n", "4 32 64 1024 128 1 9\n",
