@@ -133,7 +133,8 @@ class NonRigidSolver:
             self.h = C.c_void_p()
 
     def __del__(self):
-        self.close()
+        if lib is not None:  # module globals are cleared at interpreter exit
+            self.close()
 
     def _check(self, rc):
         if rc != 0:
@@ -271,7 +272,8 @@ class NonRigidBatch:
             self.h = C.c_void_p()
 
     def __del__(self):
-        self.close()
+        if lib is not None:  # module globals are cleared at interpreter exit
+            self.close()
 
     def _check(self, rc):
         if rc != 0:
