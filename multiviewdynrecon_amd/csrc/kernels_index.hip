@@ -888,26 +888,44 @@ __global__ void __launch_bounds__(64 * kWaves) pair_rows_kernel(const Buffers* _
         return;
     }
     if (w == 0) {
-        // ascending extraction, one partner per step (wave minimum)
-        uint32_t mine[kRowHash / 64];
+        // ascending partner lists: the set's entries are compacted as
+        // (node << 16 | slot) and each is ranked by a count of the smaller ones
+        // (nodes are distinct, so the packed order is the node order); the
+        // lists are written at the ranks.  kRowGram also writes each upper
+        // partner's weight column into its set slot (node | column << 16).
+        uint32_t* cand = ll;  // ll is rewritten with the lower list once every rank is known
+        uint32_t total = 0;
 #pragma unroll
-        for (int k = 0; k < (int)(kRowHash / 64); ++k) mine[k] = set[lane + 64 * k];
-        uint32_t up = 0, low = 0;
-        uint32_t next_min = 0;  // the last extracted partner + 1
-        for (;;) {
-            uint32_t m = 0xFFFFFFFFu;
+        for (int k = 0; k < (int)(kRowHash / 64); ++k) {
+            const uint32_t v = set[lane + 64 * k];
+            const uint64_t full = __ballot(v != 0xFFFFFFFFu);
+            if (v != 0xFFFFFFFFu) cand[total + (uint32_t)__popcll(full & below)] = v << 16 | (lane + 64 * k);
+            total += (uint32_t)__popcll(full);
+        }
+        wave_lds_sync();
+        uint32_t low = 0;
+        uint32_t key[kRowHash / 64], rank[kRowHash / 64];
 #pragma unroll
-            for (int k = 0; k < (int)(kRowHash / 64); ++k)
-                if (mine[k] >= next_min && mine[k] < m) m = mine[k];
+        for (int r = 0; r < (int)(kRowHash / 64); ++r) {
+            const uint32_t j = 64 * r + lane;
+            key[r] = j < total ? cand[j] : 0xFFFFFFFFu;
+            rank[r] = 0;
+            if (64 * (uint32_t)r >= total) continue;  // uniform
+            low += (uint32_t)__popcll(__ballot(j < total && (key[r] >> 16) < i));
+            if (kMode != kRowCount)
+                for (uint32_t q = 0; q < total; ++q) rank[r] += cand[q] < key[r] ? 1u : 0u;
+        }
+        const uint32_t up = total - low;
+        if (kMode != kRowCount) {
+            wave_lds_sync();  // every read of cand is done before the lists overwrite ll
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const uint32_t u = (uint32_t)__shfl_xor((int)m, o, 64);
-                m = u < m ? u : m;
+            for (int r = 0; r < (int)(kRowHash / 64); ++r) {
+                if (64 * (uint32_t)r + lane >= total) continue;
+                const uint32_t v = key[r] >> 16;
+                if (v > i) ul[rank[r] - low] = v;
+                else ll[rank[r]] = v;
+                if (kMode == kRowGram && v > i) set[key[r] & 0xFFFFu] = v | (2 + rank[r] - low) << 16;
             }
-            if (m == 0xFFFFFFFFu) break;
-            if (m > i) ul[up++] = m;  // same value from every lane
-            else ll[low++] = m;
-            next_min = m + 1;
         }
         if (lane == 0) {
             nup = up;
@@ -947,7 +965,14 @@ __global__ void __launch_bounds__(64 * kWaves) pair_rows_kernel(const Buffers* _
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const uint32_t v = slot(kq[u], q);
-                        const uint32_t col = (v != 0xFFFF && v > i) ? 2 + lower_bound_lds(ul, up, v) : 0xFFu;
+                        uint32_t col = 0xFFu;
+                        if (v != 0xFFFF && v > i) {
+                            // v is in the set: its slot holds its column (above the node)
+                            uint32_t h = (v * 2654435761u) >> 23;
+                            for (uint32_t probe = 0; probe < kRowHash && (set[h] & 0xFFFFu) != v; ++probe)
+                                h = (h + 1) & (kRowHash - 1);
+                            col = (set[h] & 0xFFFFu) == v ? set[h] >> 16 : 0xFFu;
+                        }
                         packed |= col << (8 * q);
                     }
                 }
