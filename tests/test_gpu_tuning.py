@@ -69,3 +69,22 @@ def test_tuning_changes_no_result(scene1, precision, use_graph):
     single.autotune(reps=3)
     single.SolveNonRigidAlignment()
     assert_same(solved([single]), ref[:1], "single after autotune")
+
+
+@pytest.mark.parametrize("jtj_mode", [1, 2])
+def test_gram_rows_per_wave_change_no_result_matrix_free(scene1, jtj_mode):
+    """The matrix-free solve assembles only the diagonal blocks; 1, 2 or 4 Gram
+    rows per wave give the same solve bit for bit."""
+    cfg = workload_config(3 if jtj_mode == 2 else 1, precision=1, jtj_mode=jtj_mode, use_graph=1)
+    s = NonRigidSolver(cfg)
+    s.SetSolverInput(DeviceFrame(scene1))
+    s.SolveNonRigidAlignment()
+    ref = solved([s])
+    for gr in (2, 4, 1):
+        sv.set_launch_params(scene1.num_nodes, 1, gram_rows=gr)
+        s.SolveNonRigidAlignment()
+        assert_same(solved([s]), ref, f"matrix-free mode {jtj_mode}, gram rows {gr}")
+    chosen = s.autotune(reps=2)
+    assert chosen["gram_rows"] in (1, 2, 4) and chosen["cg_workgroups"] == 0
+    s.SolveNonRigidAlignment()
+    assert_same(solved([s]), ref, f"matrix-free mode {jtj_mode} after autotune")
