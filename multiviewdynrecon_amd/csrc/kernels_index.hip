@@ -18,6 +18,12 @@ namespace mvdr {
 constexpr int kTileThreads = 256;
 constexpr int kTileRounds = 16;
 constexpr int kTilePixels = kTileThreads * kTileRounds;  // 4096 pixels per tile
+constexpr uint32_t kSelectBlocks = 32;  // workgroups per frame of the pair selection's first step
+
+// The feature / cross-view pair selection shares the dense compaction's
+// launches (defined with the selection below).
+__device__ void select_eval(const Buffers& b, uint32_t block, int use_cross);
+__device__ void select_pairs(const Buffers& b, bool is_cross, int use_cross);
 
 // ------------------------------------------------------------- term selection
 struct PixelCoord {
@@ -113,9 +119,14 @@ __global__ void __launch_bounds__(kTileThreads) compact_count_kernel(const Buffe
     if (threadIdx.x == 0) b.tile_count[blockIdx.x] = count;
 }
 
-// Pass 2: exclusive scan of the tile counts (one block).
-__global__ void __launch_bounds__(1024) compact_scan_kernel(const Buffers* __restrict__ bs, int which) {
+// Pass 2: exclusive scan of the tile counts (block 0).  select >= 0: blocks 1
+// and 2 compact the selected feature / cross-view pairs (select = use_cross).
+__global__ void __launch_bounds__(1024) compact_scan_kernel(const Buffers* __restrict__ bs, int which, int select) {
     const Buffers b = frame_buffers(bs);
+    if (blockIdx.x > 0) {
+        select_pairs(b, blockIdx.x == 2, select);
+        return;
+    }
     const FrameDev& f = *b.frame;
     const uint32_t M = f.V * f.W * f.H;
     const uint32_t tiles = (M + kTilePixels - 1) / kTilePixels;
@@ -214,25 +225,26 @@ __device__ inline void dense_tile_surfels(const FrameDev& f, uint32_t base, uint
     }
 }
 
-__global__ void __launch_bounds__(kTileThreads) dense_count_kernel(const Buffers* __restrict__ bs) {
+// Workgroups [0, kSelectBlocks) evaluate the pair candidates; workgroup
+// kSelectBlocks + t counts tile t.
+__global__ void __launch_bounds__(kTileThreads) dense_count_kernel(const Buffers* __restrict__ bs, int use_cross) {
     const Buffers b = frame_buffers(bs);
+    if (blockIdx.x < kSelectBlocks) {
+        select_eval(b, blockIdx.x, use_cross);
+        return;
+    }
+    const uint32_t tile = blockIdx.x - kSelectBlocks;
     const FrameDev& f = *b.frame;
     const uint32_t M = f.V * f.W * f.H;
     // clears the counters the rest of the index build accumulates into
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= b.cap_nodes; i += stride) {
+    const uint32_t stride = (gridDim.x - kSelectBlocks) * blockDim.x;
+    for (uint32_t i = tile * blockDim.x + threadIdx.x; i <= b.cap_nodes; i += stride) {
         b.n2t_count[i] = 0;
         b.row_up[i] = 0;
         b.row_low[i] = 0;
         b.row_kv2[i] = 0;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        Counts* c = b.counts;
-        c->dense = c->smooth = c->fg = c->feat = c->cross = 0;
-        c->kv1 = c->kv2 = c->pairs = c->max_seg = c->max_row = 0;
-        c->error = 0;
-    }
-    const uint32_t base = blockIdx.x * kTilePixels;
+    const uint32_t base = tile * kTilePixels;
     if (base >= M) return;
     uint32_t sv[kTileRounds];
     dense_tile_surfels(f, base, sv);
@@ -245,7 +257,7 @@ __global__ void __launch_bounds__(kTileThreads) dense_count_kernel(const Buffers
     }
     if (bad) atomicOr(&b.counts->error, kErrNodes);
     count = block_sum_u32<kTileThreads>(count);
-    if (threadIdx.x == 0) b.tile_count[blockIdx.x] = count;
+    if (threadIdx.x == 0) b.tile_count[tile] = count;
 }
 
 __global__ void __launch_bounds__(kTileThreads) dense_scatter_kernel(const Buffers* __restrict__ bs) {
@@ -431,12 +443,13 @@ __device__ inline bool choose_cross(const FrameDev& f, uint32_t k, P2PRec& rec, 
 // Selection, step 1: every candidate (feature pairs, then cross-view pairs)
 // evaluated by its own thread into the staging slot of its candidate index.
 // The candidate counts were checked against cap_feat / cap_cross at set-up.
-__global__ void __launch_bounds__(256) select_eval_kernel(const Buffers* __restrict__ bs, int use_cross) {
-    const Buffers b = frame_buffers(bs);
+// Runs as the first kSelectBlocks workgroups of the dense count's launch: it
+// reads only the frame's inputs and writes only the staging arrays.
+__device__ void select_eval(const Buffers& b, uint32_t block, int use_cross) {
     const FrameDev& f = *b.frame;
     const uint32_t nf = f.feat_base[f.V], total = nf + (use_cross ? f.num_cross : 0u);
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+    const uint32_t stride = kSelectBlocks * kTileThreads;
+    for (uint32_t g = block * kTileThreads + threadIdx.x; g < total; g += stride) {
         P2PRec rec;
         float4 canon;
         const bool ok = g < nf ? choose_feature(f, g, rec, canon) : choose_cross(f, g - nf, rec, canon);
@@ -448,12 +461,11 @@ __global__ void __launch_bounds__(256) select_eval_kernel(const Buffers* __restr
     }
 }
 
-// Step 2: block 0 compacts feature pairs, block 1 cross-view pairs, each in
-// candidate order with a running block-wide prefix.
-__global__ void __launch_bounds__(1024) select_pairs_kernel(const Buffers* __restrict__ bs, int use_cross) {
-    const Buffers b = frame_buffers(bs);
+// Step 2: one 1024-thread block compacts the feature pairs, another the
+// cross-view pairs, each in candidate order with a running block-wide prefix
+// (blocks 1 and 2 of the dense scan's launch).
+__device__ void select_pairs(const Buffers& b, bool is_cross, int use_cross) {
     const FrameDev& f = *b.frame;
-    const bool is_cross = blockIdx.x == 1;
     if (is_cross && !use_cross) {
         if (threadIdx.x == 0) b.counts->cross = 0;
         return;
@@ -1119,14 +1131,18 @@ __global__ void __launch_bounds__(256) pair_edges_kernel(const Buffers* __restri
 
 // Pair index of every lower entry (k, .) of row j: the position of (k, j) in
 // row k's ascending upper list (and, in the mixed mode, the inverse map).
+constexpr uint32_t kLinkLanes = 8;  // lanes per row of the pair link
+
 __global__ void __launch_bounds__(256) pair_link_kernel(const Buffers* __restrict__ bs) {
     const Buffers b = frame_buffers(bs);
     const Counts& c = *b.counts;
     if (c.error) return;
     const uint32_t N = b.frame->N;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < N; j += stride) {
-        for (uint32_t l = b.low_offset[j]; l < b.low_offset[j + 1]; ++l) {
+    const uint32_t sub = threadIdx.x % kLinkLanes;
+    const uint32_t stride = gridDim.x * blockDim.x / kLinkLanes;
+    for (uint32_t j = (blockIdx.x * blockDim.x + threadIdx.x) / kLinkLanes; j < N; j += stride) {
+        const uint32_t l1 = b.low_offset[j + 1];
+        for (uint32_t l = b.low_offset[j] + sub; l < l1; l += kLinkLanes) {
             const uint32_t k = b.low_ent[l].x;
             const uint32_t key = (k << 16) | j;
             uint32_t lo = b.pair_offset[k], hi = b.pair_offset[k + 1];
@@ -1149,21 +1165,24 @@ __global__ void __launch_bounds__(256) pair_link_kernel(const Buffers* __restric
 // mode kIndexLists: the NodePair2Term lists of the reference's assembly;
 // kIndexGram: pair keys, knn-slot columns and graph edges per pair for the row
 // Gram assembly; kIndexNodes: Node2Term alone (the matrix-free solve).
+// The frame's Counts start at zero: launch_copy_se3 clears them earlier on
+// the same stream.  The pair selection rides in the dense count's and the
+// dense scan's launches (its own workgroups; disjoint outputs).
 void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, int mode, hipStream_t stream) {
     const bool gram = mode == kIndexGram;
     const uint32_t F = B.frames;
-    hipLaunchKernelGGL(dense_count_kernel, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream, B.dev);
-    hipLaunchKernelGGL(compact_scan_kernel, dim3(1, F), dim3(1024), 0, stream, B.dev, 0);
+    const int cross = use_cross ? 1 : 0;
+    hipLaunchKernelGGL(dense_count_kernel, dim3(kSelectBlocks + B.cap_tiles, F), dim3(kTileThreads), 0, stream, B.dev,
+                       cross);
+    hipLaunchKernelGGL(compact_scan_kernel, dim3(3, F), dim3(1024), 0, stream, B.dev, 0, cross);
     hipLaunchKernelGGL(dense_scatter_kernel, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream, B.dev);
     if (use_foreground) {
         hipLaunchKernelGGL(compact_count_kernel<ForegroundPred>, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream,
                            B.dev, ForegroundPred{}, 0);
-        hipLaunchKernelGGL(compact_scan_kernel, dim3(1, F), dim3(1024), 0, stream, B.dev, 1);
+        hipLaunchKernelGGL(compact_scan_kernel, dim3(1, F), dim3(1024), 0, stream, B.dev, 1, -1);
         hipLaunchKernelGGL(compact_scatter_kernel<ForegroundPred>, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream,
                            B.dev, ForegroundPred{}, 1);
     }
-    hipLaunchKernelGGL(select_eval_kernel, dim3(32, F), dim3(256), 0, stream, B.dev, use_cross ? 1 : 0);
-    hipLaunchKernelGGL(select_pairs_kernel, dim3(2, F), dim3(1024), 0, stream, B.dev, use_cross ? 1 : 0);
     const uint32_t tile = hist_tile(B.nodes);
     hipLaunchKernelGGL(n2t_hist_kernel, dim3(B.cap_chunks, F), dim3(64), tile * sizeof(uint32_t), stream, B.dev, tile);
     hipLaunchKernelGGL(n2t_colscan_kernel, dim3((B.nodes + 63) / 64, F), dim3(64 * kScanWaves), 0, stream, B.dev);
@@ -1181,7 +1200,7 @@ void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, int
         hipLaunchKernelGGL(node_scan_kernel, dim3(3, F), dim3(1024), 0, stream, B.dev, 1);
         hipLaunchKernelGGL((pair_rows_kernel<kRowTerms, kRowWaves>), dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
     }
-    hipLaunchKernelGGL(pair_link_kernel, dim3((B.nodes + 255) / 256, F), dim3(256), 0, stream, B.dev);
+    hipLaunchKernelGGL(pair_link_kernel, dim3((B.nodes * kLinkLanes + 255) / 256, F), dim3(256), 0, stream, B.dev);
     if (gram) hipLaunchKernelGGL(pair_edges_kernel, dim3((8 * B.nodes + 255) / 256, F), dim3(256), 0, stream, B.dev);
 }
 
@@ -1191,7 +1210,7 @@ void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, int
 void launch_pair_term_lists(const Batch& B, hipStream_t stream) {
     const uint32_t row_grid = (B.nodes + kXcds - 1) / kXcds * kXcds;
     hipLaunchKernelGGL((pair_rows_kernel<kRowTerms, kRowWaves>), dim3(row_grid, B.frames), dim3(64 * kRowWaves), 0, stream, B.dev);
-    hipLaunchKernelGGL(pair_link_kernel, dim3((B.nodes + 255) / 256, B.frames), dim3(256), 0, stream, B.dev);
+    hipLaunchKernelGGL(pair_link_kernel, dim3((B.nodes * kLinkLanes + 255) / 256, B.frames), dim3(256), 0, stream, B.dev);
 }
 
 uint32_t index_chunks(uint64_t max_terms) { return (uint32_t)((max_terms + kChunkTerms - 1) / kChunkTerms); }

@@ -824,11 +824,18 @@ __global__ void __launch_bounds__(256) apply_update_kernel(const Buffers* __rest
     }
 }
 
-// Start of a frame: the warp field the GN loop reads first is the input one.
+// Start of a frame: the warp field the GN loop reads first is the input one,
+// and the counters the index build accumulates into start at zero.
 __global__ void copy_se3_kernel(const Buffers* __restrict__ bs) {
     const Buffers b = frame_buffers(bs);
     DQ* __restrict__ dst = b.se3[0];
     const FrameDev& f = *b.frame;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Counts* c = b.counts;
+        c->dense = c->smooth = c->fg = c->feat = c->cross = 0;
+        c->kv1 = c->kv2 = c->pairs = c->max_seg = c->max_row = 0;
+        c->error = 0;
+    }
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < f.N; n += stride) dst[n] = f.node_se3_in[n];
 }
