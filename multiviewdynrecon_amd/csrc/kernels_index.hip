@@ -545,11 +545,19 @@ __device__ inline uint32_t slot(ushort4 k, int q) { return q == 0 ? k.x : (q == 
 //   n2t_place   each chunk writes its entries at offset[n] + H[c][n] + rank,
 //               rank = position among the chunk's earlier entries of node n
 // No global atomics and no sorts: the result is deterministic by construction.
+// A chunk holds at most 4 kChunkTerms entries, so the per-chunk node counters
+// fit 16 bits: two share a 32-bit LDS word (half the LDS of u32 counters, twice
+// the resident workgroups).
 constexpr uint32_t kChunkTerms = 1024;
-constexpr uint32_t kHistNodes = 16384;  // largest LDS node tile (64 KB of u32 counters)
+constexpr uint32_t kHistNodes = 16384;  // largest LDS node tile (32 KB of u16 counters)
+static_assert(4 * kChunkTerms <= 0xFFFFu, "per-chunk node counts must fit 16 bits");
 
 // LDS node tile of the counting-sort kernels (dynamic shared memory).
 uint32_t hist_tile(uint32_t nodes) { return std::min<uint32_t>(nodes, kHistNodes); }
+size_t hist_tile_bytes(uint32_t tile) { return (size_t)(tile + 1) / 2 * sizeof(uint32_t); }
+// Chunk workgroups (over all frames of a launch) up to which the placement
+// keeps u32 counters: about what the chip holds at once with 16 KB tiles.
+constexpr size_t kPlaceResident = 2048;
 
 __device__ inline uint32_t num_chunks(const Counts& c) { return (c.off[4] + kChunkTerms - 1) / kChunkTerms; }
 
@@ -571,24 +579,34 @@ __global__ void __launch_bounds__(64) n2t_hist_kernel(const Buffers* __restrict_
     uint32_t* H = b.n2t_hist + (size_t)blockIdx.x * N;
     for (uint32_t n0 = 0; n0 < N; n0 += tile) {
         const uint32_t span = min(tile, N - n0);
-        for (uint32_t k = threadIdx.x; k < span; k += 64) hist[k] = 0;
+        for (uint32_t k = threadIdx.x; k < (span + 1) / 2; k += 64) hist[k] = 0;
         wave_lds_sync();
-        for (uint32_t t = t0 + threadIdx.x; t < t1; t += 64) {
-            ushort4 k;
-            const int m = term_nodes(b, f, off, t, k);
+        // four rounds of term nodes in flight, then their counts
+        for (uint32_t tb = t0 + threadIdx.x; tb < t1; tb += 4 * 64) {
+            ushort4 k[4];
+            int m[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t n = slot(k, q);
-                if (q >= m) continue;
-                if (n >= N) {
-                    if (n0 == 0) atomicOr(&c.error, kErrNodes);
-                } else if (n >= n0 && n - n0 < span) {
-                    atomicAdd(&hist[n - n0], 1u);
-                }
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t t = tb + 64 * r;
+                k[r] = make_ushort4(0, 0, 0, 0);
+                m[r] = t < t1 ? term_nodes(b, f, off, t, k[r]) : 0;
             }
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t n = slot(k[r], q);
+                    if (q >= m[r]) continue;
+                    if (n >= N) {
+                        if (n0 == 0) atomicOr(&c.error, kErrNodes);
+                    } else if (n >= n0 && n - n0 < span) {
+                        const uint32_t l = n - n0;
+                        atomicAdd(&hist[l >> 1], 1u << (16 * (l & 1)));
+                    }
+                }
         }
         wave_lds_sync();
-        for (uint32_t k = threadIdx.x; k < span; k += 64) H[n0 + k] = hist[k];
+        for (uint32_t k = threadIdx.x; k < span; k += 64) H[n0 + k] = (hist[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
         wave_lds_sync();
     }
 }
@@ -661,6 +679,12 @@ __device__ inline void wave_sort256(uint32_t k[4]) {
 
 // with_nodes: also each entry's node (n2t_node), which only the segmented
 // assembly of the plain path reads.
+// kLdsBase: the tile's u32 LDS counters start at each node's list offset plus
+// its entries in earlier chunks (loaded once per tile: a small grid, all of it
+// resident at once); otherwise u16 counters start at zero and each entry loads
+// that base while its round sorts (half the LDS: twice the resident
+// workgroups of a large grid).  Both place every entry at the same position.
+template <bool kLdsBase>
 __global__ void __launch_bounds__(64) n2t_place_kernel(const Buffers* __restrict__ bs, uint32_t tile, int with_nodes) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
@@ -671,21 +695,40 @@ __global__ void __launch_bounds__(64) n2t_place_kernel(const Buffers* __restrict
     const uint32_t t1 = min(c.off[4], t0 + kChunkTerms);
     const uint32_t N = f.N;
     const uint32_t lane = threadIdx.x;
-    extern __shared__ uint32_t cur[];
+    extern __shared__ uint32_t cur_words[];
+    using u16_lds = uint16_t __attribute__((may_alias));
+    u16_lds* cur16 = reinterpret_cast<u16_lds*>(cur_words);
     __shared__ uint4 rank_of[64];  // rank of entry (lane, slot) among the round's entries of its node
     const uint32_t* H = b.n2t_hist + (size_t)blockIdx.x * N;
     for (uint32_t n0 = 0; n0 < N; n0 += tile) {
         const uint32_t span = min(tile, N - n0);
+        if (kLdsBase) {
 #pragma unroll 8
-        for (uint32_t k = lane; k < span; k += 64) cur[k] = b.n2t_offset[n0 + k] + H[n0 + k];
+            for (uint32_t k = lane; k < span; k += 64) cur_words[k] = b.n2t_offset[n0 + k] + H[n0 + k];
+        } else {
+            for (uint32_t k = lane; k < (span + 1) / 2; k += 64) cur_words[k] = 0;
+        }
+        // the next round's term nodes load while this round sorts
+        ushort4 kn_next = make_ushort4(0, 0, 0, 0);
+        int m_next = t0 + lane < t1 ? term_nodes(b, f, c.off, t0 + lane, kn_next) : 0;
         for (uint32_t tb = t0; tb < t1; tb += 64) {
             const uint32_t t = tb + lane;
-            ushort4 kn = make_ushort4(0, 0, 0, 0);
-            const int m = t < t1 ? term_nodes(b, f, c.off, t, kn) : 0;
+            const ushort4 kn = kn_next;
+            const int m = m_next;
+            if (t + 64 < t1) m_next = term_nodes(b, f, c.off, t + 64, kn_next);
+            else m_next = 0;
             uint32_t n[4] = {kn.x, kn.y, kn.z, kn.w};
 #pragma unroll
             for (int q = 0; q < 4; ++q)
                 if (!(q < m && n[q] >= n0 && n[q] - n0 < span)) n[q] = 0xFFFFFFFFu;  // not placed here
+            // where this chunk's entries of each node start: the node's list
+            // offset plus its entries in earlier chunks (in flight during the sort)
+            uint32_t base[4] = {0u, 0u, 0u, 0u};
+            if (!kLdsBase) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (n[q] != 0xFFFFFFFFu) base[q] = b.n2t_offset[n[q]] + H[n[q]];
+            }
             // the round's 256 entries sorted by (node, term, slot): key = local
             // node << 8 | 4 lane + slot; entries not placed here sort last
             uint32_t k[4];
@@ -727,7 +770,8 @@ __global__ void __launch_bounds__(64) n2t_place_kernel(const Buffers* __restrict
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 if (n[q] != 0xFFFFFFFFu) {
-                    const uint32_t at = cur[n[q] - n0] + mine[q];
+                    const uint32_t l = n[q] - n0;
+                    const uint32_t at = kLdsBase ? cur_words[l] + mine[q] : base[q] + cur16[l] + mine[q];
                     b.n2t_index[at] = t;
                     if (with_nodes) b.n2t_node[at] = (uint16_t)n[q];
                 }
@@ -737,7 +781,10 @@ __global__ void __launch_bounds__(64) n2t_place_kernel(const Buffers* __restrict
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const bool tail = j < 3 ? nd[j + 1] != nd[j] : (lane == 63 || next != nd[3]);
-                if (k[j] != 0xFFFFFFFFu && tail) cur[nd[j]] += rank[j] + 1;
+                if (k[j] != 0xFFFFFFFFu && tail) {
+                    if (kLdsBase) cur_words[nd[j]] += rank[j] + 1;
+                    else cur16[nd[j]] = (uint16_t)(cur16[nd[j]] + rank[j] + 1);
+                }
             }
             wave_lds_sync();
         }
@@ -1184,11 +1231,16 @@ void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, int
                            B.dev, ForegroundPred{}, 1);
     }
     const uint32_t tile = hist_tile(B.nodes);
-    hipLaunchKernelGGL(n2t_hist_kernel, dim3(B.cap_chunks, F), dim3(64), tile * sizeof(uint32_t), stream, B.dev, tile);
+    hipLaunchKernelGGL(n2t_hist_kernel, dim3(B.cap_chunks, F), dim3(64), hist_tile_bytes(tile), stream, B.dev, tile);
     hipLaunchKernelGGL(n2t_colscan_kernel, dim3((B.nodes + 63) / 64, F), dim3(64 * kScanWaves), 0, stream, B.dev);
     hipLaunchKernelGGL(node_scan_kernel, dim3(1, F), dim3(1024), 0, stream, B.dev, 0);
-    hipLaunchKernelGGL(n2t_place_kernel, dim3(B.cap_chunks, F), dim3(64), tile * sizeof(uint32_t), stream, B.dev, tile,
-                       mode == kIndexLists ? 1 : 0);
+    const int with_nodes = mode == kIndexLists ? 1 : 0;
+    if ((size_t)B.cap_chunks * F <= kPlaceResident)
+        hipLaunchKernelGGL(n2t_place_kernel<true>, dim3(B.cap_chunks, F), dim3(64), tile * sizeof(uint32_t), stream,
+                           B.dev, tile, with_nodes);
+    else
+        hipLaunchKernelGGL(n2t_place_kernel<false>, dim3(B.cap_chunks, F), dim3(64), hist_tile_bytes(tile), stream,
+                           B.dev, tile, with_nodes);
     if (mode == kIndexNodes) return;
     const uint32_t row_grid = (B.nodes + kXcds - 1) / kXcds * kXcds;
     if (gram) {
