@@ -280,47 +280,63 @@ __global__ void __launch_bounds__(kTileThreads) dense_scatter_kernel(const Buffe
     __syncthreads();
     uint32_t running = b.tile_offset[blockIdx.x];
     const uint32_t cap = b.cap_pixels;
+    // the surfels' knn and weights of kGatherRounds rounds are gathered before
+    // any of their stores (the stores could alias them for the compiler)
+    constexpr int kGatherRounds = 4;
 #pragma unroll
-    for (int r = 0; r < kTileRounds; ++r) {
-        uint32_t before = 0, round_total = 0;
+    for (int r0 = 0; r0 < kTileRounds; r0 += kGatherRounds) {
+        ushort4 knn_g[kGatherRounds];
+        float4 w_g[kGatherRounds];
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) {
-            const uint32_t cw = wave_cnt[r][w];
-            before += (w < (int)wave) ? cw : 0u;
-            round_total += cw;
+        for (int g = 0; g < kGatherRounds; ++g) {
+            const uint32_t s = sv[r0 + g];
+            const bool ok = s != kInvalidIndex && s < f.num_surfels;
+            knn_g[g] = ok ? f.surfel_knn[s] : make_ushort4(0, 0, 0, 0);
+            w_g[g] = ok ? f.surfel_w[s] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        const uint32_t s = sv[r];
-        if (s != kInvalidIndex && s < f.num_surfels) {
-            const uint32_t dst = running + before + lane_prefix(mask[r]);
-            if (dst < cap) {
-                const uint32_t p = base + r * kTileThreads + threadIdx.x;
-                const uint32_t v = view_of(f, p, wh), o = p - v * wh;
-                const uint32_t y = o / f.W, x = o - y * f.W;
-                ushort4 knn = f.surfel_knn[s];
-                if (!knn_ok(f, knn)) {
-                    atomicOr(&b.counts->error, kErrNodes);
-                    knn = make_ushort4(0, 0, 0, 0);
-                }
-                const float4 w = f.surfel_w[s];
-                b.dense_px[dst] = make_ushort4((uint16_t)x, (uint16_t)y, (uint16_t)v, 0);
-                b.dense_knn[dst] = knn;
-                // the record's first half (knn, r = 0, weights); J is written by
-                // the term evaluation before any use
-                const uint4 h0 = make_uint4((uint32_t)knn.x | ((uint32_t)knn.y << 16),
-                                            (uint32_t)knn.z | ((uint32_t)knn.w << 16), 0u, 0u);
-                const uint4 h1 = make_uint4(__float_as_uint(w.x), __float_as_uint(w.y), __float_as_uint(w.z),
-                                            __float_as_uint(w.w));
-                uint4* d = reinterpret_cast<uint4*>(b.dense + dst);
-                d[0] = h0;
-                d[1] = h1;
-                if (b.density) {
-                    d = reinterpret_cast<uint4*>(b.density + dst);
+    #pragma unroll
+        for (int g = 0; g < kGatherRounds; ++g) {
+            const int r = r0 + g;
+            uint32_t before = 0, round_total = 0;
+    #pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                const uint32_t cw = wave_cnt[r][w];
+                before += (w < (int)wave) ? cw : 0u;
+                round_total += cw;
+            }
+            const uint32_t s = sv[r];
+            if (s != kInvalidIndex && s < f.num_surfels) {
+                const uint32_t dst = running + before + lane_prefix(mask[r]);
+                if (dst < cap) {
+                    const uint32_t p = base + r * kTileThreads + threadIdx.x;
+                    const uint32_t v = view_of(f, p, wh), o = p - v * wh;
+                    const uint32_t y = o / f.W, x = o - y * f.W;
+                    ushort4 knn = knn_g[g];
+                    if (!knn_ok(f, knn)) {
+                        atomicOr(&b.counts->error, kErrNodes);
+                        knn = make_ushort4(0, 0, 0, 0);
+                    }
+                    const float4 w = w_g[g];
+                    b.dense_px[dst] = make_ushort4((uint16_t)x, (uint16_t)y, (uint16_t)v, 0);
+                    b.dense_knn[dst] = knn;
+                    // the record's first half (knn, r = 0, weights); J is written by
+                    // the term evaluation before any use
+                    const uint4 h0 = make_uint4((uint32_t)knn.x | ((uint32_t)knn.y << 16),
+                                                (uint32_t)knn.z | ((uint32_t)knn.w << 16), 0u, 0u);
+                    const uint4 h1 = make_uint4(__float_as_uint(w.x), __float_as_uint(w.y), __float_as_uint(w.z),
+                                                __float_as_uint(w.w));
+                    uint4* d = reinterpret_cast<uint4*>(b.dense + dst);
                     d[0] = h0;
                     d[1] = h1;
+                    if (b.density) {
+                        d = reinterpret_cast<uint4*>(b.density + dst);
+                        d[0] = h0;
+                        d[1] = h1;
+                    }
                 }
             }
+            running += round_total;
         }
-        running += round_total;
     }
 }
 
