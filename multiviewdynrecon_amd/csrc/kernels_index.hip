@@ -1165,13 +1165,11 @@ __global__ void __launch_bounds__(256) pair_emit_kernel(const Buffers* __restric
 // blocks in the fix-up, in edge order.  The first kPairEdges edges found are
 // kept; the count tells the fix-up when a pair has more (a node graph with
 // repeated edges) and it walks the row's Node2Term list instead.
-__global__ void __launch_bounds__(256) pair_edges_kernel(const Buffers* __restrict__ bs) {
-    const Buffers b = frame_buffers(bs);
+__device__ void pair_edges(const Buffers& b, uint32_t block, uint32_t blocks) {
     const FrameDev& f = *b.frame;
     Counts& c = *b.counts;
-    if (c.error) return;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < f.E; e += stride) {
+    const uint32_t stride = blocks * blockDim.x;
+    for (uint32_t e = block * blockDim.x + threadIdx.x; e < f.E; e += stride) {
         const ushort2 g = f.graph[e];
         if (g.x == g.y) continue;
         const uint32_t i = min(g.x, g.y), j = max(g.x, g.y);
@@ -1196,13 +1194,19 @@ __global__ void __launch_bounds__(256) pair_edges_kernel(const Buffers* __restri
 // row k's ascending upper list (and, in the mixed mode, the inverse map).
 constexpr uint32_t kLinkLanes = 8;  // lanes per row of the pair link
 
-__global__ void __launch_bounds__(256) pair_link_kernel(const Buffers* __restrict__ bs) {
+// Workgroups [0, link_blocks) link the lower entries; the rest (row Gram
+// index only) collect the graph edges of each pair (pair_edges).
+__global__ void __launch_bounds__(256) pair_link_kernel(const Buffers* __restrict__ bs, uint32_t link_blocks) {
     const Buffers b = frame_buffers(bs);
     const Counts& c = *b.counts;
     if (c.error) return;
+    if (blockIdx.x >= link_blocks) {
+        pair_edges(b, blockIdx.x - link_blocks, gridDim.x - link_blocks);
+        return;
+    }
     const uint32_t N = b.frame->N;
     const uint32_t sub = threadIdx.x % kLinkLanes;
-    const uint32_t stride = gridDim.x * blockDim.x / kLinkLanes;
+    const uint32_t stride = link_blocks * blockDim.x / kLinkLanes;
     for (uint32_t j = (blockIdx.x * blockDim.x + threadIdx.x) / kLinkLanes; j < N; j += stride) {
         const uint32_t l1 = b.low_offset[j + 1];
         for (uint32_t l = b.low_offset[j] + sub; l < l1; l += kLinkLanes) {
@@ -1268,8 +1272,9 @@ void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, int
         hipLaunchKernelGGL(node_scan_kernel, dim3(3, F), dim3(1024), 0, stream, B.dev, 1);
         hipLaunchKernelGGL((pair_rows_kernel<kRowTerms, kRowWaves>), dim3(row_grid, F), dim3(64 * kRowWaves), 0, stream, B.dev);
     }
-    hipLaunchKernelGGL(pair_link_kernel, dim3((B.nodes * kLinkLanes + 255) / 256, F), dim3(256), 0, stream, B.dev);
-    if (gram) hipLaunchKernelGGL(pair_edges_kernel, dim3((8 * B.nodes + 255) / 256, F), dim3(256), 0, stream, B.dev);
+    const uint32_t link_blocks = (B.nodes * kLinkLanes + 255) / 256;
+    const uint32_t edge_blocks = gram ? (8 * B.nodes + 255) / 256 : 0u;
+    hipLaunchKernelGGL(pair_link_kernel, dim3(link_blocks + edge_blocks, F), dim3(256), 0, stream, B.dev, link_blocks);
 }
 
 // The NodePair2Term lists of an index built for the row Gram assembly (which
@@ -1278,7 +1283,8 @@ void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, int
 void launch_pair_term_lists(const Batch& B, hipStream_t stream) {
     const uint32_t row_grid = (B.nodes + kXcds - 1) / kXcds * kXcds;
     hipLaunchKernelGGL((pair_rows_kernel<kRowTerms, kRowWaves>), dim3(row_grid, B.frames), dim3(64 * kRowWaves), 0, stream, B.dev);
-    hipLaunchKernelGGL(pair_link_kernel, dim3((B.nodes * kLinkLanes + 255) / 256, B.frames), dim3(256), 0, stream, B.dev);
+    const uint32_t link_blocks = (B.nodes * kLinkLanes + 255) / 256;
+    hipLaunchKernelGGL(pair_link_kernel, dim3(link_blocks, B.frames), dim3(256), 0, stream, B.dev, link_blocks);
 }
 
 uint32_t index_chunks(uint64_t max_terms) { return (uint32_t)((max_terms + kChunkTerms - 1) / kChunkTerms); }
