@@ -86,28 +86,12 @@ struct ForegroundPred {
     }
 };
 
-// Pass 1: valid pixels per 4096-pixel tile.  Block 0 also clears the
-// counters the rest of the index build accumulates into.
+// Pass 1: valid pixels per 4096-pixel tile.
 template <typename Pred>
-__global__ void __launch_bounds__(kTileThreads) compact_count_kernel(const Buffers* __restrict__ bs, Pred pred, int clear) {
+__global__ void __launch_bounds__(kTileThreads) compact_count_kernel(const Buffers* __restrict__ bs, Pred pred) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const uint32_t M = f.V * f.W * f.H;
-    if (clear) {
-        const uint32_t stride = gridDim.x * blockDim.x;
-        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= b.cap_nodes; i += stride) {
-            b.n2t_count[i] = 0;
-            b.row_up[i] = 0;
-            b.row_low[i] = 0;
-            b.row_kv2[i] = 0;
-        }
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            Counts* c = b.counts;
-            c->dense = c->smooth = c->fg = c->feat = c->cross = 0;
-            c->kv1 = c->kv2 = c->pairs = c->max_seg = c->max_row = 0;
-            c->error = 0;
-        }
-    }
     const uint32_t base = blockIdx.x * kTilePixels;
     if (base >= M) return;
     uint32_t count = 0;
@@ -1245,7 +1229,7 @@ void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, int
     hipLaunchKernelGGL(dense_scatter_kernel, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream, B.dev);
     if (use_foreground) {
         hipLaunchKernelGGL(compact_count_kernel<ForegroundPred>, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream,
-                           B.dev, ForegroundPred{}, 0);
+                           B.dev, ForegroundPred{});
         hipLaunchKernelGGL(compact_scan_kernel, dim3(1, F), dim3(1024), 0, stream, B.dev, 1, -1);
         hipLaunchKernelGGL(compact_scatter_kernel<ForegroundPred>, dim3(B.cap_tiles, F), dim3(kTileThreads), 0, stream,
                            B.dev, ForegroundPred{}, 1);
