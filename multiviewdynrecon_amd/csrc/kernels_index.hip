@@ -545,12 +545,14 @@ __device__ inline uint32_t slot(ushort4 k, int q) { return q == 0 ? k.x : (q == 
 //   n2t_place   each chunk writes its entries at offset[n] + H[c][n] + rank,
 //               rank = position among the chunk's earlier entries of node n
 // No global atomics and no sorts: the result is deterministic by construction.
-// A chunk holds at most 4 kChunkTerms entries, so the per-chunk node counters
-// fit 16 bits: two share a 32-bit LDS word (half the LDS of u32 counters, twice
-// the resident workgroups).
-constexpr uint32_t kChunkTerms = 1024;
+// A chunk holds at most 4 kBigChunkTerms entries, so the per-chunk node
+// counters fit 16 bits: two share a 32-bit LDS word (half the LDS of u32
+// counters, twice the resident workgroups).
+constexpr uint32_t kChunkTerms = 1024;     // chunk of a launch the chip holds at once; the capacity unit
+constexpr uint32_t kBigChunkTerms = 2048;  // chunk of a larger launch (half the histogram rows)
 constexpr uint32_t kHistNodes = 16384;  // largest LDS node tile (32 KB of u16 counters)
-static_assert(4 * kChunkTerms <= 0xFFFFu, "per-chunk node counts must fit 16 bits");
+static_assert(4 * kBigChunkTerms <= 0xFFFFu && kBigChunkTerms % kChunkTerms == 0,
+              "per-chunk node counts must fit 16 bits");
 
 // LDS node tile of the counting-sort kernels (dynamic shared memory).
 uint32_t hist_tile(uint32_t nodes) { return std::min<uint32_t>(nodes, kHistNodes); }
@@ -559,9 +561,9 @@ size_t hist_tile_bytes(uint32_t tile) { return (size_t)(tile + 1) / 2 * sizeof(u
 // keeps u32 counters: about what the chip holds at once with 16 KB tiles.
 constexpr size_t kPlaceResident = 2048;
 
-__device__ inline uint32_t num_chunks(const Counts& c) { return (c.off[4] + kChunkTerms - 1) / kChunkTerms; }
+__device__ inline uint32_t num_chunks(const Counts& c, uint32_t chunk) { return (c.off[4] + chunk - 1) / chunk; }
 
-__global__ void __launch_bounds__(64) n2t_hist_kernel(const Buffers* __restrict__ bs, uint32_t tile) {
+__global__ void __launch_bounds__(64) n2t_hist_kernel(const Buffers* __restrict__ bs, uint32_t tile, uint32_t chunk) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     Counts& c = *b.counts;
@@ -571,9 +573,9 @@ __global__ void __launch_bounds__(64) n2t_hist_kernel(const Buffers* __restrict_
         c.smooth = f.E;
         for (int k = 0; k < 5; ++k) c.off[k] = off[k];
     }
-    const uint32_t t0 = blockIdx.x * kChunkTerms;
+    const uint32_t t0 = blockIdx.x * chunk;
     if (t0 >= off[4]) return;
-    const uint32_t t1 = min(off[4], t0 + kChunkTerms);
+    const uint32_t t1 = min(off[4], t0 + chunk);
     const uint32_t N = f.N;
     extern __shared__ uint32_t hist[];
     uint32_t* H = b.n2t_hist + (size_t)blockIdx.x * N;
@@ -615,12 +617,12 @@ __global__ void __launch_bounds__(64) n2t_hist_kernel(const Buffers* __restrict_
 // number of node-n entries in chunks before c; n2t_count[n] the node's total.
 constexpr uint32_t kScanWaves = 16;
 
-__global__ void __launch_bounds__(64 * kScanWaves) n2t_colscan_kernel(const Buffers* __restrict__ bs) {
+__global__ void __launch_bounds__(64 * kScanWaves) n2t_colscan_kernel(const Buffers* __restrict__ bs, uint32_t chunk) {
     const Buffers b = frame_buffers(bs);
     const uint32_t N = b.frame->N;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t n = blockIdx.x * 64 + lane;
-    const uint32_t C = num_chunks(*b.counts);
+    const uint32_t C = num_chunks(*b.counts, chunk);
     const uint32_t per = (C + kScanWaves - 1) / kScanWaves;
     const uint32_t c0 = min(C, w * per), c1 = min(C, c0 + per);
     __shared__ uint32_t part[kScanWaves][64];
@@ -685,14 +687,14 @@ __device__ inline void wave_sort256(uint32_t k[4]) {
 // that base while its round sorts (half the LDS: twice the resident
 // workgroups of a large grid).  Both place every entry at the same position.
 template <bool kLdsBase>
-__global__ void __launch_bounds__(64) n2t_place_kernel(const Buffers* __restrict__ bs, uint32_t tile, int with_nodes) {
+__global__ void __launch_bounds__(64) n2t_place_kernel(const Buffers* __restrict__ bs, uint32_t tile, int with_nodes, uint32_t chunk) {
     const Buffers b = frame_buffers(bs);
     const FrameDev& f = *b.frame;
     const Counts& c = *b.counts;
     if (c.error) return;
-    const uint32_t t0 = blockIdx.x * kChunkTerms;
+    const uint32_t t0 = blockIdx.x * chunk;
     if (t0 >= c.off[4]) return;
-    const uint32_t t1 = min(c.off[4], t0 + kChunkTerms);
+    const uint32_t t1 = min(c.off[4], t0 + chunk);
     const uint32_t N = f.N;
     const uint32_t lane = threadIdx.x;
     extern __shared__ uint32_t cur_words[];
@@ -1235,16 +1237,22 @@ void launch_index_build(const Batch& B, bool use_foreground, bool use_cross, int
                            B.dev, ForegroundPred{}, 1);
     }
     const uint32_t tile = hist_tile(B.nodes);
-    hipLaunchKernelGGL(n2t_hist_kernel, dim3(B.cap_chunks, F), dim3(64), hist_tile_bytes(tile), stream, B.dev, tile);
-    hipLaunchKernelGGL(n2t_colscan_kernel, dim3((B.nodes + 63) / 64, F), dim3(64 * kScanWaves), 0, stream, B.dev);
+    // the capacity counts kChunkTerms chunks; a launch larger than the chip
+    // holds at once runs kBigChunkTerms chunks (fewer histogram rows to scan)
+    const bool resident = (size_t)B.cap_chunks * F <= kPlaceResident;
+    const uint32_t chunk = resident ? kChunkTerms : kBigChunkTerms;
+    const uint32_t chunks = (uint32_t)(((uint64_t)B.cap_chunks * kChunkTerms + chunk - 1) / chunk);
+    hipLaunchKernelGGL(n2t_hist_kernel, dim3(chunks, F), dim3(64), hist_tile_bytes(tile), stream, B.dev, tile, chunk);
+    hipLaunchKernelGGL(n2t_colscan_kernel, dim3((B.nodes + 63) / 64, F), dim3(64 * kScanWaves), 0, stream, B.dev,
+                       chunk);
     hipLaunchKernelGGL(node_scan_kernel, dim3(1, F), dim3(1024), 0, stream, B.dev, 0);
     const int with_nodes = mode == kIndexLists ? 1 : 0;
-    if ((size_t)B.cap_chunks * F <= kPlaceResident)
-        hipLaunchKernelGGL(n2t_place_kernel<true>, dim3(B.cap_chunks, F), dim3(64), tile * sizeof(uint32_t), stream,
-                           B.dev, tile, with_nodes);
+    if (resident)
+        hipLaunchKernelGGL(n2t_place_kernel<true>, dim3(chunks, F), dim3(64), tile * sizeof(uint32_t), stream, B.dev,
+                           tile, with_nodes, chunk);
     else
-        hipLaunchKernelGGL(n2t_place_kernel<false>, dim3(B.cap_chunks, F), dim3(64), hist_tile_bytes(tile), stream,
-                           B.dev, tile, with_nodes);
+        hipLaunchKernelGGL(n2t_place_kernel<false>, dim3(chunks, F), dim3(64), hist_tile_bytes(tile), stream, B.dev,
+                           tile, with_nodes, chunk);
     if (mode == kIndexNodes) return;
     const uint32_t row_grid = (B.nodes + kXcds - 1) / kXcds * kXcds;
     if (gram) {
