@@ -537,8 +537,8 @@ __device__ inline uint32_t slot(ushort4 k, int q) { return q == 0 ? k.x : (q == 
 // The reference builds Node2Term by a stable key sort of (node, term) pairs
 // generated in term order (Node2TermsIndex.cu:112-166): each node's list holds
 // its terms ascending, a term once per knn slot naming the node.  Here the same
-// lists come from a stable counting sort over chunks of kChunkTerms terms, one
-// wave per chunk:
+// lists come from a stable counting sort over chunks of kChunkTerms (or
+// kBigChunkTerms) terms, one wave per chunk:
 //   n2t_hist    per-chunk node histograms H[c][n] (LDS counters)
 //   n2t_colscan H[c][n] -> exclusive prefix over chunks, per-node totals
 //   node_scan   per-node totals -> list offsets
