@@ -798,8 +798,10 @@ gram_rows_kernel(const Buffers* __restrict__ bs, Penalty pen, int diag_only) {
     // The grid covers the batch's largest node count in one sweep, so this
     // wave's rows are known before its frame's sizes arrive: their offsets
     // load alongside them, lane k holding row r0 + k's (clamped to the
-    // capacity; rows past the frame's node count are not used).
-    const uint32_t r0 = xcd_remap(blockIdx.x, gridDim.x, gridDim.x) * kGramRows;
+    // capacity; rows past the frame's node count are not used).  Rows follow
+    // the launch order, consecutive rows on consecutive XCDs (measured faster
+    // here than a contiguous range of rows per XCD).
+    const uint32_t r0 = blockIdx.x * kGramRows;
     const uint32_t ol = min(r0 + lane, b.cap_nodes);
     const uint32_t off_l = b.n2t_offset[ol];
     const uint32_t poff_l = diag_only ? 0u : b.pair_offset[ol];
