@@ -548,7 +548,7 @@ __device__ inline uint32_t slot(ushort4 k, int q) { return q == 0 ? k.x : (q == 
 // A chunk holds at most 4 kBigChunkTerms entries, so the per-chunk node
 // counters fit 16 bits: two share a 32-bit LDS word (half the LDS of u32
 // counters, twice the resident workgroups).
-constexpr uint32_t kChunkTerms = 1024;     // chunk of a launch the chip holds at once; the capacity unit
+constexpr uint32_t kChunkTerms = 512;      // chunk of a launch the chip holds at once; the capacity unit
 constexpr uint32_t kBigChunkTerms = 2048;  // chunk of a larger launch (half the histogram rows)
 constexpr uint32_t kHistNodes = 16384;  // largest LDS node tile (32 KB of u16 counters)
 static_assert(4 * kBigChunkTerms <= 0xFFFFu && kBigChunkTerms % kChunkTerms == 0,
@@ -558,8 +558,8 @@ static_assert(4 * kBigChunkTerms <= 0xFFFFu && kBigChunkTerms % kChunkTerms == 0
 uint32_t hist_tile(uint32_t nodes) { return std::min<uint32_t>(nodes, kHistNodes); }
 size_t hist_tile_bytes(uint32_t tile) { return (size_t)(tile + 1) / 2 * sizeof(uint32_t); }
 // Chunk workgroups (over all frames of a launch) up to which the placement
-// keeps u32 counters: about what the chip holds at once with 16 KB tiles.
-constexpr size_t kPlaceResident = 2048;
+// keeps u32 counters: what the chip holds at once with 16 KB tiles (9 per CU).
+constexpr size_t kPlaceResident = 2304;
 
 __device__ inline uint32_t num_chunks(const Counts& c, uint32_t chunk) { return (c.off[4] + chunk - 1) / chunk; }
 
