@@ -795,19 +795,17 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 gram_rows_kernel(const Buffers* __restrict__ bs, Penalty pen, int diag_only) {
     const Buffers b = frame_buffers(bs);
     const uint32_t lane = threadIdx.x;
-    // The grid covers the batch's largest node count in one sweep, so this
-    // wave's rows are known before its frame's sizes arrive: their offsets
-    // load alongside them, lane k holding row r0 + k's (clamped to the
-    // capacity; rows past the frame's node count are not used).  Rows follow
-    // the launch order, consecutive rows on consecutive XCDs (measured faster
-    // here than a contiguous range of rows per XCD).
-    const uint32_t r0 = blockIdx.x * kGramRows;
+    // XCD-aware rows: each XCD takes a contiguous range of the frame's rows
+    // (whose entries share term records) into its L2, spread over all XCDs
+    // whatever the frame's node count; lane k loads row r0 + k's offsets
+    // (clamped to the capacity; rows past the frame's node count are not used).
+    const FrameDev& f = *b.frame;
+    const Counts& c = *b.counts;
+    const uint32_t r0 = xcd_remap(blockIdx.x, gridDim.x, (f.N + kGramRows - 1) / kGramRows) * kGramRows;
+    if (c.error || r0 >= f.N) return;
     const uint32_t ol = min(r0 + lane, b.cap_nodes);
     const uint32_t off_l = b.n2t_offset[ol];
     const uint32_t poff_l = diag_only ? 0u : b.pair_offset[ol];
-    const FrameDev& f = *b.frame;
-    const Counts& c = *b.counts;
-    if (c.error || r0 >= f.N) return;
     const uint32_t nrows = min(kGramRows, f.N - r0);
     const float4 g_l = f.nodes[min(r0 + lane, f.N - 1)];
     __shared__ float Gs[64 * kRowWords];
